@@ -1,0 +1,214 @@
+"""Benchmark of the hip Poisson solver on one MI355X.
+
+Default workload (``grid4096``): GridSolver, 4096 x 4096 x 3 uint8 blob
+composites, centred square mask of side 3800 (14 440 000 solved pixels in a
+3802 x 3802 crop), gradient ``src``, float32 state, 5000 Jacobi sweeps per
+step (the g+U+L+R+D order of the openmp GridSolver).  A step is one
+``GridSolver.step(5000)``: 5000 sweeps, then the uint8 crop and the residual
+sums copied back.  Inputs are resident in HBM (``reset`` is outside the
+timed region).
+
+Printed: one JSON line with the metric (solved-pixel updates per second),
+the roofline of the sweep kernel (algorithmic bytes per launch over its
+event-timed average launch duration, against the 8 TB/s HBM peak) and the
+reference's CPU path timed on this host (its openmp GridSolver compiled from
+the reference sources, or the oracle's restatement when that is absent) on a
+bounded sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "fast-poisson-image-editing_amd")
+sys.path.insert(0, PKG)
+
+HBM_PEAK_GBPS = 8000.0
+BYTES_PER_UPDATE = 36  # read x (12 B) + read grad (12 B) + write x (12 B)
+
+
+def build_inputs(name: str):
+  from fpie_amd import process, synth
+
+  class Grab:
+    def reset(self, *a):
+      self.a = a
+
+    def partition(self, mask):
+      flat = mask > 0
+      return (np.cumsum(flat.reshape(-1)).reshape(mask.shape) *
+              flat).astype(np.int32)
+
+  w = synth.WORKLOADS[name]()
+  process.register_backend("grab", lambda **kw: Grab(), lambda **kw: Grab())
+  cls = process.EquProcessor if w.method == "equ" else process.GridProcessor
+  p = cls(w.gradient, "grab")
+  p.reset(w.src, w.mask, w.tgt, w.mask_on_src, w.mask_on_tgt)
+  return w, p.core.a
+
+
+def traffic_from_profiles(workload: str, schedule: str):
+  """HBM bytes per launch of the sweep kernel measured with rocprofv3 PMC
+  counters (profiles/traffic_<workload>_<schedule>.json), if recorded."""
+  path = os.path.join(ROOT, "profiles",
+                      f"traffic_{workload}_{schedule}.json")
+  if not os.path.exists(path):
+    return None
+  with open(path) as f:
+    return json.load(f)
+
+
+def cpu_baseline(w, budget_s: float):
+  """The reference's openmp solver core (compiled from the reference sources
+  into oracle/_ref) driven by the processor on the same composite, timed on
+  a number of sweeps sized to about budget_s seconds; the oracle's port of
+  the same loops when oracle/_ref is absent."""
+  sys.path.insert(0, os.path.join(ROOT, "oracle"))
+  import oracle as O
+  from fpie_amd import process
+
+  threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (
+      os.cpu_count() or 1)
+  threads = min(threads, 16)
+  ref = O.ref_core("core_openmp")
+  if ref is not None:
+    # openmp flags of docs/benchmark.md (GridSolver --grid-x 2 --grid-y 16)
+    process.register_backend("cpu-baseline",
+                             lambda **kw: ref.EquSolver(threads),
+                             lambda **kw: ref.GridSolver(2, 16, threads))
+    kind, cores = "reference", threads
+    what = "fpie/core/openmp compiled from the reference sources"
+  else:
+    process.register_backend("cpu-baseline", lambda **kw: O.EquCore("openmp"),
+                             lambda **kw: O.GridCore(2, 16, "openmp"))
+    kind, cores = "port", 1
+    what = "oracle restatement of fpie/core/openmp (1 thread)"
+  cls = process.EquProcessor if w.method == "equ" else process.GridProcessor
+  p = cls(w.gradient, "cpu-baseline")
+  p.reset(w.src, w.mask, w.tgt, w.mask_on_src, w.mask_on_tgt)
+  masked = int(process.prepare_mask(w.mask)[0].sum())
+  p.core.step(1)  # warm-up
+  t0 = time.perf_counter()
+  p.core.step(2)
+  dt = (time.perf_counter() - t0) / 2
+  k = max(2, int(budget_s / max(dt, 1e-6)))
+  t0 = time.perf_counter()
+  p.core.step(k)
+  dt = time.perf_counter() - t0
+  return {"value": masked * k / dt, "unit": "pixel-updates/s",
+          "cores": cores, "kind": kind,
+          "sample": f"{what}: {k} sweeps of the {w.name} problem "
+                    f"({masked} solved pixels), {dt:.2f} s, "
+                    f"{cores} thread(s)"}
+
+
+def main():
+  ap = argparse.ArgumentParser()
+  ap.add_argument("--gpus", type=int, default=1)
+  ap.add_argument("--steps", type=int, default=3)
+  ap.add_argument("--warmup", type=int, default=1)
+  ap.add_argument("--workload", default="grid4096",
+                  choices=["grid4096", "equ512", "grid8192"])
+  ap.add_argument("--schedule", default="jacobi",
+                  choices=["jacobi", "gs_lex", "redblack"])
+  ap.add_argument("--iters", type=int, default=0,
+                  help="sweeps per step (default: the workload's)")
+  ap.add_argument("--tile", type=str, default="",
+                  help="S,C,rows to skip the autotuner")
+  ap.add_argument("--cpu-budget", type=float, default=12.0)
+  ap.add_argument("--no-cpu-baseline", action="store_true")
+  a = ap.parse_args()
+  if a.gpus != 1:
+    print(f"bench.py measures one GPU: this build has no multi-GPU path "
+          f"(--gpus {a.gpus} requested)", file=sys.stderr)
+    sys.exit(2)
+
+  from fpie_amd import core_hip, native
+
+  print(f"# {native.device_info()}", file=sys.stderr, flush=True)
+  t_in = time.perf_counter()
+  w, args = build_inputs(a.workload)
+  iters = a.iters or w.iters
+  print(f"# inputs {w.name} built in {time.perf_counter() - t_in:.1f} s",
+        file=sys.stderr, flush=True)
+  if w.method == "grid":
+    _, m, t, g = args
+    core = core_hip.GridSolver(schedule=a.schedule, order="ulrd")
+    core.reset(0, m, t, g)
+  else:
+    N, A, X, B = args
+    core = core_hip.EquSolver(schedule=a.schedule)
+    from fpie_amd.process import prepare_mask
+    pm, *_ = prepare_mask(w.mask)
+    core.partition(pm)
+    core.reset(N, A, X, B)
+  masked = core.shape()["masked"]
+  if a.tile:
+    core.set_tile(*[int(v) for v in a.tile.split(",")])
+  else:
+    core.autotune(3)
+  tile = core.tile()
+  print(f"# tile S={tile[0]} C={tile[1]} rows={tile[2]}", file=sys.stderr,
+        flush=True)
+
+  for _ in range(a.warmup):
+    core.step(iters)
+  core.sync()
+  core.set_timing(True)
+  t0 = time.perf_counter()
+  for _ in range(a.steps):
+    core.step(iters)
+  core.sync()
+  dt = time.perf_counter() - t0
+  tm = core.timing()
+  updates = masked * iters * a.steps
+  value = updates / dt
+  launch_ms = tm["kernel_ms"] / max(tm["launches"], 1)
+  sweeps_per_launch = tm["pixel_updates"] / max(tm["launches"], 1) / masked
+  alg_bytes = BYTES_PER_UPDATE * masked * sweeps_per_launch
+  achieved = alg_bytes / (launch_ms * 1e-3) / 1e9
+  traffic = traffic_from_profiles(w.name, a.schedule)
+  line = {
+      "metric": "Jacobi pixel-updates/s + % HBM peak",
+      "value": value,
+      "unit": "pixel-updates/s",
+      "n_gpus": 1,
+      "steps": a.steps,
+      "warmup": a.warmup,
+      "ms_per_step": dt * 1e3 / a.steps,
+      "higher_is_better": True,
+      "scaling": "weak",
+      "vs_baseline": None,
+      "dtype": "f32",
+      "data": "synthetic (seeded blob composites, fpie_amd.synth)",
+      "config": {"workload": w.name, "method": w.method,
+                 "image": list(w.src.shape), "solved_pixels": masked,
+                 "gradient": w.gradient, "sweeps_per_step": iters,
+                 "schedule": a.schedule, "order": "ulrd",
+                 "tile": {"sweeps_per_launch": tile[0],
+                          "cols_per_lane": tile[1], "rows_per_task": tile[2]},
+                 "parallelism": "single GPU"},
+      "roofline": {"bound": "hbm", "achieved": achieved,
+                   "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                   "frac": achieved / HBM_PEAK_GBPS,
+                   "traffic": traffic["bytes_per_launch"] if traffic else None,
+                   "kernel": "march_kernel (temporally blocked sweep)",
+                   "bytes_per_update": BYTES_PER_UPDATE,
+                   "sweeps_per_launch": sweeps_per_launch,
+                   "avg_launch_ms": launch_ms,
+                   "launches_timed": tm["launches"]},
+  }
+  if not a.no_cpu_baseline:
+    line["cpu_baseline"] = cpu_baseline(w, a.cpu_budget)
+  print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+  main()

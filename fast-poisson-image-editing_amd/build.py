@@ -1,0 +1,72 @@
+"""Build recipe of the gfx950 solver library ``fpie_amd/libfpie_hip.so``.
+
+hipcc only (no cmake): every translation unit under ``csrc/`` is compiled for
+gfx950 in parallel, the sweep kernel once per compiled tile variant, then
+linked into one shared library that ``fpie_amd.native`` loads with ctypes.
+"""
+
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import re
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "fpie_amd", "libfpie_hip.so")
+OBJ = os.path.join(HERE, "_build")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    # every add and multiply rounds on its own (bit-exact to the CPU loops)
+    "-ffp-contract=off", "-fno-gpu-flush-denormals-to-zero",
+    # no packed FP32 instructions (v_pk_*_f32) in the device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+
+
+def tiles() -> list[tuple[int, int]]:
+  text = open(os.path.join(CSRC, "kernels.h")).read()
+  line = re.search(r"#define PIE_TILES\(X\)(.*)", text).group(1)
+  return [(int(a), int(b)) for a, b in re.findall(r"X\((\d+), (\d+)\)", line)]
+
+
+def _compile(src: str, obj: str, defines: list[str]) -> str:
+  cmd = [HIPCC, *FLAGS, *defines, "-c", src, "-o", obj]
+  r = subprocess.run(cmd, capture_output=True, text=True)
+  if r.returncode != 0:
+    raise RuntimeError(f"hipcc failed on {os.path.basename(src)}:\n"
+                       f"{' '.join(cmd)}\n{r.stdout}{r.stderr}")
+  return obj
+
+
+def build(jobs: int | None = None, verbose: bool = True) -> str:
+  os.makedirs(OBJ, exist_ok=True)
+  units = [(os.path.join(CSRC, "kernels.hip"),
+            os.path.join(OBJ, "kernels.o"), []),
+           (os.path.join(CSRC, "solver.hip"),
+            os.path.join(OBJ, "solver.o"), [])]
+  for s, c in tiles():
+    units.append((os.path.join(CSRC, "march_inst.hip"),
+                  os.path.join(OBJ, f"march_{s}x{c}.o"),
+                  [f"-DPIE_S={s}", f"-DPIE_C={c}"]))
+  jobs = jobs or min(len(units), max(1, (os.cpu_count() or 2)))
+  with cf.ThreadPoolExecutor(jobs) as ex:
+    objs = list(ex.map(lambda u: _compile(*u), units))
+  tmp = OUT + ".tmp"
+  cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
+  r = subprocess.run(cmd, capture_output=True, text=True)
+  if r.returncode != 0:
+    raise RuntimeError(f"link failed:\n{r.stdout}{r.stderr}")
+  os.replace(tmp, OUT)
+  if verbose:
+    print(f"built {OUT}")
+  return OUT
+
+
+if __name__ == "__main__":
+  build(int(sys.argv[1]) if len(sys.argv) > 1 else None)

@@ -1,0 +1,105 @@
+// Device kernels of the Poisson solver and their host-side launchers.
+//
+// Layout in HBM: a solve works on a pitched grid of H rows x W columns
+// (pitch P elements, P a multiple of 64).  The float32 RGB state is kept as
+// three planes (channel c at offset c * plane), the guidance (grad for the
+// GridSolver, b for the EquSolver) likewise, and the mask as one uint8
+// plane.  Planar float rows make every row load of a wavefront one
+// contiguous 256-byte (or 512-byte) transaction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pie {
+
+enum Sched { SCHED_JACOBI = 0, SCHED_GS_LEX = 1, SCHED_REDBLACK = 2 };
+enum Order { ORDER_UDLR = 0, ORDER_ULRD = 1 };
+
+// One launch of the temporally blocked sweep kernel: levels 1..S of the
+// launch are global steps t0 .. t0+S-1; steps >= t_end leave the state as
+// it is.  For SCHED_GS_LEX a pixel (i, j) is updated at step t when
+// d = t - i - j is even and 0 <= d < band (band = 2 * sweeps of the call).
+struct MarchArgs {
+  const float* xin;
+  float* xout;
+  const float* g;
+  const uint8_t* m;
+  long long plane;  // elements per plane (H * P)
+  int H, W, P;
+  int t0, t_end, band;
+  int stripes, segs, seg_rows, ntask;
+};
+
+// Temporal-blocking tile: S sweeps per launch, C columns per lane
+// (a wavefront covers 64*C columns, 64*C - 2S of them written), seg_rows
+// rows written per wave task.
+struct TileCfg {
+  int S, C, seg_rows;
+};
+
+// Compiled (sweeps, columns per lane) variants of the sweep kernel; each is
+// built from march_inst.hip with -DPIE_S=s -DPIE_C=c.
+#define PIE_TILES(X) X(4, 1) X(8, 1) X(4, 2) X(8, 2)
+#define PIE_DECL(s, c)                                             \
+  hipError_t launch_march_##s##_##c(int sched, int order, const MarchArgs& a, \
+                                    hipStream_t st);
+PIE_TILES(PIE_DECL)
+#undef PIE_DECL
+
+bool tile_supported(int S, int C);
+// The compiled (S, C) pairs, for the autotuner.
+int tile_variants(int (*out)[2], int max);
+// Wave tasks (stripes x segments) of a tile on an H x W grid.
+void tile_tasks(const TileCfg& cfg, int H, int W, int* stripes, int* segs);
+
+hipError_t launch_march(const TileCfg& cfg, int sched, int order,
+                        MarchArgs a, hipStream_t st);
+
+// float[H][W][3] (interleaved) -> three pitched planes
+hipError_t launch_deinterleave(const float* src, float* dst, long long plane,
+                               int H, int W, int P, hipStream_t st);
+// three pitched planes -> float[H][W][3]
+hipError_t launch_interleave(const float* src, float* dst, long long plane,
+                             int H, int W, int P, hipStream_t st);
+// int32 mask[H][W] -> uint8 plane (0/1); counts[0] += solved pixels,
+// counts[1] += solved pixels on the outermost rows/columns.
+hipError_t launch_mask(const int32_t* src, uint8_t* dst, int H, int W, int P,
+                       unsigned long long* counts, hipStream_t st);
+// fill three planes (H x P each) with a per-channel constant
+hipError_t launch_fill3(float* dst, long long plane, float v0, float v1,
+                        float v2, hipStream_t st);
+// Equ system on the grid layout: X[id], B[id] (float[N][3]) to the pixel
+// lin[id] = i * P + j of each id >= 1.
+hipError_t launch_equ_scatter(const float* X, const float* B,
+                              const int32_t* lin, int N, float* x, float* g,
+                              long long plane, hipStream_t st);
+// uint8[N][3] (out_u8) or float[N][3] (out_f32) in id order; row 0 = x0.
+hipError_t launch_equ_gather(const float* x, const int32_t* lin, int N,
+                             long long plane, float x00, float x01, float x02,
+                             uint8_t* out_u8, float* out_f32,
+                             hipStream_t st);
+// Residual sum over the solved pixels into partial[nblocks][3] (double) and,
+// if img != nullptr, the clamped/truncated image uint8[H][W][3].
+int finalize_blocks(int H, int W);
+hipError_t launch_finalize(int order, const float* x, const float* g,
+                           const uint8_t* m, long long plane, int H, int W,
+                           int P, uint8_t* img, double* partial, int nblocks,
+                           hipStream_t st);
+// Fixed-order sum of partial[n][3] -> err[3] (float).
+hipError_t launch_reduce(const double* partial, int n, float* err,
+                         hipStream_t st);
+
+// ---- EquSolver on an arbitrary A (no grid geometry) -------------------
+hipError_t launch_equ_pack(const float* X, float4* out, int N,
+                           hipStream_t st);
+hipError_t launch_equ_jacobi(const int4* A, const float4* B,
+                             const float4* xin, float4* xout, int N,
+                             hipStream_t st);
+int equ_generic_blocks(int N);
+hipError_t launch_equ_finalize(const int4* A, const float4* B,
+                               const float4* x, int N, uint8_t* img,
+                               float* state, double* partial, int nblocks,
+                               hipStream_t st);
+
+}  // namespace pie

@@ -1,0 +1,414 @@
+// gfx950 kernels of the Poisson solver other than the sweep kernel: layout
+// conversion, the residual/output pass, the EquSolver gather path, and the
+// dispatch of the sweep kernel over the compiled tile variants.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "march.h"
+
+namespace pie {
+
+namespace {
+
+using namespace detail;
+
+// ------------------------------------------------------------ layout ----
+
+__global__ void __launch_bounds__(kBlock)
+    deinterleave_kernel(const float* __restrict__ src, float* __restrict__ dst,
+                        long long plane, int H, int W, int P) {
+  const long long n = (long long)H * W;
+  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < n;
+       id += (long long)gridDim.x * kBlock) {
+    const int i = (int)(id / W), j = (int)(id % W);
+    const long long o = (long long)i * P + j;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dst[c * plane + o] = src[id * 3 + c];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    interleave_kernel(const float* __restrict__ src, float* __restrict__ dst,
+                      long long plane, int H, int W, int P) {
+  const long long n = (long long)H * W;
+  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < n;
+       id += (long long)gridDim.x * kBlock) {
+    const int i = (int)(id / W), j = (int)(id % W);
+    const long long o = (long long)i * P + j;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dst[id * 3 + c] = src[c * plane + o];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    mask_kernel(const int32_t* __restrict__ src, uint8_t* __restrict__ dst,
+                int H, int W, int P, unsigned long long* counts) {
+  const long long n = (long long)H * W;
+  unsigned long long solved = 0, border = 0;
+  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < n;
+       id += (long long)gridDim.x * kBlock) {
+    const int i = (int)(id / W), j = (int)(id % W);
+    const uint8_t v = src[id] != 0;
+    dst[(long long)i * P + j] = v;
+    solved += v;
+    if (v && (i == 0 || j == 0 || i == H - 1 || j == W - 1)) ++border;
+  }
+  // wave sums through shuffles, then one atomic per wave
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    solved += __shfl_down(solved, off, kWave);
+    border += __shfl_down(border, off, kWave);
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    if (solved) atomicAdd(&counts[0], solved);
+    if (border) atomicAdd(&counts[1], border);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    fill3_kernel(float* __restrict__ dst, long long plane, float v0, float v1,
+                 float v2) {
+  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < plane;
+       id += (long long)gridDim.x * kBlock) {
+    dst[id] = v0;
+    dst[plane + id] = v1;
+    dst[2 * plane + id] = v2;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    equ_scatter_kernel(const float* __restrict__ X, const float* __restrict__ B,
+                       const int32_t* __restrict__ lin, int N,
+                       float* __restrict__ x, float* __restrict__ g,
+                       long long plane) {
+  for (int id = 1 + blockIdx.x * kBlock + threadIdx.x; id < N;
+       id += gridDim.x * kBlock) {
+    const long long o = lin[id];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      x[c * plane + o] = X[(long long)id * 3 + c];
+      g[c * plane + o] = B[(long long)id * 3 + c];
+    }
+  }
+}
+
+__device__ __forceinline__ uint8_t clamp_u8(float v) {
+  // fpie/core/gcc/equ.cc:82-84: clamp to [0, 255], then truncate
+  return (uint8_t)(v < 0.0f ? 0.0f : v > 255.0f ? 255.0f : v);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    equ_gather_kernel(const float* __restrict__ x,
+                      const int32_t* __restrict__ lin, int N, long long plane,
+                      float x00, float x01, float x02, uint8_t* out_u8,
+                      float* out_f32) {
+  for (int id = blockIdx.x * kBlock + threadIdx.x; id < N;
+       id += gridDim.x * kBlock) {
+    float v[3];
+    if (id == 0) {
+      v[0] = x00;
+      v[1] = x01;
+      v[2] = x02;
+    } else {
+      const long long o = lin[id];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[c] = x[c * plane + o];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (out_u8) out_u8[(long long)id * 3 + c] = clamp_u8(v[c]);
+      if (out_f32) out_f32[(long long)id * 3 + c] = v[c];
+    }
+  }
+}
+
+// Block-wide fixed-order sum of three doubles; thread 0 gets the totals.
+__device__ __forceinline__ void block_sum3(double (&v)[3]) {
+  __shared__ double red[3][kBlock];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) red[c][threadIdx.x] = v[c];
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        red[c][threadIdx.x] = red[c][threadIdx.x] + red[c][threadIdx.x + s];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) v[c] = red[c][0];
+}
+
+// Residual |(b + n1 + n2 + n3 + n4) - 4x| of the reference's calc_error
+// (fpie/core/gcc/equ.cc:56-72, gcc/grid.cc:38-55): the neighbour sum in
+// float in the solver's order, the subtraction and absolute value in
+// double; the terms are summed in double in a fixed order.
+template <int ORDER>
+__global__ void __launch_bounds__(kBlock)
+    finalize_kernel(const float* __restrict__ x, const float* __restrict__ g,
+                    const uint8_t* __restrict__ m, long long plane, int H,
+                    int W, int P, uint8_t* __restrict__ img,
+                    double* __restrict__ partial) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  const long long n = (long long)H * W;
+  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < n;
+       id += (long long)gridDim.x * kBlock) {
+    const int i = (int)(id / W), j = (int)(id % W);
+    const long long o = (long long)i * P + j;
+    if (m[o]) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float* xc = x + c * plane;
+        float s = g[c * plane + o];
+        if (ORDER == ORDER_UDLR) {
+          s = s + xc[o - P];
+          s = s + xc[o + P];
+          s = s + xc[o - 1];
+          s = s + xc[o + 1];
+        } else {
+          s = s + xc[o - P];
+          s = s + xc[o - 1];
+          s = s + xc[o + 1];
+          s = s + xc[o + P];
+        }
+        acc[c] += fabs((double)s - (double)xc[o] * 4.0);
+      }
+    }
+    if (img) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) img[id * 3 + c] = clamp_u8(x[c * plane + o]);
+    }
+  }
+  block_sum3(acc);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) partial[blockIdx.x * 3 + c] = acc[c];
+}
+
+__global__ void __launch_bounds__(kBlock)
+    reduce_kernel(const double* __restrict__ partial, int n, float* err) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < n; i += kBlock)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[c] += partial[i * 3 + c];
+  block_sum3(acc);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) err[c] = (float)acc[c];
+}
+
+// ------------------------------------------- EquSolver on arbitrary A ---
+
+__global__ void __launch_bounds__(kBlock)
+    equ_pack_kernel(const float* __restrict__ X, float4* __restrict__ out,
+                    int N) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < N;
+       i += gridDim.x * kBlock)
+    out[i] = make_float4(X[3LL * i], X[3LL * i + 1], X[3LL * i + 2], 0.0f);
+}
+
+// Jacobi on the id system: x'[i] = (B[i] + x[A0] + x[A1] + x[A2] + x[A3])
+// / 4 for ids 1..N-1 (the order of fpie/core/gcc/equ.cc:48-53 and
+// np_solver.py:33-41); id 0 keeps its value.
+__global__ void __launch_bounds__(kBlock)
+    equ_jacobi_kernel(const int4* __restrict__ A, const float4* __restrict__ B,
+                      const float4* __restrict__ xin,
+                      float4* __restrict__ xout, int N) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= N) return;
+  if (i == 0) {
+    xout[0] = xin[0];
+    return;
+  }
+  const int4 a = A[i];
+  const float4 b = B[i], u = xin[a.x], d = xin[a.y], l = xin[a.z],
+               r = xin[a.w];
+  float4 o;
+  o.x = ((((b.x + u.x) + d.x) + l.x) + r.x) * 0.25f;
+  o.y = ((((b.y + u.y) + d.y) + l.y) + r.y) * 0.25f;
+  o.z = ((((b.z + u.z) + d.z) + l.z) + r.z) * 0.25f;
+  o.w = 0.0f;
+  xout[i] = o;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    equ_finalize_kernel(const int4* __restrict__ A,
+                        const float4* __restrict__ B,
+                        const float4* __restrict__ x, int N,
+                        uint8_t* __restrict__ img, float* __restrict__ state,
+                        double* __restrict__ partial) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < N;
+       i += gridDim.x * kBlock) {
+    const float4 xi = x[i];
+    const float v[3] = {xi.x, xi.y, xi.z};
+    if (i > 0) {
+      const int4 a = A[i];
+      const float4 b = B[i], u = x[a.x], d = x[a.y], l = x[a.z], r = x[a.w];
+      const float bs[3] = {b.x, b.y, b.z}, us[3] = {u.x, u.y, u.z},
+                  ds[3] = {d.x, d.y, d.z}, ls[3] = {l.x, l.y, l.z},
+                  rs[3] = {r.x, r.y, r.z};
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        float s = bs[c];
+        s = s + us[c];
+        s = s + ds[c];
+        s = s + ls[c];
+        s = s + rs[c];
+        acc[c] += fabs((double)s - (double)v[c] * 4.0);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (img) img[3LL * i + c] = clamp_u8(v[c]);
+      if (state) state[3LL * i + c] = v[c];
+    }
+  }
+  block_sum3(acc);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) partial[blockIdx.x * 3 + c] = acc[c];
+}
+
+int grid_blocks(long long n, int cap) {
+  long long b = (n + kBlock - 1) / kBlock;
+  if (b < 1) b = 1;
+  return (int)(b < cap ? b : cap);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ launchers --
+
+bool tile_supported(int S, int C) {
+#define PIE_T(s, c) \
+  if (S == s && C == c) return true;
+  PIE_TILES(PIE_T)
+#undef PIE_T
+  return false;
+}
+
+int tile_variants(int (*out)[2], int max) {
+  int n = 0;
+#define PIE_T(s, c)   \
+  if (n < max) {      \
+    out[n][0] = s;    \
+    out[n][1] = c;    \
+  }                   \
+  ++n;
+  PIE_TILES(PIE_T)
+#undef PIE_T
+  return n;
+}
+
+void tile_tasks(const TileCfg& cfg, int H, int W, int* stripes, int* segs) {
+  const int U = kWave * cfg.C - 2 * cfg.S;
+  *stripes = (W + U - 1) / U;
+  *segs = (H + cfg.seg_rows - 1) / cfg.seg_rows;
+}
+
+hipError_t launch_march(const TileCfg& cfg, int sched, int order, MarchArgs a,
+                        hipStream_t st) {
+  tile_tasks(cfg, a.H, a.W, &a.stripes, &a.segs);
+  a.seg_rows = cfg.seg_rows;
+  a.ntask = a.stripes * a.segs;
+#define PIE_T(s, c) \
+  if (cfg.S == s && cfg.C == c) return launch_march_##s##_##c(sched, order, a, st);
+  PIE_TILES(PIE_T)
+#undef PIE_T
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_deinterleave(const float* src, float* dst, long long plane,
+                               int H, int W, int P, hipStream_t st) {
+  hipLaunchKernelGGL(deinterleave_kernel, dim3(grid_blocks((long long)H * W, 8192)),
+                     dim3(kBlock), 0, st, src, dst, plane, H, W, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_interleave(const float* src, float* dst, long long plane,
+                             int H, int W, int P, hipStream_t st) {
+  hipLaunchKernelGGL(interleave_kernel, dim3(grid_blocks((long long)H * W, 8192)),
+                     dim3(kBlock), 0, st, src, dst, plane, H, W, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_mask(const int32_t* src, uint8_t* dst, int H, int W, int P,
+                       unsigned long long* counts, hipStream_t st) {
+  hipLaunchKernelGGL(mask_kernel, dim3(grid_blocks((long long)H * W, 8192)),
+                     dim3(kBlock), 0, st, src, dst, H, W, P, counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill3(float* dst, long long plane, float v0, float v1,
+                        float v2, hipStream_t st) {
+  hipLaunchKernelGGL(fill3_kernel, dim3(grid_blocks(plane, 8192)), dim3(kBlock),
+                     0, st, dst, plane, v0, v1, v2);
+  return hipGetLastError();
+}
+
+hipError_t launch_equ_scatter(const float* X, const float* B,
+                              const int32_t* lin, int N, float* x, float* g,
+                              long long plane, hipStream_t st) {
+  hipLaunchKernelGGL(equ_scatter_kernel, dim3(grid_blocks(N, 8192)), dim3(kBlock),
+                     0, st, X, B, lin, N, x, g, plane);
+  return hipGetLastError();
+}
+
+hipError_t launch_equ_gather(const float* x, const int32_t* lin, int N,
+                             long long plane, float x00, float x01, float x02,
+                             uint8_t* out_u8, float* out_f32,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(equ_gather_kernel, dim3(grid_blocks(N, 8192)), dim3(kBlock),
+                     0, st, x, lin, N, plane, x00, x01, x02, out_u8, out_f32);
+  return hipGetLastError();
+}
+
+int finalize_blocks(int H, int W) { return grid_blocks((long long)H * W, 2048); }
+
+hipError_t launch_finalize(int order, const float* x, const float* g,
+                           const uint8_t* m, long long plane, int H, int W,
+                           int P, uint8_t* img, double* partial, int nblocks,
+                           hipStream_t st) {
+  if (order == ORDER_UDLR)
+    hipLaunchKernelGGL(finalize_kernel<ORDER_UDLR>, dim3(nblocks), dim3(kBlock),
+                       0, st, x, g, m, plane, H, W, P, img, partial);
+  else
+    hipLaunchKernelGGL(finalize_kernel<ORDER_ULRD>, dim3(nblocks), dim3(kBlock),
+                       0, st, x, g, m, plane, H, W, P, img, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce(const double* partial, int n, float* err,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kBlock), 0, st, partial, n,
+                     err);
+  return hipGetLastError();
+}
+
+hipError_t launch_equ_pack(const float* X, float4* out, int N,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(equ_pack_kernel, dim3(grid_blocks(N, 8192)), dim3(kBlock),
+                     0, st, X, out, N);
+  return hipGetLastError();
+}
+
+hipError_t launch_equ_jacobi(const int4* A, const float4* B,
+                             const float4* xin, float4* xout, int N,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(equ_jacobi_kernel, dim3((N + kBlock - 1) / kBlock),
+                     dim3(kBlock), 0, st, A, B, xin, xout, N);
+  return hipGetLastError();
+}
+
+int equ_generic_blocks(int N) { return grid_blocks(N, 2048); }
+
+hipError_t launch_equ_finalize(const int4* A, const float4* B,
+                               const float4* x, int N, uint8_t* img,
+                               float* state, double* partial, int nblocks,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(equ_finalize_kernel, dim3(nblocks), dim3(kBlock), 0, st, A,
+                     B, x, N, img, state, partial);
+  return hipGetLastError();
+}
+
+}  // namespace pie

@@ -1,0 +1,657 @@
+// C ABI of the gfx950 Poisson solver (declared in include/fpie_hip.h).
+//
+// A pie_solver owns its device buffers and one HIP stream.  reset() deep-
+// copies the caller's arrays to HBM (as fpie/core/base_solver.h:27-59,
+// 101-140 copies numpy arrays to new[] buffers) and lays them out as
+// pitched float planes; step() runs the sweeps on the device and copies
+// back only the uint8 image and the three residual sums.
+//
+// EquSolver systems whose A matches the geometry recorded by partition()
+// (what fpie/process.py:248-266 always builds) are solved on the grid
+// layout by the same kernels as the GridSolver: b takes the place of grad
+// and every unsolved pixel holds X[0] (the value the reference reads through
+// id 0).  Any other A is solved by a gather kernel (Jacobi only).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fpie_hip.h"
+#include "kernels.h"
+
+using pie::MarchArgs;
+using pie::TileCfg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define PIE_HIP(expr)                                                   \
+  do {                                                                  \
+    hipError_t e_ = (expr);                                             \
+    if (e_ != hipSuccess)                                               \
+      return fail(PIE_EHIP, std::string(#expr) + ": " +                 \
+                                hipGetErrorString(e_));                 \
+  } while (0)
+
+#define PIE_CHECK(expr)          \
+  do {                           \
+    int r_ = (expr);             \
+    if (r_ != PIE_OK) return r_; \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t count) {
+    release();
+    n = count;
+    return hipMalloc(reinterpret_cast<void**>(&p),
+                     std::max<size_t>(count, 1) * sizeof(T));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+};
+
+int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+}  // namespace
+
+struct pie_solver {
+  int kind = PIE_GRID, sched = PIE_JACOBI, order = PIE_ORDER_ULRD;
+  int param_a = 0, param_b = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  bool loaded = false;
+  bool grid_layout = false;  // Grid, or Equ solved on the grid layout
+  int H = 0, W = 0, P = 0;
+  long long plane = 0, masked = 0;
+  int cur = 0;  // which of xa/xb holds the state
+  DevBuf<float> xa, xb, g;
+  DevBuf<uint8_t> m, img;
+  DevBuf<double> partial;
+  DevBuf<float> err;
+
+  // EquSolver
+  int N = 0;
+  float x0[3] = {0, 0, 0};
+  std::vector<int32_t> part_ids;
+  int ph = 0, pw = 0, part_count = -1;
+  DevBuf<int32_t> lin;
+  DevBuf<int4> A4;
+  DevBuf<float4> B4, X4a, X4b;
+
+  TileCfg cfg{8, 1, 64};
+  bool tile_fixed = false, tuned = false;
+
+  bool timing = false;
+  double t_ms = 0.0, t_updates = 0.0;
+  long long t_launches = 0;
+
+  float* state() { return cur ? xb.p : xa.p; }
+  float* other() { return cur ? xa.p : xb.p; }
+  int eff_order() const {
+    return kind == PIE_EQU ? PIE_ORDER_UDLR : order;  // gcc/equ.cc:48-53
+  }
+};
+
+namespace {
+
+void free_problem(pie_solver* s) {
+  s->loaded = false;
+  s->grid_layout = false;
+  s->xa.release();
+  s->xb.release();
+  s->g.release();
+  s->m.release();
+  s->img.release();
+  s->partial.release();
+  s->err.release();
+  s->lin.release();
+  s->A4.release();
+  s->B4.release();
+  s->X4a.release();
+  s->X4b.release();
+  s->H = s->W = s->P = 0;
+  s->plane = s->masked = 0;
+  s->N = 0;
+  s->cur = 0;
+  s->tuned = false;
+}
+
+int upload(const void* host, size_t bytes, void* dev, hipStream_t st) {
+  PIE_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, st));
+  return PIE_OK;
+}
+
+// Total global steps of one call of `iters` sweeps and the band width.
+void schedule_steps(const pie_solver* s, int iters, int* steps, int* band) {
+  *band = 2 * iters;
+  if (s->sched == PIE_JACOBI)
+    *steps = iters;
+  else if (s->sched == PIE_REDBLACK)
+    *steps = 2 * iters;
+  else  // sweep k of pixel (i, j) runs at step 2k + i + j
+    *steps = iters > 0 ? 2 * (iters - 1) + (s->H - 1) + (s->W - 1) + 1 : 0;
+}
+
+MarchArgs march_args(pie_solver* s, const float* xin, float* xout, int t0,
+                     int t_end, int band) {
+  MarchArgs a{};
+  a.xin = xin;
+  a.xout = xout;
+  a.g = s->g.p;
+  a.m = s->m.p;
+  a.plane = s->plane;
+  a.H = s->H;
+  a.W = s->W;
+  a.P = s->P;
+  a.t0 = t0;
+  a.t_end = t_end;
+  a.band = band;
+  return a;
+}
+
+int alloc_grid(pie_solver* s, int H, int W) {
+  if (H < 1 || W < 1) return fail(PIE_EINVAL, "empty grid");
+  s->H = H;
+  s->W = W;
+  s->P = round_up(W, 64);
+  s->plane = (long long)H * s->P;
+  if (s->plane > (1LL << 31) - 1)
+    return fail(PIE_EINVAL, "grid larger than 2^31 pixels");
+  PIE_HIP(s->xa.alloc(3 * s->plane));
+  PIE_HIP(s->xb.alloc(3 * s->plane));
+  PIE_HIP(s->g.alloc(3 * s->plane));
+  PIE_HIP(s->m.alloc(s->plane));
+  PIE_HIP(hipMemsetAsync(s->m.p, 0, s->plane, s->stream));
+  PIE_HIP(hipMemsetAsync(s->g.p, 0, 3 * s->plane * sizeof(float), s->stream));
+  const int nb = pie::finalize_blocks(H, W);
+  PIE_HIP(s->partial.alloc(3 * (size_t)nb));
+  PIE_HIP(s->err.alloc(3));
+  s->grid_layout = true;
+  return PIE_OK;
+}
+
+int default_tile(pie_solver* s) {
+  if (s->tile_fixed) return PIE_OK;
+  // small grids: short row ranges so that enough waves are in flight
+  const long long px = (long long)s->H * s->W;
+  s->cfg = px < (1LL << 20) ? TileCfg{4, 1, 16} : TileCfg{8, 1, 64};
+  return PIE_OK;
+}
+
+int run_sweeps(pie_solver* s, int iters);
+
+int autotune(pie_solver* s, int launches) {
+  if (!s->grid_layout) return PIE_OK;
+  int var[32][2];
+  const int nv = std::min(pie::tile_variants(var, 32), 32);
+  const int rows[] = {16, 32, 64, 128, 256};
+  float best = 1e30f;
+  TileCfg best_cfg = s->cfg;
+  int steps, band;
+  schedule_steps(s, 1 << 20, &steps, &band);
+  for (int v = 0; v < nv; ++v)
+    for (int rr : rows) {
+      if (rr > 2 * s->H && rr != rows[0]) continue;
+      TileCfg c{var[v][0], var[v][1], rr};
+      // the state buffer is only read; the spare buffer takes the output.
+      // t0 = H + W puts every pixel inside the wavefront band of gs_lex.
+      MarchArgs a = march_args(s, s->state(), s->other(), s->H + s->W,
+                               1 << 30, band);
+      PIE_HIP(pie::launch_march(c, s->sched, s->eff_order(), a, s->stream));
+      PIE_HIP(hipEventRecord(s->ev0, s->stream));
+      for (int l = 0; l < launches; ++l)
+        PIE_HIP(pie::launch_march(c, s->sched, s->eff_order(), a, s->stream));
+      PIE_HIP(hipEventRecord(s->ev1, s->stream));
+      PIE_HIP(hipEventSynchronize(s->ev1));
+      float ms = 0.0f;
+      PIE_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+      const float per_sweep = ms / (launches * c.S);
+      if (per_sweep < best) {
+        best = per_sweep;
+        best_cfg = c;
+      }
+    }
+  s->cfg = best_cfg;
+  s->tuned = true;
+  return PIE_OK;
+}
+
+int run_sweeps(pie_solver* s, int iters) {
+  if (iters <= 0) return PIE_OK;
+  if (!s->grid_layout) {  // EquSolver on an arbitrary A: one sweep a launch
+    if (s->timing) PIE_HIP(hipEventRecord(s->ev0, s->stream));
+    for (int k = 0; k < iters; ++k) {
+      const float4* in = s->cur ? s->X4b.p : s->X4a.p;
+      float4* out = s->cur ? s->X4a.p : s->X4b.p;
+      PIE_HIP(pie::launch_equ_jacobi(s->A4.p, s->B4.p, in, out, s->N,
+                                     s->stream));
+      s->cur ^= 1;
+    }
+    if (s->timing) {
+      PIE_HIP(hipEventRecord(s->ev1, s->stream));
+      PIE_HIP(hipEventSynchronize(s->ev1));
+      float ms = 0.0f;
+      PIE_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+      s->t_ms += ms;
+      s->t_launches += iters;
+      s->t_updates += (double)iters * s->masked;
+    }
+    return PIE_OK;
+  }
+  int steps, band;
+  schedule_steps(s, iters, &steps, &band);
+  const TileCfg c = s->cfg;
+  if (s->timing) PIE_HIP(hipEventRecord(s->ev0, s->stream));
+  long long launches = 0;
+  for (int t0 = 0; t0 < steps; t0 += c.S) {
+    MarchArgs a = march_args(s, s->state(), s->other(), t0, steps, band);
+    PIE_HIP(pie::launch_march(c, s->sched, s->eff_order(), a, s->stream));
+    s->cur ^= 1;
+    ++launches;
+  }
+  if (s->timing) {
+    PIE_HIP(hipEventRecord(s->ev1, s->stream));
+    PIE_HIP(hipEventSynchronize(s->ev1));
+    float ms = 0.0f;
+    PIE_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    s->t_ms += ms;
+    s->t_launches += launches;
+    s->t_updates += (double)iters * s->masked;
+  }
+  return PIE_OK;
+}
+
+int read_outputs(pie_solver* s, uint8_t* host_img, float* host_err,
+                 float* host_state) {
+  if (!s->loaded) return fail(PIE_ESTATE, "step before reset");
+  if (s->grid_layout) {
+    const int nb = pie::finalize_blocks(s->H, s->W);
+    const bool grid_img = host_img && s->kind == PIE_GRID;
+    if (grid_img && !s->img.p)
+      PIE_HIP(s->img.alloc((size_t)s->H * s->W * 3));
+    PIE_HIP(pie::launch_finalize(s->eff_order(), s->state(), s->g.p, s->m.p,
+                                 s->plane, s->H, s->W, s->P,
+                                 grid_img ? s->img.p : nullptr, s->partial.p,
+                                 nb, s->stream));
+    PIE_HIP(pie::launch_reduce(s->partial.p, nb, s->err.p, s->stream));
+    if (s->kind == PIE_EQU && (host_img || host_state)) {
+      if (host_img && !s->img.p) PIE_HIP(s->img.alloc((size_t)s->N * 3));
+      DevBuf<float> st;
+      if (host_state) PIE_HIP(st.alloc((size_t)s->N * 3));
+      PIE_HIP(pie::launch_equ_gather(s->state(), s->lin.p, s->N, s->plane,
+                                     s->x0[0], s->x0[1], s->x0[2],
+                                     host_img ? s->img.p : nullptr,
+                                     host_state ? st.p : nullptr, s->stream));
+      if (host_img)
+        PIE_HIP(hipMemcpyAsync(host_img, s->img.p, (size_t)s->N * 3,
+                               hipMemcpyDeviceToHost, s->stream));
+      if (host_state)
+        PIE_HIP(hipMemcpyAsync(host_state, st.p, (size_t)s->N * 3 * 4,
+                               hipMemcpyDeviceToHost, s->stream));
+      PIE_HIP(hipStreamSynchronize(s->stream));
+    } else if (s->kind == PIE_GRID) {
+      if (grid_img)
+        PIE_HIP(hipMemcpyAsync(host_img, s->img.p, (size_t)s->H * s->W * 3,
+                               hipMemcpyDeviceToHost, s->stream));
+      if (host_state) {
+        DevBuf<float> st;
+        PIE_HIP(st.alloc((size_t)s->H * s->W * 3));
+        PIE_HIP(pie::launch_interleave(s->state(), st.p, s->plane, s->H, s->W,
+                                       s->P, s->stream));
+        PIE_HIP(hipMemcpyAsync(host_state, st.p, (size_t)s->H * s->W * 3 * 4,
+                               hipMemcpyDeviceToHost, s->stream));
+        PIE_HIP(hipStreamSynchronize(s->stream));
+      }
+    }
+  } else {
+    const int nb = pie::equ_generic_blocks(s->N);
+    if (host_img && !s->img.p) PIE_HIP(s->img.alloc((size_t)s->N * 3));
+    DevBuf<float> st;
+    if (host_state) PIE_HIP(st.alloc((size_t)s->N * 3));
+    const float4* x = s->cur ? s->X4b.p : s->X4a.p;
+    PIE_HIP(pie::launch_equ_finalize(s->A4.p, s->B4.p, x, s->N,
+                                     host_img ? s->img.p : nullptr,
+                                     host_state ? st.p : nullptr, s->partial.p,
+                                     nb, s->stream));
+    PIE_HIP(pie::launch_reduce(s->partial.p, nb, s->err.p, s->stream));
+    if (host_img)
+      PIE_HIP(hipMemcpyAsync(host_img, s->img.p, (size_t)s->N * 3,
+                             hipMemcpyDeviceToHost, s->stream));
+    if (host_state)
+      PIE_HIP(hipMemcpyAsync(host_state, st.p, (size_t)s->N * 3 * 4,
+                             hipMemcpyDeviceToHost, s->stream));
+    PIE_HIP(hipStreamSynchronize(s->stream));
+  }
+  float e[3];
+  PIE_HIP(hipMemcpyAsync(e, s->err.p, sizeof(e), hipMemcpyDeviceToHost,
+                         s->stream));
+  PIE_HIP(hipStreamSynchronize(s->stream));
+  if (host_err) std::memcpy(host_err, e, sizeof(e));
+  return PIE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pie_last_error(void) { return g_err.c_str(); }
+
+int pie_create(int kind, int schedule, int order, int param_a, int param_b,
+               pie_solver** out) {
+  if (!out) return fail(PIE_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (kind != PIE_EQU && kind != PIE_GRID)
+    return fail(PIE_EINVAL, "kind must be PIE_EQU or PIE_GRID");
+  if (schedule != PIE_JACOBI && schedule != PIE_GS_LEX &&
+      schedule != PIE_REDBLACK)
+    return fail(PIE_EINVAL, "unknown schedule");
+  if (order != PIE_ORDER_UDLR && order != PIE_ORDER_ULRD)
+    return fail(PIE_EINVAL, "unknown neighbour order");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
+    return fail(PIE_EHIP, "no HIP device");
+  pie_solver* s = new pie_solver();
+  s->kind = kind;
+  s->sched = schedule;
+  s->order = order;
+  s->param_a = param_a;
+  s->param_b = param_b;
+  hipError_t e = hipGetDevice(&s->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&s->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&s->ev1);
+  if (e != hipSuccess) {
+    pie_destroy(s);
+    return fail(PIE_EHIP, std::string("solver set-up: ") + hipGetErrorString(e));
+  }
+  *out = s;
+  return PIE_OK;
+}
+
+int pie_destroy(pie_solver* s) {
+  if (!s) return PIE_OK;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  free_problem(s);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return PIE_OK;
+}
+
+int pie_partition(pie_solver* s, const int32_t* mask, int h, int w,
+                  int32_t* ids) {
+  if (!s || !mask || !ids) return fail(PIE_EINVAL, "NULL argument");
+  if (s->kind != PIE_EQU) return fail(PIE_EINVAL, "partition is an EquSolver call");
+  if (h < 1 || w < 1) return fail(PIE_EINVAL, "empty mask");
+  // fpie/core/gcc/equ.cc:14-32: row-major ids from 1 over mask > 0
+  s->part_ids.assign((size_t)h * w, 0);
+  int cnt = 0;
+  for (long long i = 0; i < (long long)h * w; ++i)
+    s->part_ids[i] = mask[i] > 0 ? ++cnt : 0;
+  s->ph = h;
+  s->pw = w;
+  s->part_count = cnt;
+  std::memcpy(ids, s->part_ids.data(), sizeof(int32_t) * (size_t)h * w);
+  return PIE_OK;
+}
+
+int pie_reset_equ(pie_solver* s, int n, const int32_t* A, const float* X,
+                  const float* B) {
+  if (!s || !A || !X || !B) return fail(PIE_EINVAL, "NULL argument");
+  if (s->kind != PIE_EQU) return fail(PIE_EINVAL, "reset_equ on a GridSolver");
+  if (n < 1) return fail(PIE_EINVAL, "N must be >= 1");
+  for (long long k = 0; k < 4LL * n; ++k)
+    if (A[k] < 0 || A[k] >= n)
+      return fail(PIE_EINVAL, "A[" + std::to_string(k / 4) + "][" +
+                                  std::to_string(k % 4) + "] = " +
+                                  std::to_string(A[k]) + " outside [0, N)");
+  PIE_HIP(hipStreamSynchronize(s->stream));
+  free_problem(s);
+  s->N = n;
+  for (int c = 0; c < 3; ++c) s->x0[c] = X[c];
+
+  // Does A describe the 4-neighbourhood of the partitioned mask?
+  std::vector<int32_t> lin;
+  bool structured = s->part_count == n - 1 && s->ph >= 3 && s->pw >= 3;
+  if (structured) {
+    const int h = s->ph, w = s->pw, P = round_up(w, 64);
+    const int32_t* ids = s->part_ids.data();
+    lin.assign(n, 0);
+    for (int i = 0; i < h && structured; ++i)
+      for (int j = 0; j < w; ++j) {
+        const int id = ids[(long long)i * w + j];
+        if (!id) continue;
+        if (i == 0 || j == 0 || i == h - 1 || j == w - 1) {
+          structured = false;
+          break;
+        }
+        const int32_t* a = A + 4LL * id;
+        if (a[0] != ids[(long long)(i - 1) * w + j] ||
+            a[1] != ids[(long long)(i + 1) * w + j] ||
+            a[2] != ids[(long long)i * w + j - 1] ||
+            a[3] != ids[(long long)i * w + j + 1]) {
+          structured = false;
+          break;
+        }
+        lin[id] = i * P + j;
+      }
+  }
+
+  if (structured) {
+    PIE_CHECK(alloc_grid(s, s->ph, s->pw));
+    std::vector<uint8_t> mh((size_t)s->plane, 0);
+    for (int i = 0; i < s->H; ++i)
+      for (int j = 0; j < s->W; ++j)
+        mh[(size_t)i * s->P + j] = s->part_ids[(size_t)i * s->W + j] ? 1 : 0;
+    DevBuf<float> Xd, Bd;
+    PIE_HIP(Xd.alloc(3 * (size_t)n));
+    PIE_HIP(Bd.alloc(3 * (size_t)n));
+    PIE_HIP(s->lin.alloc(n));
+    PIE_CHECK(upload(mh.data(), mh.size(), s->m.p, s->stream));
+    PIE_CHECK(upload(X, 12 * (size_t)n, Xd.p, s->stream));
+    PIE_CHECK(upload(B, 12 * (size_t)n, Bd.p, s->stream));
+    PIE_CHECK(upload(lin.data(), 4 * (size_t)n, s->lin.p, s->stream));
+    // unsolved pixels hold X[0]: the value the reference reads via id 0
+    PIE_HIP(pie::launch_fill3(s->xa.p, s->plane, X[0], X[1], X[2], s->stream));
+    PIE_HIP(pie::launch_equ_scatter(Xd.p, Bd.p, s->lin.p, n, s->xa.p, s->g.p,
+                                    s->plane, s->stream));
+    PIE_HIP(hipMemcpyAsync(s->xb.p, s->xa.p, 3 * s->plane * sizeof(float),
+                           hipMemcpyDeviceToDevice, s->stream));
+    PIE_HIP(hipStreamSynchronize(s->stream));
+    s->masked = n - 1;
+    PIE_CHECK(default_tile(s));
+  } else {
+    if (s->sched != PIE_JACOBI)
+      return fail(PIE_EUNSUPPORTED,
+                  "gs_lex and redblack need the system built from partition()");
+    PIE_HIP(s->A4.alloc(n));
+    PIE_HIP(s->B4.alloc(n));
+    PIE_HIP(s->X4a.alloc(n));
+    PIE_HIP(s->X4b.alloc(n));
+    DevBuf<float> Xd, Bd;
+    PIE_HIP(Xd.alloc(3 * (size_t)n));
+    PIE_HIP(Bd.alloc(3 * (size_t)n));
+    PIE_CHECK(upload(A, 16 * (size_t)n, s->A4.p, s->stream));
+    PIE_CHECK(upload(X, 12 * (size_t)n, Xd.p, s->stream));
+    PIE_CHECK(upload(B, 12 * (size_t)n, Bd.p, s->stream));
+    PIE_HIP(pie::launch_equ_pack(Xd.p, s->X4a.p, n, s->stream));
+    PIE_HIP(pie::launch_equ_pack(Bd.p, s->B4.p, n, s->stream));
+    PIE_HIP(hipMemcpyAsync(s->X4b.p, s->X4a.p, (size_t)n * sizeof(float4),
+                           hipMemcpyDeviceToDevice, s->stream));
+    const int nb = pie::equ_generic_blocks(n);
+    PIE_HIP(s->partial.alloc(3 * (size_t)nb));
+    PIE_HIP(s->err.alloc(3));
+    PIE_HIP(hipStreamSynchronize(s->stream));
+    s->masked = n - 1;
+  }
+  s->loaded = true;
+  return PIE_OK;
+}
+
+int pie_reset_grid(pie_solver* s, int h, int w, const int32_t* mask,
+                   const float* tgt, const float* grad) {
+  if (!s || !mask || !tgt || !grad) return fail(PIE_EINVAL, "NULL argument");
+  if (s->kind != PIE_GRID) return fail(PIE_EINVAL, "reset_grid on an EquSolver");
+  PIE_HIP(hipStreamSynchronize(s->stream));
+  free_problem(s);
+  PIE_CHECK(alloc_grid(s, h, w));
+  const size_t px = (size_t)h * w;
+  DevBuf<char> stage;
+  PIE_HIP(stage.alloc(px * 12));
+  DevBuf<unsigned long long> counts;
+  PIE_HIP(counts.alloc(2));
+  PIE_HIP(hipMemsetAsync(counts.p, 0, 2 * sizeof(unsigned long long), s->stream));
+  PIE_CHECK(upload(mask, px * 4, stage.p, s->stream));
+  PIE_HIP(pie::launch_mask(reinterpret_cast<const int32_t*>(stage.p), s->m.p,
+                           h, w, s->P, counts.p, s->stream));
+  unsigned long long cnt[2] = {0, 0};
+  PIE_HIP(hipMemcpyAsync(cnt, counts.p, sizeof(cnt), hipMemcpyDeviceToHost,
+                         s->stream));
+  PIE_HIP(hipStreamSynchronize(s->stream));
+  if (cnt[1]) {
+    free_problem(s);
+    return fail(PIE_EINVAL,
+                "mask has solved pixels on the outermost rows/columns");
+  }
+  PIE_CHECK(upload(tgt, px * 12, stage.p, s->stream));
+  PIE_HIP(pie::launch_deinterleave(reinterpret_cast<const float*>(stage.p),
+                                   s->xa.p, s->plane, h, w, s->P, s->stream));
+  PIE_HIP(hipMemcpyAsync(s->xb.p, s->xa.p, 3 * s->plane * sizeof(float),
+                         hipMemcpyDeviceToDevice, s->stream));
+  PIE_CHECK(upload(grad, px * 12, stage.p, s->stream));
+  PIE_HIP(pie::launch_deinterleave(reinterpret_cast<const float*>(stage.p),
+                                   s->g.p, s->plane, h, w, s->P, s->stream));
+  PIE_HIP(hipStreamSynchronize(s->stream));
+  s->masked = (long long)cnt[0];
+  s->N = h * w;
+  PIE_CHECK(default_tile(s));
+  s->loaded = true;
+  return PIE_OK;
+}
+
+int pie_sync(pie_solver* s) {
+  if (!s) return fail(PIE_EINVAL, "NULL solver");
+  PIE_HIP(hipStreamSynchronize(s->stream));
+  return PIE_OK;
+}
+
+int pie_iterate(pie_solver* s, int iterations) {
+  if (!s) return fail(PIE_EINVAL, "NULL solver");
+  if (!s->loaded) return fail(PIE_ESTATE, "step before reset");
+  if (iterations < 0) return fail(PIE_EINVAL, "negative iteration count");
+  if (s->grid_layout && !s->tuned && !s->tile_fixed &&
+      (long long)s->H * s->W >= (1LL << 22))
+    PIE_CHECK(autotune(s, 3));
+  return run_sweeps(s, iterations);
+}
+
+int pie_read(pie_solver* s, uint8_t* img, float* err) {
+  if (!s) return fail(PIE_EINVAL, "NULL solver");
+  return read_outputs(s, img, err, nullptr);
+}
+
+int pie_step(pie_solver* s, int iterations, uint8_t* img, float* err) {
+  PIE_CHECK(pie_iterate(s, iterations));
+  return pie_read(s, img, err);
+}
+
+int pie_get_state(pie_solver* s, float* out) {
+  if (!s || !out) return fail(PIE_EINVAL, "NULL argument");
+  return read_outputs(s, nullptr, nullptr, out);
+}
+
+int pie_set_tile(pie_solver* s, int sweeps, int cols_per_lane, int seg_rows) {
+  if (!s) return fail(PIE_EINVAL, "NULL solver");
+  TileCfg c = s->cfg;
+  if (sweeps > 0) c.S = sweeps;
+  if (cols_per_lane > 0) c.C = cols_per_lane;
+  if (seg_rows > 0) c.seg_rows = seg_rows;
+  if (!pie::tile_supported(c.S, c.C))
+    return fail(PIE_EUNSUPPORTED, "tile (sweeps " + std::to_string(c.S) +
+                                      ", columns/lane " + std::to_string(c.C) +
+                                      ") not compiled");
+  s->cfg = c;
+  s->tile_fixed = true;
+  return PIE_OK;
+}
+
+int pie_get_tile(pie_solver* s, int* sweeps, int* cols_per_lane,
+                 int* seg_rows) {
+  if (!s) return fail(PIE_EINVAL, "NULL solver");
+  if (sweeps) *sweeps = s->cfg.S;
+  if (cols_per_lane) *cols_per_lane = s->cfg.C;
+  if (seg_rows) *seg_rows = s->cfg.seg_rows;
+  return PIE_OK;
+}
+
+int pie_autotune(pie_solver* s, int launches) {
+  if (!s) return fail(PIE_EINVAL, "NULL solver");
+  if (!s->loaded) return fail(PIE_ESTATE, "autotune before reset");
+  if (launches < 1) launches = 1;
+  s->tile_fixed = false;
+  PIE_CHECK(autotune(s, launches));
+  return PIE_OK;
+}
+
+int pie_set_timing(pie_solver* s, int enable) {
+  if (!s) return fail(PIE_EINVAL, "NULL solver");
+  s->timing = enable != 0;
+  s->t_ms = 0.0;
+  s->t_updates = 0.0;
+  s->t_launches = 0;
+  return PIE_OK;
+}
+
+int pie_get_timing(pie_solver* s, double* kernel_ms, long long* launches,
+                   double* pixel_updates) {
+  if (!s) return fail(PIE_EINVAL, "NULL solver");
+  if (kernel_ms) *kernel_ms = s->t_ms;
+  if (launches) *launches = s->t_launches;
+  if (pixel_updates) *pixel_updates = s->t_updates;
+  return PIE_OK;
+}
+
+int pie_shape(pie_solver* s, int* h, int* w, int* n, long long* masked) {
+  if (!s) return fail(PIE_EINVAL, "NULL solver");
+  if (h) *h = s->H;
+  if (w) *w = s->W;
+  if (n) *n = s->N;
+  if (masked) *masked = s->masked;
+  return PIE_OK;
+}
+
+int pie_device_info(char* buf, int len) {
+  if (!buf || len < 1) return fail(PIE_EINVAL, "NULL buffer");
+  int dev = 0;
+  PIE_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t p;
+  PIE_HIP(hipGetDeviceProperties(&p, dev));
+  std::snprintf(buf, (size_t)len,
+                "device %d: %s (%s), %d CUs, %.1f GiB HBM, %d MHz, "
+                "L2 %d KiB, LDS/block %zu KiB",
+                dev, p.name, p.gcnArchName, p.multiProcessorCount,
+                p.totalGlobalMem / 1073741824.0, p.clockRate / 1000,
+                p.l2CacheSize / 1024, p.sharedMemPerBlock / 1024);
+  return PIE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,185 @@
+"""``hip`` solver cores with the reference's solver-core interface.
+
+``EquSolver`` and ``GridSolver`` expose the methods that
+fpie/process.py calls on ``core_gcc`` / ``core_cuda`` objects
+(fpie/core/gcc/solver.cc:3-15, fpie/core/base_solver.h):
+
+* ``EquSolver.partition(mask) -> int32[h, w]`` (gcc/equ.cc:14-32)
+* ``EquSolver.reset(N, A, X, B)`` (base_solver.h:27-59)
+* ``GridSolver.reset(N, mask, tgt, grad)`` (base_solver.h:101-140)
+* ``sync()`` and ``step(iteration) -> (uint8 image, float32[3] err)``
+  (gcc/equ.cc:74-86, gcc/grid.cc:57-81)
+
+Arguments are converted to C-contiguous arrays of the reference's dtypes (the
+pybind11 ``array_t`` force-cast) and handed to the C ABI; every array
+returned is a new numpy array, as the pybind11 cores return copies.
+
+Beyond the reference, the constructors take ``schedule`` ("jacobi",
+"gs_lex" = the gcc backend's lexicographic Gauss-Seidel, "redblack" = the
+openmp EquSolver's order) and, for the GridSolver, ``order`` ("ulrd" =
+grad+up+left+right+down as gcc/openmp, "udlr" = grad+up+down+left+right as
+numpy).  The EquSolver always adds b+up+down+left+right (gcc/equ.cc:48-53).
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import native
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+  return ctypes.c_void_p(a.ctypes.data)
+
+
+class _Core:
+  _kind = native.PIE_GRID
+
+  def __init__(self, schedule: str, order: str, a: int, b: int):
+    if schedule not in native.SCHEDULES:
+      raise ValueError(f"schedule must be one of {sorted(native.SCHEDULES)}")
+    if order not in native.ORDERS:
+      raise ValueError(f"order must be one of {sorted(native.ORDERS)}")
+    lib = native.load()
+    h = ctypes.c_void_p()
+    native.check(lib.pie_create(self._kind, native.SCHEDULES[schedule],
+                                native.ORDERS[order], int(a), int(b),
+                                ctypes.byref(h)))
+    self._h = h
+    self._lib = lib
+    self.schedule = schedule
+    self.order = order
+
+  def __del__(self):
+    h = getattr(self, "_h", None)
+    if h is not None and h.value:
+      self._lib.pie_destroy(h)
+      self._h = None
+
+  def sync(self) -> None:
+    native.check(self._lib.pie_sync(self._h))
+
+  # ---- extensions (no counterpart in the reference) --------------------
+  def iterate(self, iteration: int) -> None:
+    """Queue sweeps on the device without reading anything back."""
+    native.check(self._lib.pie_iterate(self._h, int(iteration)))
+
+  def set_tile(self, sweeps: int = 0, cols_per_lane: int = 0,
+               seg_rows: int = 0) -> None:
+    native.check(self._lib.pie_set_tile(self._h, sweeps, cols_per_lane,
+                                        seg_rows))
+
+  def tile(self) -> tuple[int, int, int]:
+    v = [ctypes.c_int() for _ in range(3)]
+    native.check(self._lib.pie_get_tile(self._h, *(ctypes.byref(x)
+                                                   for x in v)))
+    return tuple(x.value for x in v)
+
+  def autotune(self, launches: int = 3) -> tuple[int, int, int]:
+    native.check(self._lib.pie_autotune(self._h, int(launches)))
+    return self.tile()
+
+  def set_timing(self, enable: bool = True) -> None:
+    native.check(self._lib.pie_set_timing(self._h, int(bool(enable))))
+
+  def timing(self) -> dict:
+    ms = ctypes.c_double()
+    n = ctypes.c_longlong()
+    upd = ctypes.c_double()
+    native.check(self._lib.pie_get_timing(self._h, ctypes.byref(ms),
+                                          ctypes.byref(n), ctypes.byref(upd)))
+    return {"kernel_ms": ms.value, "launches": n.value,
+            "pixel_updates": upd.value}
+
+  def shape(self) -> dict:
+    h, w, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    masked = ctypes.c_longlong()
+    native.check(self._lib.pie_shape(self._h, ctypes.byref(h),
+                                     ctypes.byref(w), ctypes.byref(n),
+                                     ctypes.byref(masked)))
+    return {"h": h.value, "w": w.value, "n": n.value, "masked": masked.value}
+
+
+class EquSolver(_Core):
+  """Equation-form solver (fpie/core/base_solver.h:12-75)."""
+
+  _kind = native.PIE_EQU
+
+  def __init__(self, block_size: int = 1024, schedule: str = "jacobi"):
+    super().__init__(schedule, "udlr", block_size, 0)
+    self.N = 0
+
+  def partition(self, mask: np.ndarray) -> np.ndarray:
+    m = np.ascontiguousarray(mask, dtype=np.int32)
+    if m.ndim != 2:
+      raise ValueError("mask must be 2-D")
+    ids = np.empty_like(m)
+    native.check(self._lib.pie_partition(self._h, _ptr(m), m.shape[0],
+                                         m.shape[1], _ptr(ids)))
+    return ids
+
+  def reset(self, N: int, A: np.ndarray, X: np.ndarray,
+            B: np.ndarray) -> None:
+    N = int(N)
+    A = np.ascontiguousarray(A, dtype=np.int32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    if A.shape != (N, 4) or X.shape != (N, 3) or B.shape != (N, 3):
+      raise ValueError(f"expected A[{N},4], X[{N},3], B[{N},3]; got "
+                       f"{A.shape}, {X.shape}, {B.shape}")
+    native.check(self._lib.pie_reset_equ(self._h, N, _ptr(A), _ptr(X),
+                                         _ptr(B)))
+    self.N = N
+
+  def step(self, iteration: int) -> tuple[np.ndarray, np.ndarray]:
+    img = np.empty((self.N, 3), np.uint8)
+    err = np.empty(3, np.float32)
+    native.check(self._lib.pie_step(self._h, int(iteration), _ptr(img),
+                                    _ptr(err)))
+    return img, err
+
+  def state(self) -> np.ndarray:
+    """Float32 X[N][3] (extension)."""
+    out = np.empty((self.N, 3), np.float32)
+    native.check(self._lib.pie_get_state(self._h, _ptr(out)))
+    return out
+
+
+class GridSolver(_Core):
+  """Grid-form solver (fpie/core/base_solver.h:77-152)."""
+
+  _kind = native.PIE_GRID
+
+  def __init__(self, grid_x: int = 8, grid_y: int = 8,
+               schedule: str = "jacobi", order: str = "ulrd"):
+    super().__init__(schedule, order, grid_x, grid_y)
+    self.grid_x, self.grid_y = grid_x, grid_y
+    self.hw = (0, 0)
+
+  def reset(self, N: int, mask: np.ndarray, tgt: np.ndarray,
+            grad: np.ndarray) -> None:
+    # N is ignored; the shape comes from mask (base_solver.h:112-113)
+    m = np.ascontiguousarray(mask, dtype=np.int32)
+    t = np.ascontiguousarray(tgt, dtype=np.float32)
+    g = np.ascontiguousarray(grad, dtype=np.float32)
+    if m.ndim != 2 or t.shape != m.shape + (3,) or g.shape != t.shape:
+      raise ValueError(f"expected mask[h,w], tgt[h,w,3], grad[h,w,3]; got "
+                       f"{m.shape}, {t.shape}, {g.shape}")
+    native.check(self._lib.pie_reset_grid(self._h, m.shape[0], m.shape[1],
+                                          _ptr(m), _ptr(t), _ptr(g)))
+    self.hw = m.shape
+
+  def step(self, iteration: int) -> tuple[np.ndarray, np.ndarray]:
+    img = np.empty(self.hw + (3,), np.uint8)
+    err = np.empty(3, np.float32)
+    native.check(self._lib.pie_step(self._h, int(iteration), _ptr(img),
+                                    _ptr(err)))
+    return img, err
+
+  def state(self) -> np.ndarray:
+    """Float32 tgt[h][w][3] (extension)."""
+    out = np.empty(self.hw + (3,), np.float32)
+    native.check(self._lib.pie_get_state(self._h, _ptr(out)))
+    return out

@@ -1,0 +1,141 @@
+/*
+ * fpie_hip.h - C ABI of the MI355X (gfx950) Poisson image-editing solver.
+ *
+ * The entry points replace the pybind11 solver cores that
+ * fpie/process.py (Trinkle23897/Fast-Poisson-Image-Editing, fpie 0.3.2)
+ * drives through duck typing (process.py:161-176, 187, 270, 275, 300-317,
+ * 385, 390):
+ *
+ *   pybind11 method (fpie/core/gcc/solver.cc:3-15,
+ *                    fpie/core/base_solver.h)             C entry point
+ *   core_cuda.EquSolver(block_size)   cuda/solver.cc      pie_create(PIE_EQU,...)
+ *   core_cuda.GridSolver(gx, gy)      cuda/solver.cc      pie_create(PIE_GRID,...)
+ *   EquSolver.partition(mask)         gcc/equ.cc:14-32    pie_partition
+ *   EquSolver.reset(N, A, X, B)       base_solver.h:27-59 pie_reset_equ
+ *   GridSolver.reset(N, mask, t, g)   base_solver.h:101-140 pie_reset_grid
+ *   *.sync()                          base_solver.h:61,142 pie_sync
+ *   *.step(iteration) -> (img, err)   gcc/equ.cc:74-86,
+ *                                     gcc/grid.cc:57-81   pie_step
+ *   (object destruction)              base_solver.h:19-25 pie_destroy
+ *   (exceptions -> RuntimeError)      base_solver.h:63-74 status + pie_last_error
+ *
+ * All pointers are host pointers to plain C arrays; the library copies what
+ * it needs (reset deep-copies, as base_solver.h:35-56 does) and writes
+ * outputs into caller-owned buffers.  Every call returns PIE_OK (0) or a
+ * negative status; pie_last_error() then returns a message for the calling
+ * thread.  One solver object must not be used from two threads at once.
+ */
+#ifndef FPIE_HIP_H_
+#define FPIE_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pie_solver pie_solver;
+
+/* solver kinds: the reference's EquSolver (A/X/B system) and GridSolver */
+#define PIE_EQU 0
+#define PIE_GRID 1
+
+/* update schedules
+ *  PIE_JACOBI   every masked pixel from the previous sweep
+ *               (fpie/np_solver.py:31-41,79-88; openmp/grid.cc:75-103)
+ *  PIE_GS_LEX   lexicographic Gauss-Seidel, the result of the gcc backend's
+ *               in-place row-major loop (gcc/equ.cc:74-80, gcc/grid.cc:62-75),
+ *               computed by a skewed two-colour wavefront schedule
+ *  PIE_REDBLACK (i+j)-odd pixels, then (i+j)-even (openmp/equ.cc:107-118) */
+#define PIE_JACOBI 0
+#define PIE_GS_LEX 1
+#define PIE_REDBLACK 2
+
+/* order in which the four neighbours are added after b (Equ) or grad (Grid)
+ *  PIE_ORDER_UDLR  up, down, left, right: every EquSolver (gcc/equ.cc:48-53)
+ *                  and the numpy GridSolver (np_solver.py:83-88)
+ *  PIE_ORDER_ULRD  up, left, right, down: gcc/openmp/cuda GridSolver
+ *                  (gcc/grid.cc:23-35, openmp/grid.cc:35-43) */
+#define PIE_ORDER_UDLR 0
+#define PIE_ORDER_ULRD 1
+
+/* status codes */
+#define PIE_OK 0
+#define PIE_EINVAL (-1)       /* bad argument, shape or index            */
+#define PIE_EHIP (-2)         /* HIP runtime error (no device, OOM, ...) */
+#define PIE_EUNSUPPORTED (-3) /* combination not offered                  */
+#define PIE_ESTATE (-4)       /* call out of order (step before reset)    */
+
+/* Create a solver on the current HIP device.  param_a/param_b are the
+ * reference constructor arguments (EquSolver: block_size, unused;
+ * GridSolver: grid_x, grid_y, unused: the GPU tile is autotuned).
+ * Replaces core_cuda.EquSolver(block_size) / GridSolver(grid_x, grid_y)
+ * (fpie/core/cuda/solver.cc:4-15, fpie/process.py:174,313). */
+int pie_create(int kind, int schedule, int order, int param_a, int param_b,
+               pie_solver** out);
+
+/* Release every device and host buffer (base_solver.h:19-25). */
+int pie_destroy(pie_solver* s);
+
+/* EquSolver.partition (fpie/core/gcc/equ.cc:14-32): ids[i*w+j] = 1, 2, ...
+ * in row-major order over mask > 0, 0 elsewhere.  Records the geometry so
+ * that pie_reset_equ can solve the system on the grid layout. */
+int pie_partition(pie_solver* s, const int32_t* mask, int h, int w,
+                  int32_t* ids);
+
+/* EquSolver.reset(N, A, X, B) (fpie/core/base_solver.h:27-59): A int32[N][4]
+ * (ids of the up, down, left, right neighbours, 0 = constant), X and B
+ * float[N][3].  Every id in A must lie in [0, N). */
+int pie_reset_equ(pie_solver* s, int n, const int32_t* A, const float* X,
+                  const float* B);
+
+/* GridSolver.reset(N, mask, tgt, grad) (fpie/core/base_solver.h:101-140):
+ * mask int32[h][w] (non-zero = solve), tgt and grad float[h][w][3].  The
+ * outermost rows and columns must be unmasked (fpie/process.py:214-217
+ * guarantees it; the reference reads out of bounds otherwise). */
+int pie_reset_grid(pie_solver* s, int h, int w, const int32_t* mask,
+                   const float* tgt, const float* grad);
+
+/* sync() (base_solver.h:61): waits for queued work; no data moves. */
+int pie_sync(pie_solver* s);
+
+/* step(iteration) (gcc/equ.cc:74-86, gcc/grid.cc:57-81): run `iterations`
+ * sweeps, then write the clamped/truncated image (Equ: uint8[N][3] in id
+ * order; Grid: uint8[h][w][3]) and the per-channel sum of
+ * |b + sum(neighbours) - 4x| over the solved pixels into err[3].  Either
+ * output pointer may be NULL. */
+int pie_step(pie_solver* s, int iterations, uint8_t* img, float* err);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* pie_last_error(void);
+
+/* ---- extensions: no counterpart in the reference ---------------------- */
+
+/* Queue `iterations` sweeps on the solver's stream and return at once. */
+int pie_iterate(pie_solver* s, int iterations);
+/* The outputs of pie_step for the current state, without iterating. */
+int pie_read(pie_solver* s, uint8_t* img, float* err);
+/* Float32 state: Equ float[N][3] in id order, Grid float[h][w][3]. */
+int pie_get_state(pie_solver* s, float* out);
+/* Choose the temporal-blocking tile: sweeps per launch, columns per lane
+ * and rows per wave task (0 = keep the autotuned value). */
+int pie_set_tile(pie_solver* s, int sweeps, int cols_per_lane, int seg_rows);
+int pie_get_tile(pie_solver* s, int* sweeps, int* cols_per_lane,
+                 int* seg_rows);
+/* Time every candidate tile on the loaded problem (`launches` launches
+ * each, state untouched) and keep the fastest. */
+int pie_autotune(pie_solver* s, int launches);
+/* Event timing of the sweep kernels on the solver's stream. */
+int pie_set_timing(pie_solver* s, int enable);
+int pie_get_timing(pie_solver* s, double* kernel_ms, long long* launches,
+                   double* pixel_updates);
+/* Problem size: grid rows/cols, N (Equ ids incl. 0), solved pixels. */
+int pie_shape(pie_solver* s, int* h, int* w, int* n, long long* masked);
+/* Name and properties of the current device (cuda/utils.cu:5-22). */
+int pie_device_info(char* buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FPIE_HIP_H_ */
