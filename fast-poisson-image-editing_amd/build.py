@@ -46,10 +46,9 @@ def _compile(src: str, obj: str, defines: list[str]) -> str:
 
 def build(jobs: int | None = None, verbose: bool = True) -> str:
   os.makedirs(OBJ, exist_ok=True)
-  units = [(os.path.join(CSRC, "kernels.hip"),
-            os.path.join(OBJ, "kernels.o"), []),
-           (os.path.join(CSRC, "solver.hip"),
-            os.path.join(OBJ, "solver.o"), [])]
+  units = [(os.path.join(CSRC, f), os.path.join(OBJ, f[:-4] + ".o"), [])
+           for f in sorted(os.listdir(CSRC))
+           if f.endswith(".hip") and f != "march_inst.hip"]
   for s, c in tiles():
     units.append((os.path.join(CSRC, "march_inst.hip"),
                   os.path.join(OBJ, f"march_{s}x{c}.o"),

@@ -1,0 +1,376 @@
+// Batched GridSolver (C ABI pie_batch_*): many composites of uneven size
+// and mask, each with its own stop rule, solved together.
+//
+// Every item keeps its own padded planes (same layout as pie_solver).  One
+// round is two kernels: the persistent sweep kernel, whose waves take tiles
+// of any live item from one atomic work counter and advance them by S
+// Jacobi sweeps, and an epilogue that applies the stop rules (last sweep's
+// max |update| below the item's tolerance, or its sweep budget used) and
+// resets the counter.  R rounds are captured once into a hipGraph and the
+// graph is replayed until no item is live, so the host only looks at one
+// counter every R rounds.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "host_util.h"
+
+using pie::BatchArgs;
+using pie::BatchItem;
+using pie::TileCfg;
+using pie::host::DevBuf;
+using pie::host::fail;
+using pie::host::pitch_of;
+using pie::host::upload;
+
+namespace {
+
+struct ItemBuffers {
+  DevBuf<float> xa, xb, g, x0;  // x0: initial state, for restarts
+  DevBuf<uint8_t> m;
+  int H = 0, W = 0, P = 0;
+  long long plane = 0, masked = 0;
+  int max_sweeps = 0;
+  float tol = 0.0f;
+};
+
+}  // namespace
+
+struct pie_batch {
+  int order = PIE_ORDER_ULRD;
+  TileCfg cfg{8, 1, 64};
+  hipStream_t stream = nullptr;
+  int device = 0;
+  double wall_hz = 1e8;
+  std::vector<std::unique_ptr<ItemBuffers>> items;
+  std::vector<BatchItem> desc;  // descriptors as last read back
+  DevBuf<BatchItem> d_items;
+  DevBuf<int> d_task_item;
+  DevBuf<unsigned> d_counters;  // [0] work queue, [1] live items
+  DevBuf<unsigned long long> d_clock;
+  unsigned* h_live = nullptr;   // pinned
+  int ntask = 0;
+  bool dirty = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_rounds = 0;
+  int blocks = 0;
+  unsigned long long t_start = 0;
+  DevBuf<double> partial;
+  DevBuf<float> err;
+  DevBuf<uint8_t> img;
+};
+
+namespace {
+
+void drop_graph(pie_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->graph_rounds = 0;
+}
+
+// Descriptor table and task table for the current item list.
+int prepare(pie_batch* b) {
+  if (!b->dirty) return PIE_OK;
+  const int n = (int)b->items.size();
+  b->desc.assign(n, BatchItem{});
+  std::vector<int> task_item;
+  for (int i = 0; i < n; ++i) {
+    ItemBuffers& it = *b->items[i];
+    BatchItem& d = b->desc[i];
+    d.xa = it.xa.p + pie::kPadLeft;
+    d.xb = it.xb.p + pie::kPadLeft;
+    d.g = it.g.p + pie::kPadLeft;
+    d.m = it.m.p + pie::kPadLeft;
+    d.plane = it.plane;
+    d.H = it.H;
+    d.W = it.W;
+    d.P = it.P;
+    pie::tile_tasks(b->cfg, it.H, it.W, &d.stripes, &d.segs);
+    d.task0 = (int)task_item.size();
+    d.t_end = it.max_sweeps;
+    d.max_sweeps = it.max_sweeps;
+    d.tol = it.tol;
+    task_item.insert(task_item.end(), (size_t)d.stripes * d.segs, i);
+  }
+  b->ntask = (int)task_item.size();
+  PIE_HIP(b->d_items.alloc(std::max(n, 1)));
+  PIE_HIP(b->d_task_item.alloc(std::max(b->ntask, 1)));
+  if (b->ntask)
+    PIE_CHECK(upload(task_item.data(), sizeof(int) * task_item.size(),
+                     b->d_task_item.p, b->stream));
+  int per_cu = pie::batch_blocks_per_cu(b->cfg, PIE_JACOBI, b->order);
+  hipDeviceProp_t prop;
+  PIE_HIP(hipGetDeviceProperties(&prop, b->device));
+  b->blocks = std::max(1, per_cu * prop.multiProcessorCount);
+  drop_graph(b);
+  b->dirty = false;
+  return PIE_OK;
+}
+
+// Every item back to its initial state and stop-rule bookkeeping.
+int restart(pie_batch* b) {
+  for (size_t i = 0; i < b->items.size(); ++i) {
+    ItemBuffers& it = *b->items[i];
+    PIE_HIP(hipMemcpyAsync(it.xa.p, it.x0.p, 3 * it.plane * sizeof(float),
+                           hipMemcpyDeviceToDevice, b->stream));
+    BatchItem& d = b->desc[i];
+    d.launches = d.sweeps = d.done = 0;
+    d.dmax_bits = 0u;
+    d.last_dmax = 0.0f;
+    d.t_done = 0ull;
+  }
+  if (!b->desc.empty())
+    PIE_CHECK(upload(b->desc.data(), sizeof(BatchItem) * b->desc.size(),
+                     b->d_items.p, b->stream));
+  PIE_HIP(hipMemsetAsync(b->d_counters.p, 0, 2 * sizeof(unsigned), b->stream));
+  return PIE_OK;
+}
+
+int capture(pie_batch* b, int rounds) {
+  if (b->exec && b->graph_rounds == rounds) return PIE_OK;
+  drop_graph(b);
+  BatchArgs a{};
+  a.items = b->d_items.p;
+  a.task_item = b->d_task_item.p;
+  a.queue = b->d_counters.p;
+  a.ntask = b->ntask;
+  a.seg_rows = b->cfg.seg_rows;
+  PIE_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipSuccess;
+  for (int r = 0; r < rounds && e == hipSuccess; ++r) {
+    e = pie::launch_batch(b->cfg, PIE_JACOBI, b->order, a, b->blocks,
+                          b->stream);
+    if (e == hipSuccess)
+      e = pie::launch_batch_epilogue(b->d_items.p, (int)b->items.size(),
+                                     b->cfg.S, b->d_counters.p,
+                                     b->d_counters.p + 1, b->stream);
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(PIE_EHIP, std::string("batch graph capture: ") +
+                              hipGetErrorString(e != hipSuccess ? e : e2));
+  }
+  b->graph = g;
+  PIE_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  b->graph_rounds = rounds;
+  return PIE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pie_batch_create(int order, int sweeps_per_launch, int cols_per_lane,
+                     int seg_rows, pie_batch** out) {
+  if (!out) return fail(PIE_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (order != PIE_ORDER_UDLR && order != PIE_ORDER_ULRD)
+    return fail(PIE_EINVAL, "unknown neighbour order");
+  TileCfg cfg{sweeps_per_launch > 0 ? sweeps_per_launch : 8,
+              cols_per_lane > 0 ? cols_per_lane : 1,
+              seg_rows > 0 ? seg_rows : 64};
+  if (!pie::tile_supported(cfg.S, cfg.C))
+    return fail(PIE_EUNSUPPORTED, "tile not compiled");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
+    return fail(PIE_EHIP, "no HIP device");
+  std::unique_ptr<pie_batch> b(new pie_batch());
+  b->order = order;
+  b->cfg = cfg;
+  PIE_HIP(hipGetDevice(&b->device));
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate,
+                            b->device) == hipSuccess && khz > 0)
+    b->wall_hz = khz * 1e3;
+  PIE_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  PIE_HIP(b->d_counters.alloc(2));
+  PIE_HIP(b->d_clock.alloc(1));
+  PIE_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->h_live), sizeof(unsigned),
+                        hipHostMallocDefault));
+  PIE_HIP(b->err.alloc(3));
+  *out = b.release();
+  return PIE_OK;
+}
+
+int pie_batch_destroy(pie_batch* b) {
+  if (!b) return PIE_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_graph(b);
+  if (b->h_live) (void)hipHostFree(b->h_live);
+  b->items.clear();
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return PIE_OK;
+}
+
+int pie_batch_add(pie_batch* b, int h, int w, const int32_t* mask,
+                  const float* tgt, const float* grad, int max_sweeps,
+                  float tol, int* index) {
+  if (!b || !mask || !tgt || !grad) return fail(PIE_EINVAL, "NULL argument");
+  if (h < 3 || w < 3) return fail(PIE_EINVAL, "grid smaller than 3 x 3");
+  if (max_sweeps < 0) return fail(PIE_EINVAL, "negative sweep budget");
+  std::unique_ptr<ItemBuffers> it(new ItemBuffers());
+  it->H = h;
+  it->W = w;
+  it->P = pitch_of(w);
+  it->plane = (long long)h * it->P;
+  if (it->plane > (1LL << 31) - 1)
+    return fail(PIE_EINVAL, "grid larger than 2^31 elements per plane");
+  it->max_sweeps = max_sweeps;
+  it->tol = tol;
+  const size_t fl = 3 * (size_t)it->plane;
+  PIE_HIP(it->xa.alloc(fl));
+  PIE_HIP(it->xb.alloc(fl));
+  PIE_HIP(it->g.alloc(fl));
+  PIE_HIP(it->x0.alloc(fl));
+  PIE_HIP(it->m.alloc(it->plane));
+  PIE_HIP(hipMemsetAsync(it->x0.p, 0, fl * sizeof(float), b->stream));
+  PIE_HIP(hipMemsetAsync(it->g.p, 0, fl * sizeof(float), b->stream));
+  PIE_HIP(hipMemsetAsync(it->m.p, 0, it->plane, b->stream));
+  const size_t px = (size_t)h * w;
+  DevBuf<char> stage;
+  PIE_HIP(stage.alloc(px * 12));
+  DevBuf<unsigned long long> counts;
+  PIE_HIP(counts.alloc(2));
+  PIE_HIP(hipMemsetAsync(counts.p, 0, 2 * sizeof(unsigned long long),
+                         b->stream));
+  PIE_CHECK(upload(mask, px * 4, stage.p, b->stream));
+  PIE_HIP(pie::launch_mask(reinterpret_cast<const int32_t*>(stage.p),
+                           it->m.p + pie::kPadLeft, h, w, it->P, counts.p,
+                           b->stream));
+  unsigned long long cnt[2] = {0, 0};
+  PIE_HIP(hipMemcpyAsync(cnt, counts.p, sizeof(cnt), hipMemcpyDeviceToHost,
+                         b->stream));
+  PIE_HIP(hipStreamSynchronize(b->stream));
+  if (cnt[1])
+    return fail(PIE_EINVAL,
+                "mask has solved pixels on the outermost rows/columns");
+  it->masked = (long long)cnt[0];
+  PIE_CHECK(upload(tgt, px * 12, stage.p, b->stream));
+  PIE_HIP(pie::launch_deinterleave(reinterpret_cast<const float*>(stage.p),
+                                   it->x0.p + pie::kPadLeft, it->plane, h, w,
+                                   it->P, b->stream));
+  PIE_CHECK(upload(grad, px * 12, stage.p, b->stream));
+  PIE_HIP(pie::launch_deinterleave(reinterpret_cast<const float*>(stage.p),
+                                   it->g.p + pie::kPadLeft, it->plane, h, w,
+                                   it->P, b->stream));
+  // both state buffers start as the initial state (padding included)
+  PIE_HIP(hipMemcpyAsync(it->xa.p, it->x0.p, fl * sizeof(float),
+                         hipMemcpyDeviceToDevice, b->stream));
+  PIE_HIP(hipMemcpyAsync(it->xb.p, it->x0.p, fl * sizeof(float),
+                         hipMemcpyDeviceToDevice, b->stream));
+  PIE_HIP(hipStreamSynchronize(b->stream));
+  if (index) *index = (int)b->items.size();
+  b->items.push_back(std::move(it));
+  b->dirty = true;
+  return PIE_OK;
+}
+
+int pie_batch_solve(pie_batch* b, int rounds_per_graph, double* seconds,
+                    int* rounds_run) {
+  if (!b) return fail(PIE_EINVAL, "NULL batch");
+  if (b->items.empty()) return fail(PIE_ESTATE, "empty batch");
+  const int R = rounds_per_graph > 0 ? rounds_per_graph : 16;
+  PIE_CHECK(prepare(b));
+  PIE_CHECK(restart(b));
+  PIE_CHECK(capture(b, R));
+  int max_rounds = 1;
+  for (auto& it : b->items)
+    max_rounds = std::max(max_rounds,
+                          (it->max_sweeps + b->cfg.S - 1) / b->cfg.S);
+  PIE_HIP(pie::launch_clock(b->d_clock.p, b->stream));
+  PIE_HIP(hipStreamSynchronize(b->stream));
+  const auto t0 = std::chrono::steady_clock::now();
+  int rounds = 0;
+  // every item stops within max_rounds rounds (sweep budget), so the loop
+  // ends even if no tolerance is ever met
+  while (rounds < max_rounds) {
+    PIE_HIP(hipGraphLaunch(b->exec, b->stream));
+    rounds += R;
+    PIE_HIP(hipMemcpyAsync(b->h_live, b->d_counters.p + 1, sizeof(unsigned),
+                           hipMemcpyDeviceToHost, b->stream));
+    PIE_HIP(hipStreamSynchronize(b->stream));
+    if (*b->h_live == 0) break;
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+  if (rounds_run) *rounds_run = rounds;
+  PIE_HIP(hipMemcpyAsync(b->desc.data(), b->d_items.p,
+                         sizeof(BatchItem) * b->desc.size(),
+                         hipMemcpyDeviceToHost, b->stream));
+  PIE_HIP(hipMemcpyAsync(&b->t_start, b->d_clock.p, sizeof(b->t_start),
+                         hipMemcpyDeviceToHost, b->stream));
+  PIE_HIP(hipStreamSynchronize(b->stream));
+  return PIE_OK;
+}
+
+int pie_batch_item(pie_batch* b, int i, int* sweeps, float* last_dmax,
+                   double* latency_s) {
+  if (!b) return fail(PIE_EINVAL, "NULL batch");
+  if (i < 0 || i >= (int)b->desc.size())
+    return fail(PIE_EINVAL, "item index out of range (or not solved yet)");
+  const BatchItem& d = b->desc[i];
+  if (sweeps) *sweeps = d.sweeps;
+  if (last_dmax) *last_dmax = d.last_dmax;
+  if (latency_s)
+    *latency_s = d.done ? (double)(d.t_done - b->t_start) / b->wall_hz : -1.0;
+  return PIE_OK;
+}
+
+int pie_batch_read(pie_batch* b, int i, uint8_t* img, float* err,
+                   float* state) {
+  if (!b) return fail(PIE_EINVAL, "NULL batch");
+  if (i < 0 || i >= (int)b->items.size())
+    return fail(PIE_EINVAL, "item index out of range");
+  ItemBuffers& it = *b->items[i];
+  const bool odd = i < (int)b->desc.size() && (b->desc[i].launches & 1);
+  const float* x = (odd ? it.xb.p : it.xa.p) + pie::kPadLeft;
+  const int nb = pie::finalize_blocks(it.H, it.W);
+  PIE_HIP(b->partial.alloc(3 * (size_t)nb));
+  DevBuf<uint8_t> dimg;
+  if (img) PIE_HIP(dimg.alloc((size_t)it.H * it.W * 3));
+  PIE_HIP(pie::launch_finalize(b->order, x, it.g.p + pie::kPadLeft,
+                               it.m.p + pie::kPadLeft, it.plane, it.H, it.W,
+                               it.P, img ? dimg.p : nullptr, b->partial.p, nb,
+                               b->stream));
+  PIE_HIP(pie::launch_reduce(b->partial.p, nb, b->err.p, b->stream));
+  if (img)
+    PIE_HIP(hipMemcpyAsync(img, dimg.p, (size_t)it.H * it.W * 3,
+                           hipMemcpyDeviceToHost, b->stream));
+  DevBuf<float> st;
+  if (state) {
+    PIE_HIP(st.alloc((size_t)it.H * it.W * 3));
+    PIE_HIP(pie::launch_interleave(x, st.p, it.plane, it.H, it.W, it.P,
+                                   b->stream));
+    PIE_HIP(hipMemcpyAsync(state, st.p, (size_t)it.H * it.W * 3 * 4,
+                           hipMemcpyDeviceToHost, b->stream));
+  }
+  float e[3];
+  PIE_HIP(hipMemcpyAsync(e, b->err.p, sizeof(e), hipMemcpyDeviceToHost,
+                         b->stream));
+  PIE_HIP(hipStreamSynchronize(b->stream));
+  if (err) std::memcpy(err, e, sizeof(e));
+  return PIE_OK;
+}
+
+int pie_batch_size(pie_batch* b, int* items, int* tasks, long long* masked) {
+  if (!b) return fail(PIE_EINVAL, "NULL batch");
+  PIE_CHECK(prepare(b));
+  long long total = 0;
+  for (auto& it : b->items) total += it->masked;
+  if (items) *items = (int)b->items.size();
+  if (tasks) *tasks = b->ntask;
+  if (masked) *masked = total;
+  return PIE_OK;
+}
+
+}  // extern "C"

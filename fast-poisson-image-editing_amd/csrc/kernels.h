@@ -1,17 +1,24 @@
 // Device kernels of the Poisson solver and their host-side launchers.
 //
-// Layout in HBM: a solve works on a pitched grid of H rows x W columns
-// (pitch P elements, P a multiple of 64).  The float32 RGB state is kept as
-// three planes (channel c at offset c * plane), the guidance (grad for the
-// GridSolver, b for the EquSolver) likewise, and the mask as one uint8
-// plane.  Planar float rows make every row load of a wavefront one
-// contiguous 256-byte (or 512-byte) transaction.
+// Layout in HBM: a solve works on a pitched grid of H rows x W columns.
+// Each row holds kPadLeft padding elements, the W values and at least
+// kPadRight more padding (pitch P, a multiple of 64); the base pointers the
+// kernels get point at column 0 of row 0, so element (i, j) is base[i*P+j]
+// for j in [-kPadLeft, P - kPadLeft).  The float32 RGB state is kept as
+// three planes (channel c at offset c * plane, plane = H * P), the guidance
+// (grad for the GridSolver, b for the EquSolver) likewise, and the mask as
+// one uint8 plane.  Planar float rows make every row load of a wavefront one
+// contiguous 256-byte (or 512-byte) transaction, and the padding lets the
+// sweep kernel load whole stripes without bounds tests.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace pie {
+
+constexpr int kPadLeft = 64;
+constexpr int kPadRight = 256;  // >= 64 * (largest columns per lane)
 
 enum Sched { SCHED_JACOBI = 0, SCHED_GS_LEX = 1, SCHED_REDBLACK = 2 };
 enum Order { ORDER_UDLR = 0, ORDER_ULRD = 1 };
@@ -29,6 +36,35 @@ struct MarchArgs {
   int H, W, P;
   int t0, t_end, band;
   int stripes, segs, seg_rows, ntask;
+  unsigned* dmax;  // optional: max |change| of the last step (float bits)
+};
+
+// One composite of a batch.  The host fills the geometry; the device keeps
+// the solve state (launches, sweeps, dmax_bits, done, t_done) up to date.
+struct BatchItem {
+  float* xa;  // column-0 base pointers of the two state buffers
+  float* xb;
+  const float* g;
+  const uint8_t* m;
+  long long plane;
+  int H, W, P;
+  int stripes, segs, task0;  // wave tasks of the item, first global task
+  int t_end;                 // sweeps never run past this step count
+  int max_sweeps;            // stop rule: sweep budget
+  float tol;                 // stop rule: max |update| of the last sweep
+  int launches;              // sweep launches done (parity of the state)
+  int sweeps;                // sweeps done
+  int done;                  // 1 once a stop rule fired
+  unsigned dmax_bits;        // max |update| of the launch in flight (bits)
+  float last_dmax;           // max |update| of the last finished launch
+  unsigned long long t_done; // device wall clock when the item stopped
+};
+
+struct BatchArgs {
+  BatchItem* items;
+  const int* task_item;  // item of every global task
+  unsigned* queue;       // work counter, reset by the epilogue
+  int ntask, seg_rows;
 };
 
 // Temporal-blocking tile: S sweeps per launch, C columns per lane
@@ -43,7 +79,10 @@ struct TileCfg {
 #define PIE_TILES(X) X(4, 1) X(8, 1) X(4, 2) X(8, 2)
 #define PIE_DECL(s, c)                                             \
   hipError_t launch_march_##s##_##c(int sched, int order, const MarchArgs& a, \
-                                    hipStream_t st);
+                                    hipStream_t st);                         \
+  hipError_t launch_batch_##s##_##c(int sched, int order, const BatchArgs& b, \
+                                    int blocks, hipStream_t st);             \
+  int batch_blocks_per_cu_##s##_##c(int sched, int order);
 PIE_TILES(PIE_DECL)
 #undef PIE_DECL
 
@@ -55,6 +94,18 @@ void tile_tasks(const TileCfg& cfg, int H, int W, int* stripes, int* segs);
 
 hipError_t launch_march(const TileCfg& cfg, int sched, int order,
                         MarchArgs a, hipStream_t st);
+
+// Persistent batched sweep kernel (S Jacobi sweeps of every live item) and
+// its epilogue (advance live items, apply the stop rules, reset the work
+// counter, count live items into *live).
+hipError_t launch_batch(const TileCfg& cfg, int sched, int order,
+                        const BatchArgs& b, int blocks, hipStream_t st);
+int batch_blocks_per_cu(const TileCfg& cfg, int sched, int order);
+hipError_t launch_batch_epilogue(BatchItem* items, int n, int S,
+                                 unsigned* queue, unsigned* live,
+                                 hipStream_t st);
+// *out = device wall clock (for per-item latencies)
+hipError_t launch_clock(unsigned long long* out, hipStream_t st);
 
 // float[H][W][3] (interleaved) -> three pitched planes
 hipError_t launch_deinterleave(const float* src, float* dst, long long plane,

@@ -269,6 +269,46 @@ __global__ void __launch_bounds__(kBlock)
     for (int c = 0; c < 3; ++c) partial[blockIdx.x * 3 + c] = acc[c];
 }
 
+// After each batched launch: advance every live item by S sweeps, stop it
+// when its last sweep changed no value by tol or more, or when its sweep
+// budget is used up; count the live items and reset the work counter.
+__global__ void __launch_bounds__(kBlock)
+    batch_epilogue_kernel(BatchItem* items, int n, int S, unsigned* queue,
+                          unsigned* live) {
+  __shared__ unsigned cnt[kBlock];
+  const unsigned long long now = wall_clock64();
+  unsigned c = 0;
+  for (int i = threadIdx.x; i < n; i += kBlock) {
+    BatchItem& it = items[i];
+    if (it.done) continue;
+    it.launches += 1;
+    it.sweeps = min(it.sweeps + S, it.t_end);
+    const float d = __uint_as_float(it.dmax_bits);
+    it.last_dmax = d;
+    if (d < it.tol || it.sweeps >= it.max_sweeps) {
+      it.done = 1;
+      it.t_done = now;
+    } else {
+      ++c;
+    }
+    it.dmax_bits = 0u;
+  }
+  cnt[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) cnt[threadIdx.x] += cnt[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    *live = cnt[0];
+    *queue = 0u;
+  }
+}
+
+__global__ void clock_kernel(unsigned long long* out) {
+  if (threadIdx.x == 0) *out = wall_clock64();
+}
+
 int grid_blocks(long long n, int cap) {
   long long b = (n + kBlock - 1) / kBlock;
   if (b < 1) b = 1;
@@ -316,6 +356,37 @@ hipError_t launch_march(const TileCfg& cfg, int sched, int order, MarchArgs a,
   PIE_TILES(PIE_T)
 #undef PIE_T
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_batch(const TileCfg& cfg, int sched, int order,
+                        const BatchArgs& b, int blocks, hipStream_t st) {
+#define PIE_T(s, c)                \
+  if (cfg.S == s && cfg.C == c)    \
+    return launch_batch_##s##_##c(sched, order, b, blocks, st);
+  PIE_TILES(PIE_T)
+#undef PIE_T
+  return hipErrorInvalidValue;
+}
+
+int batch_blocks_per_cu(const TileCfg& cfg, int sched, int order) {
+#define PIE_T(s, c) \
+  if (cfg.S == s && cfg.C == c) return batch_blocks_per_cu_##s##_##c(sched, order);
+  PIE_TILES(PIE_T)
+#undef PIE_T
+  return 1;
+}
+
+hipError_t launch_batch_epilogue(BatchItem* items, int n, int S,
+                                 unsigned* queue, unsigned* live,
+                                 hipStream_t st) {
+  hipLaunchKernelGGL(batch_epilogue_kernel, dim3(1), dim3(kBlock), 0, st, items,
+                     n, S, queue, live);
+  return hipGetLastError();
+}
+
+hipError_t launch_clock(unsigned long long* out, hipStream_t st) {
+  hipLaunchKernelGGL(clock_kernel, dim3(1), dim3(kWave), 0, st, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_deinterleave(const float* src, float* dst, long long plane,

@@ -57,6 +57,14 @@ __device__ __forceinline__ float from_right(float v) {
                                          0xF, 0xF, false));
 }
 
+// Maximum of a 32-bit value over the wavefront.
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1)
+    v = max(v, (unsigned)__shfl_xor((int)v, off, kWave));
+  return v;
+}
+
 // Workgroups are dealt to the 8 XCDs round-robin; renumber them so that each
 // XCD gets one contiguous range of tasks, which keeps the halo columns and
 // rows shared by neighbouring tasks inside one L2.
@@ -66,91 +74,75 @@ __device__ __forceinline__ int xcd_block() {
   return x * q + (x < r ? x : r) + k;
 }
 
+// Rows are padded (kPadLeft zero columns before column 0, at least
+// kPadRight after column W-1, see kernels.h), so a stripe's vector load at
+// col0 in [-S, W + 64C) always stays inside the row: loads need no per-lane
+// bounds test, and a prefetched value is not forced through a branch.
+// Padding and clamped rows only ever feed halo lanes or unsolved pixels.
 template <int C>
 __device__ __forceinline__ void load_row(const float* __restrict__ base,
-                                         long long plane, int P, int H, int W,
-                                         int row, int col0, float (&v)[C][3]) {
-  if (row < H) {
-    const float* p = base + (long long)row * P + col0;
-    if (col0 >= 0 && col0 + C <= W) {
+                                         long long plane, int P, int row,
+                                         int col0, float (&v)[C][3]) {
+  const float* p = base + (long long)row * P + col0;
 #pragma unroll
-      for (int ch = 0; ch < 3; ++ch) {
-        if constexpr (C == 2) {
-          const float2 q = *reinterpret_cast<const float2*>(p + ch * plane);
-          v[0][ch] = q.x;
-          v[1][ch] = q.y;
-        } else if constexpr (C == 4) {
-          const float4 q = *reinterpret_cast<const float4*>(p + ch * plane);
-          v[0][ch] = q.x;
-          v[1][ch] = q.y;
-          v[2][ch] = q.z;
-          v[3][ch] = q.w;
-        } else {
-#pragma unroll
-          for (int cc = 0; cc < C; ++cc) v[cc][ch] = p[ch * plane + cc];
-        }
-      }
-      return;
+  for (int ch = 0; ch < 3; ++ch) {
+    if constexpr (C == 2) {
+      const float2 q = *reinterpret_cast<const float2*>(p + ch * plane);
+      v[0][ch] = q.x;
+      v[1][ch] = q.y;
+    } else if constexpr (C == 4) {
+      const float4 q = *reinterpret_cast<const float4*>(p + ch * plane);
+      v[0][ch] = q.x;
+      v[1][ch] = q.y;
+      v[2][ch] = q.z;
+      v[3][ch] = q.w;
+    } else {
+      v[0][ch] = p[ch * plane];
     }
-#pragma unroll
-    for (int cc = 0; cc < C; ++cc) {
-      const int col = col0 + cc;
-      const bool ok = col >= 0 && col < W;
-#pragma unroll
-      for (int ch = 0; ch < 3; ++ch) v[cc][ch] = ok ? p[ch * plane + cc] : 0.0f;
-    }
-    return;
   }
-#pragma unroll
-  for (int cc = 0; cc < C; ++cc)
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) v[cc][ch] = 0.0f;
 }
 
+// Mask bits of C adjacent columns (mask bytes are 0 or 1).
 template <int C>
 __device__ __forceinline__ uint32_t load_mask(const uint8_t* __restrict__ m,
-                                              int P, int H, int W, int row,
-                                              int col0) {
-  uint32_t bits = 0;
-  if (row < H) {
-    const uint8_t* p = m + (long long)row * P + col0;
-#pragma unroll
-    for (int cc = 0; cc < C; ++cc) {
-      const int col = col0 + cc;
-      if (col >= 0 && col < W && p[cc]) bits |= 1u << cc;
-    }
+                                              int P, int row, int col0) {
+  const uint8_t* p = m + (long long)row * P + col0;
+  if constexpr (C == 4) {
+    const uint32_t v = *reinterpret_cast<const uint32_t*>(p);
+    return (v & 1u) | ((v >> 7) & 2u) | ((v >> 14) & 4u) | ((v >> 21) & 8u);
+  } else if constexpr (C == 2) {
+    const uint32_t v = *reinterpret_cast<const uint16_t*>(p);
+    return (v & 1u) | ((v >> 7) & 2u);
+  } else {
+    return *p;
   }
-  return bits;
 }
 
+// Stores may land in the right padding (col >= W): those values are finite
+// and only ever read by halo lanes.
 template <int C>
 __device__ __forceinline__ void store_row(float* __restrict__ base,
-                                          long long plane, int P, int W,
-                                          int row, int col0,
-                                          const float (&v)[C][3]) {
+                                          long long plane, int P, int row,
+                                          int col0, const float (&v)[C][3]) {
   float* p = base + (long long)row * P + col0;
-  if (col0 + C <= W) {
 #pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-      if constexpr (C == 2) {
-        *reinterpret_cast<float2*>(p + ch * plane) =
-            make_float2(v[0][ch], v[1][ch]);
-      } else if constexpr (C == 4) {
-        *reinterpret_cast<float4*>(p + ch * plane) =
-            make_float4(v[0][ch], v[1][ch], v[2][ch], v[3][ch]);
-      } else {
-#pragma unroll
-        for (int cc = 0; cc < C; ++cc) p[ch * plane + cc] = v[cc][ch];
-      }
+  for (int ch = 0; ch < 3; ++ch) {
+    if constexpr (C == 2) {
+      *reinterpret_cast<float2*>(p + ch * plane) =
+          make_float2(v[0][ch], v[1][ch]);
+    } else if constexpr (C == 4) {
+      *reinterpret_cast<float4*>(p + ch * plane) =
+          make_float4(v[0][ch], v[1][ch], v[2][ch], v[3][ch]);
+    } else {
+      p[ch * plane] = v[0][ch];
     }
-    return;
   }
-#pragma unroll
-  for (int cc = 0; cc < C; ++cc)
-    if (col0 + cc < W)
-#pragma unroll
-      for (int ch = 0; ch < 3; ++ch) p[ch * plane + cc] = v[cc][ch];
 }
+
+// Row steps of look-ahead of the sweep kernel's loads (divides S, so the
+// ring slot of every unrolled step is a compile-time constant).
+template <int S>
+constexpr int kPrefetch = S % 4 == 0 ? 4 : 2;
 
 template <bool B, class T>
 __device__ __forceinline__ T* pick(T* x, T* y) {
@@ -160,16 +152,16 @@ __device__ __forceinline__ T* pick(T* x, T* y) {
     return x;
 }
 
+// One wave task: column stripe `stripe`, row range `seg`, S steps.  When
+// dmax is not null the largest |change| of the last step over the pixels
+// the task writes is merged into *dmax (float bits, non-negative).
 template <int S, int C, int SCHED, int ORDER>
-__global__ void __launch_bounds__(kBlock) march_kernel(MarchArgs a) {
+__device__ __forceinline__ void march_task(const MarchArgs& a, int stripe,
+                                           int seg, unsigned* dmax) {
   static_assert(S % 2 == 0 && S % C == 0, "S even and a multiple of C");
   static_assert(S * C <= 32, "mask queue is one 32-bit word");
   constexpr int U = kWave * C - 2 * S;  // written columns per stripe
   const int lane = threadIdx.x & (kWave - 1);
-  const int task = xcd_block() * kBlockWaves + (threadIdx.x / kWave);
-  if (task >= a.ntask) return;  // whole wave leaves together
-  const int stripe = task % a.stripes;
-  const int seg = task / a.stripes;
   const int col0 = stripe * U - S + lane * C;
   const int r0 = seg * a.seg_rows;
   const int r1 = min(r0 + a.seg_rows, a.H);
@@ -183,13 +175,14 @@ __global__ void __launch_bounds__(kBlock) march_kernel(MarchArgs a) {
     if (r0 + c_lo > t_last || (r1 - 1) + c_hi <= a.t0 - a.band) {
       for (int r = r0; r < r1; ++r) {
         float v[C][3];
-        load_row<C>(a.xin, a.plane, a.P, a.H, a.W, r, col0, v);
-        if (lane_out) store_row<C>(a.xout, a.plane, a.P, a.W, r, col0, v);
+        load_row<C>(a.xin, a.plane, a.P, r, col0, v);
+        if (lane_out) store_row<C>(a.xout, a.plane, a.P, r, col0, v);
       }
       return;
     }
   }
 
+  float dm = 0.0f;
   const int rb = max(0, r0 - S);
   const int nsteps = ((r1 - 1 + S) - rb + 1 + S - 1) / S * S;
 
@@ -206,15 +199,40 @@ __global__ void __launch_bounds__(kBlock) march_kernel(MarchArgs a) {
       for (int ch = 0; ch < 3; ++ch)
         wa[j][cc][ch] = wb[j][cc][ch] = gq[j][cc][ch] = 0.0f;
 
+  // Rows are loaded PF row steps before they are used (a register ring of
+  // PF slots), so each wave keeps PF rows of loads in flight instead of
+  // waiting for every row it consumes.
+  constexpr int PF = kPrefetch<S>;
+  float px[PF][C][3], pg[PF][C][3];
+  uint32_t pm[PF];
+  static_for<0, PF>([&](auto pc) {
+    constexpr int q = decltype(pc)::value;
+    const int row = min(rb + q, a.H - 1);
+    load_row<C>(a.xin, a.plane, a.P, row, col0, px[q]);
+    load_row<C>(a.g, a.plane, a.P, row, col0, pg[q]);
+    pm[q] = rb + q < a.H ? load_mask<C>(a.m, a.P, row, col0) : 0u;
+  });
+
   for (int base = 0; base < nsteps; base += S) {
     static_for<0, S>([&](auto uc) {
       constexpr int u = decltype(uc)::value;
+      constexpr int q = u % PF;
       const int rho = rb + base + u;
-      float fr[C][3];
-      load_row<C>(a.xin, a.plane, a.P, a.H, a.W, rho, col0, fr);
-      float gn[C][3];
-      load_row<C>(a.g, a.plane, a.P, a.H, a.W, rho, col0, gn);
-      const uint32_t mn = load_mask<C>(a.m, a.P, a.H, a.W, rho, col0);
+      float fr[C][3], gn[C][3];
+#pragma unroll
+      for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+          fr[cc][ch] = px[q][cc][ch];
+          gn[cc][ch] = pg[q][cc][ch];
+        }
+      const uint32_t mn = pm[q];
+      {
+        const int row = min(rho + PF, a.H - 1);  // wave-uniform
+        load_row<C>(a.xin, a.plane, a.P, row, col0, px[q]);
+        load_row<C>(a.g, a.plane, a.P, row, col0, pg[q]);
+        pm[q] = rho + PF < a.H ? load_mask<C>(a.m, a.P, row, col0) : 0u;
+      }
 
       static_for<1, S + 1>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
@@ -224,48 +242,45 @@ __global__ void __launch_bounds__(kBlock) march_kernel(MarchArgs a) {
         float(&newer)[C][3] = *pick<(u & 1) != 0>(&wb[j], &wa[j]);
         const int r = rho - k;
         const int t = a.t0 + j;
+        // Steps past t_end keep their value; folding that (wave-uniform)
+        // test into the predicate keeps the level free of branches, so
+        // the window registers are renamed rather than copied.
+        const bool live = t < a.t_end;
         float out[C][3];
-        if (t < a.t_end) {
-          const uint32_t mrow = mq >> (slot * C);
+        const uint32_t mrow = mq >> (slot * C);
 #pragma unroll
-          for (int cc = 0; cc < C; ++cc) {
-            const int col = col0 + cc;
-            bool upd = (mrow >> cc) & 1u;
-            if constexpr (SCHED == SCHED_REDBLACK) {
-              upd = upd && ((r + col + t) & 1);
-            } else if constexpr (SCHED == SCHED_GS_LEX) {
-              const int d = t - r - col;
-              upd = upd && d >= 0 && d < a.band && !(d & 1);
-            }
-#pragma unroll
-            for (int ch = 0; ch < 3; ++ch) {
-              const float ctr = newer[cc][ch];
-              const float up = older[cc][ch];
-              const float dn = fr[cc][ch];
-              const float lf =
-                  cc > 0 ? newer[cc > 0 ? cc - 1 : 0][ch] : from_left(newer[C - 1][ch]);
-              const float rt = cc < C - 1 ? newer[cc < C - 1 ? cc + 1 : 0][ch]
-                                          : from_right(newer[0][ch]);
-              float s = gq[slot][cc][ch];
-              if constexpr (ORDER == ORDER_UDLR) {
-                s = s + up;
-                s = s + dn;
-                s = s + lf;
-                s = s + rt;
-              } else {
-                s = s + up;
-                s = s + lf;
-                s = s + rt;
-                s = s + dn;
-              }
-              out[cc][ch] = upd ? s * 0.25f : ctr;
-            }
+        for (int cc = 0; cc < C; ++cc) {
+          const int col = col0 + cc;
+          bool upd = live && ((mrow >> cc) & 1u);
+          if constexpr (SCHED == SCHED_REDBLACK) {
+            upd = upd && ((r + col + t) & 1);
+          } else if constexpr (SCHED == SCHED_GS_LEX) {
+            const int d = t - r - col;
+            upd = upd && d >= 0 && d < a.band && !(d & 1);
           }
-        } else {
 #pragma unroll
-          for (int cc = 0; cc < C; ++cc)
-#pragma unroll
-            for (int ch = 0; ch < 3; ++ch) out[cc][ch] = newer[cc][ch];
+          for (int ch = 0; ch < 3; ++ch) {
+            const float ctr = newer[cc][ch];
+            const float up = older[cc][ch];
+            const float dn = fr[cc][ch];
+            const float lf = cc > 0 ? newer[cc > 0 ? cc - 1 : 0][ch]
+                                    : from_left(newer[C - 1][ch]);
+            const float rt = cc < C - 1 ? newer[cc < C - 1 ? cc + 1 : 0][ch]
+                                        : from_right(newer[0][ch]);
+            float s = gq[slot][cc][ch];
+            if constexpr (ORDER == ORDER_UDLR) {
+              s = s + up;
+              s = s + dn;
+              s = s + lf;
+              s = s + rt;
+            } else {
+              s = s + up;
+              s = s + lf;
+              s = s + rt;
+              s = s + dn;
+            }
+            out[cc][ch] = upd ? s * 0.25f : ctr;
+          }
         }
 #pragma unroll
         for (int cc = 0; cc < C; ++cc)
@@ -277,8 +292,19 @@ __global__ void __launch_bounds__(kBlock) march_kernel(MarchArgs a) {
       });
 
       const int rs = rho - S;
-      if (lane_out && rs >= r0 && rs < r1)
-        store_row<C>(a.xout, a.plane, a.P, a.W, rs, col0, fr);
+      if (lane_out && rs >= r0 && rs < r1) {
+        store_row<C>(a.xout, a.plane, a.P, rs, col0, fr);
+        if (dmax) {
+          // level S - 1 of row rs is still the newer row of that level
+          const float(&prev)[C][3] =
+              *pick<(u & 1) != 0>(&wb[S - 1], &wa[S - 1]);
+#pragma unroll
+          for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch)
+              dm = fmaxf(dm, fabsf(fr[cc][ch] - prev[cc][ch]));
+        }
+      }
 
       // the guidance and mask of row rho enter slot u only now: level S of
       // this step still needed the row that slot held (rho - S)
@@ -288,6 +314,56 @@ __global__ void __launch_bounds__(kBlock) march_kernel(MarchArgs a) {
         for (int ch = 0; ch < 3; ++ch) gq[u][cc][ch] = gn[cc][ch];
       mq = (mq & ~(((1u << C) - 1u) << (u * C))) | (mn << (u * C));
     });
+  }
+  if (dmax) {
+    // differences are non-negative, so float order is unsigned-int order
+    const unsigned wave_max = wave_max_u32(__float_as_uint(dm));
+    if (lane == 0 && wave_max) atomicMax(dmax, wave_max);
+  }
+}
+
+template <int S, int C, int SCHED, int ORDER>
+__global__ void __launch_bounds__(kBlock) march_kernel(MarchArgs a) {
+  const int task = xcd_block() * kBlockWaves + (threadIdx.x / kWave);
+  if (task >= a.ntask) return;  // whole wave leaves together
+  march_task<S, C, SCHED, ORDER>(a, task % a.stripes, task / a.stripes,
+                                 a.dmax);
+}
+
+// Persistent sweep kernel of a batch: every wave takes tasks (tiles of any
+// item) from one atomic work counter until the counter passes the total,
+// so uneven items share the machine.  Items already stopped are skipped.
+// Each item's launch count selects its input buffer and first step.
+template <int S, int C, int SCHED, int ORDER>
+__global__ void __launch_bounds__(kBlock) march_batch_kernel(BatchArgs b) {
+  const int lane = threadIdx.x & (kWave - 1);
+  for (;;) {
+    int t = 0;
+    if (lane == 0) t = (int)atomicAdd(b.queue, 1u);
+    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, kWave));
+    if (t >= b.ntask) break;
+    const int id = b.task_item[t];
+    const BatchItem& it = b.items[id];
+    if (it.done) continue;
+    MarchArgs a{};
+    const bool odd = it.launches & 1;
+    a.xin = odd ? it.xb : it.xa;
+    a.xout = odd ? it.xa : it.xb;
+    a.g = it.g;
+    a.m = it.m;
+    a.plane = it.plane;
+    a.H = it.H;
+    a.W = it.W;
+    a.P = it.P;
+    a.t0 = it.launches * S;
+    a.t_end = it.t_end;
+    a.band = 0;
+    a.stripes = it.stripes;
+    a.segs = it.segs;
+    a.seg_rows = b.seg_rows;
+    const int local = t - it.task0;
+    march_task<S, C, SCHED, ORDER>(a, local % it.stripes, local / it.stripes,
+                                   &b.items[id].dmax_bits);
   }
 }
 

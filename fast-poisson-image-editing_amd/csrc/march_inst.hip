@@ -32,4 +32,38 @@ hipError_t PIE_NAME(PIE_S, PIE_C)(int sched, int order, const MarchArgs& a,
   return hipErrorInvalidValue;
 }
 
+#define PIE_BNAME2(s, c) launch_batch_##s##_##c
+#define PIE_BNAME(s, c) PIE_BNAME2(s, c)
+#define PIE_ONAME2(s, c) batch_blocks_per_cu_##s##_##c
+#define PIE_ONAME(s, c) PIE_ONAME2(s, c)
+
+hipError_t PIE_BNAME(PIE_S, PIE_C)(int sched, int order, const BatchArgs& b,
+                                   int blocks, hipStream_t st) {
+  using namespace detail;
+#define PIE_L(SC, OR)                                                          \
+  if (sched == SC && order == OR) {                                           \
+    hipLaunchKernelGGL((march_batch_kernel<PIE_S, PIE_C, SC, OR>),             \
+                       dim3(blocks), dim3(kBlock), 0, st, b);                  \
+    return hipGetLastError();                                                 \
+  }
+  PIE_L(SCHED_JACOBI, ORDER_UDLR)
+  PIE_L(SCHED_JACOBI, ORDER_ULRD)
+#undef PIE_L
+  return hipErrorInvalidValue;
+}
+
+int PIE_ONAME(PIE_S, PIE_C)(int sched, int order) {
+  using namespace detail;
+  int n = 0;
+  hipError_t e = hipErrorInvalidValue;
+#define PIE_L(SC, OR)                                                       \
+  if (sched == SC && order == OR)                                          \
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(                       \
+        &n, (march_batch_kernel<PIE_S, PIE_C, SC, OR>), kBlock, 0);
+  PIE_L(SCHED_JACOBI, ORDER_UDLR)
+  PIE_L(SCHED_JACOBI, ORDER_ULRD)
+#undef PIE_L
+  return e == hipSuccess && n > 0 ? n : 1;
+}
+
 }  // namespace pie

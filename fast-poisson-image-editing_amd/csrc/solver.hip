@@ -19,56 +19,24 @@
 #include <string>
 #include <vector>
 
-#include "../../include/fpie_hip.h"
-#include "kernels.h"
+#include "host_util.h"
 
 using pie::MarchArgs;
 using pie::TileCfg;
+using pie::host::DevBuf;
+using pie::host::fail;
+using pie::host::pitch_of;
+using pie::host::round_up;
+using pie::host::upload;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const std::string& msg) {
+int pie_fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
-
-#define PIE_HIP(expr)                                                   \
-  do {                                                                  \
-    hipError_t e_ = (expr);                                             \
-    if (e_ != hipSuccess)                                               \
-      return fail(PIE_EHIP, std::string(#expr) + ": " +                 \
-                                hipGetErrorString(e_));                 \
-  } while (0)
-
-#define PIE_CHECK(expr)          \
-  do {                           \
-    int r_ = (expr);             \
-    if (r_ != PIE_OK) return r_; \
-  } while (0)
-
-template <class T>
-struct DevBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  hipError_t alloc(size_t count) {
-    release();
-    n = count;
-    return hipMalloc(reinterpret_cast<void**>(&p),
-                     std::max<size_t>(count, 1) * sizeof(T));
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  ~DevBuf() { release(); }
-};
-
-int round_up(int v, int m) { return (v + m - 1) / m * m; }
-
-}  // namespace
 
 struct pie_solver {
   int kind = PIE_GRID, sched = PIE_JACOBI, order = PIE_ORDER_ULRD;
@@ -103,8 +71,13 @@ struct pie_solver {
   double t_ms = 0.0, t_updates = 0.0;
   long long t_launches = 0;
 
-  float* state() { return cur ? xb.p : xa.p; }
-  float* other() { return cur ? xa.p : xb.p; }
+  // column-0 base pointers of the padded planes
+  float* xa0() { return xa.p + pie::kPadLeft; }
+  float* xb0() { return xb.p + pie::kPadLeft; }
+  float* g0() { return g.p + pie::kPadLeft; }
+  uint8_t* m0() { return m.p + pie::kPadLeft; }
+  float* state() { return cur ? xb0() : xa0(); }
+  float* other() { return cur ? xa0() : xb0(); }
   int eff_order() const {
     return kind == PIE_EQU ? PIE_ORDER_UDLR : order;  // gcc/equ.cc:48-53
   }
@@ -134,11 +107,6 @@ void free_problem(pie_solver* s) {
   s->tuned = false;
 }
 
-int upload(const void* host, size_t bytes, void* dev, hipStream_t st) {
-  PIE_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, st));
-  return PIE_OK;
-}
-
 // Total global steps of one call of `iters` sweeps and the band width.
 void schedule_steps(const pie_solver* s, int iters, int* steps, int* band) {
   *band = 2 * iters;
@@ -155,8 +123,8 @@ MarchArgs march_args(pie_solver* s, const float* xin, float* xout, int t0,
   MarchArgs a{};
   a.xin = xin;
   a.xout = xout;
-  a.g = s->g.p;
-  a.m = s->m.p;
+  a.g = s->g0();
+  a.m = s->m0();
   a.plane = s->plane;
   a.H = s->H;
   a.W = s->W;
@@ -171,16 +139,19 @@ int alloc_grid(pie_solver* s, int H, int W) {
   if (H < 1 || W < 1) return fail(PIE_EINVAL, "empty grid");
   s->H = H;
   s->W = W;
-  s->P = round_up(W, 64);
+  s->P = pitch_of(W);
   s->plane = (long long)H * s->P;
   if (s->plane > (1LL << 31) - 1)
-    return fail(PIE_EINVAL, "grid larger than 2^31 pixels");
+    return fail(PIE_EINVAL, "grid larger than 2^31 elements per plane");
   PIE_HIP(s->xa.alloc(3 * s->plane));
   PIE_HIP(s->xb.alloc(3 * s->plane));
   PIE_HIP(s->g.alloc(3 * s->plane));
   PIE_HIP(s->m.alloc(s->plane));
-  PIE_HIP(hipMemsetAsync(s->m.p, 0, s->plane, s->stream));
+  // padding must hold finite values (zeros) and an unsolved mask
+  PIE_HIP(hipMemsetAsync(s->xa.p, 0, 3 * s->plane * sizeof(float), s->stream));
+  PIE_HIP(hipMemsetAsync(s->xb.p, 0, 3 * s->plane * sizeof(float), s->stream));
   PIE_HIP(hipMemsetAsync(s->g.p, 0, 3 * s->plane * sizeof(float), s->stream));
+  PIE_HIP(hipMemsetAsync(s->m.p, 0, s->plane, s->stream));
   const int nb = pie::finalize_blocks(H, W);
   PIE_HIP(s->partial.alloc(3 * (size_t)nb));
   PIE_HIP(s->err.alloc(3));
@@ -287,7 +258,7 @@ int read_outputs(pie_solver* s, uint8_t* host_img, float* host_err,
     const bool grid_img = host_img && s->kind == PIE_GRID;
     if (grid_img && !s->img.p)
       PIE_HIP(s->img.alloc((size_t)s->H * s->W * 3));
-    PIE_HIP(pie::launch_finalize(s->eff_order(), s->state(), s->g.p, s->m.p,
+    PIE_HIP(pie::launch_finalize(s->eff_order(), s->state(), s->g0(), s->m0(),
                                  s->plane, s->H, s->W, s->P,
                                  grid_img ? s->img.p : nullptr, s->partial.p,
                                  nb, s->stream));
@@ -433,7 +404,7 @@ int pie_reset_equ(pie_solver* s, int n, const int32_t* A, const float* X,
   std::vector<int32_t> lin;
   bool structured = s->part_count == n - 1 && s->ph >= 3 && s->pw >= 3;
   if (structured) {
-    const int h = s->ph, w = s->pw, P = round_up(w, 64);
+    const int h = s->ph, w = s->pw, P = pitch_of(w);
     const int32_t* ids = s->part_ids.data();
     lin.assign(n, 0);
     for (int i = 0; i < h && structured; ++i)
@@ -461,7 +432,8 @@ int pie_reset_equ(pie_solver* s, int n, const int32_t* A, const float* X,
     std::vector<uint8_t> mh((size_t)s->plane, 0);
     for (int i = 0; i < s->H; ++i)
       for (int j = 0; j < s->W; ++j)
-        mh[(size_t)i * s->P + j] = s->part_ids[(size_t)i * s->W + j] ? 1 : 0;
+        mh[(size_t)i * s->P + pie::kPadLeft + j] =
+            s->part_ids[(size_t)i * s->W + j] ? 1 : 0;
     DevBuf<float> Xd, Bd;
     PIE_HIP(Xd.alloc(3 * (size_t)n));
     PIE_HIP(Bd.alloc(3 * (size_t)n));
@@ -472,8 +444,8 @@ int pie_reset_equ(pie_solver* s, int n, const int32_t* A, const float* X,
     PIE_CHECK(upload(lin.data(), 4 * (size_t)n, s->lin.p, s->stream));
     // unsolved pixels hold X[0]: the value the reference reads via id 0
     PIE_HIP(pie::launch_fill3(s->xa.p, s->plane, X[0], X[1], X[2], s->stream));
-    PIE_HIP(pie::launch_equ_scatter(Xd.p, Bd.p, s->lin.p, n, s->xa.p, s->g.p,
-                                    s->plane, s->stream));
+    PIE_HIP(pie::launch_equ_scatter(Xd.p, Bd.p, s->lin.p, n, s->xa0(),
+                                    s->g0(), s->plane, s->stream));
     PIE_HIP(hipMemcpyAsync(s->xb.p, s->xa.p, 3 * s->plane * sizeof(float),
                            hipMemcpyDeviceToDevice, s->stream));
     PIE_HIP(hipStreamSynchronize(s->stream));
@@ -521,8 +493,8 @@ int pie_reset_grid(pie_solver* s, int h, int w, const int32_t* mask,
   PIE_HIP(counts.alloc(2));
   PIE_HIP(hipMemsetAsync(counts.p, 0, 2 * sizeof(unsigned long long), s->stream));
   PIE_CHECK(upload(mask, px * 4, stage.p, s->stream));
-  PIE_HIP(pie::launch_mask(reinterpret_cast<const int32_t*>(stage.p), s->m.p,
-                           h, w, s->P, counts.p, s->stream));
+  PIE_HIP(pie::launch_mask(reinterpret_cast<const int32_t*>(stage.p),
+                           s->m0(), h, w, s->P, counts.p, s->stream));
   unsigned long long cnt[2] = {0, 0};
   PIE_HIP(hipMemcpyAsync(cnt, counts.p, sizeof(cnt), hipMemcpyDeviceToHost,
                          s->stream));
@@ -534,12 +506,12 @@ int pie_reset_grid(pie_solver* s, int h, int w, const int32_t* mask,
   }
   PIE_CHECK(upload(tgt, px * 12, stage.p, s->stream));
   PIE_HIP(pie::launch_deinterleave(reinterpret_cast<const float*>(stage.p),
-                                   s->xa.p, s->plane, h, w, s->P, s->stream));
+                                   s->xa0(), s->plane, h, w, s->P, s->stream));
   PIE_HIP(hipMemcpyAsync(s->xb.p, s->xa.p, 3 * s->plane * sizeof(float),
                          hipMemcpyDeviceToDevice, s->stream));
   PIE_CHECK(upload(grad, px * 12, stage.p, s->stream));
   PIE_HIP(pie::launch_deinterleave(reinterpret_cast<const float*>(stage.p),
-                                   s->g.p, s->plane, h, w, s->P, s->stream));
+                                   s->g0(), s->plane, h, w, s->P, s->stream));
   PIE_HIP(hipStreamSynchronize(s->stream));
   s->masked = (long long)cnt[0];
   s->N = h * w;

@@ -183,3 +183,72 @@ class GridSolver(_Core):
     out = np.empty(self.hw + (3,), np.float32)
     native.check(self._lib.pie_get_state(self._h, _ptr(out)))
     return out
+
+
+class BatchSolver:
+  """Batched GridSolver (extension): composites of uneven size and mask,
+  each with its own stop rule, solved by one persistent Jacobi kernel over a
+  shared tile queue with the round loop in a hipGraph."""
+
+  def __init__(self, order: str = "ulrd", sweeps_per_launch: int = 8,
+               cols_per_lane: int = 1, rows_per_task: int = 64):
+    lib = native.load()
+    h = ctypes.c_void_p()
+    native.check(lib.pie_batch_create(native.ORDERS[order], sweeps_per_launch,
+                                      cols_per_lane, rows_per_task,
+                                      ctypes.byref(h)))
+    self._h, self._lib = h, lib
+    self.shapes: list[tuple[int, int]] = []
+
+  def __del__(self):
+    h = getattr(self, "_h", None)
+    if h is not None and h.value:
+      self._lib.pie_batch_destroy(h)
+      self._h = None
+
+  def add(self, mask: np.ndarray, tgt: np.ndarray, grad: np.ndarray,
+          max_sweeps: int = 5000, tol: float = 1e-3) -> int:
+    m = np.ascontiguousarray(mask, dtype=np.int32)
+    t = np.ascontiguousarray(tgt, dtype=np.float32)
+    g = np.ascontiguousarray(grad, dtype=np.float32)
+    if m.ndim != 2 or t.shape != m.shape + (3,) or g.shape != t.shape:
+      raise ValueError("expected mask[h,w], tgt[h,w,3], grad[h,w,3]")
+    idx = ctypes.c_int()
+    native.check(self._lib.pie_batch_add(self._h, m.shape[0], m.shape[1],
+                                         _ptr(m), _ptr(t), _ptr(g),
+                                         int(max_sweeps), float(tol),
+                                         ctypes.byref(idx)))
+    self.shapes.append(m.shape)
+    return idx.value
+
+  def solve(self, rounds_per_graph: int = 16) -> tuple[float, int]:
+    sec = ctypes.c_double()
+    rounds = ctypes.c_int()
+    native.check(self._lib.pie_batch_solve(self._h, int(rounds_per_graph),
+                                           ctypes.byref(sec),
+                                           ctypes.byref(rounds)))
+    return sec.value, rounds.value
+
+  def item(self, i: int) -> dict:
+    sw, dm, lat = ctypes.c_int(), ctypes.c_float(), ctypes.c_double()
+    native.check(self._lib.pie_batch_item(self._h, i, ctypes.byref(sw),
+                                          ctypes.byref(dm),
+                                          ctypes.byref(lat)))
+    return {"sweeps": sw.value, "last_dmax": dm.value,
+            "latency_s": lat.value}
+
+  def read(self, i: int, state: bool = False):
+    h, w = self.shapes[i]
+    img = np.empty((h, w, 3), np.uint8)
+    err = np.empty(3, np.float32)
+    st = np.empty((h, w, 3), np.float32) if state else None
+    native.check(self._lib.pie_batch_read(
+        self._h, i, _ptr(img), _ptr(err),
+        _ptr(st) if state else ctypes.c_void_p()))
+    return (img, err, st) if state else (img, err)
+
+  def size(self) -> dict:
+    n, t, m = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()
+    native.check(self._lib.pie_batch_size(self._h, ctypes.byref(n),
+                                          ctypes.byref(t), ctypes.byref(m)))
+    return {"items": n.value, "tasks": t.value, "masked": m.value}

@@ -49,6 +49,18 @@ _SIGNATURES = {
                    ctypes.POINTER(_ip), ctypes.POINTER(ctypes.c_longlong)],
                   _ip),
     "pie_device_info": ([ctypes.c_char_p, _ip], _ip),
+    "pie_batch_create": ([_ip, _ip, _ip, _ip, ctypes.POINTER(_vp)], _ip),
+    "pie_batch_destroy": ([_vp], _ip),
+    "pie_batch_add": ([_vp, _ip, _ip, _vp, _vp, _vp, _ip, ctypes.c_float,
+                       ctypes.POINTER(_ip)], _ip),
+    "pie_batch_solve": ([_vp, _ip, ctypes.POINTER(ctypes.c_double),
+                         ctypes.POINTER(_ip)], _ip),
+    "pie_batch_item": ([_vp, _ip, ctypes.POINTER(_ip),
+                        ctypes.POINTER(ctypes.c_float),
+                        ctypes.POINTER(ctypes.c_double)], _ip),
+    "pie_batch_read": ([_vp, _ip, _vp, _vp, _vp], _ip),
+    "pie_batch_size": ([_vp, ctypes.POINTER(_ip), ctypes.POINTER(_ip),
+                        ctypes.POINTER(ctypes.c_longlong)], _ip),
 }
 
 _lib: ctypes.CDLL | None = None

@@ -14,7 +14,11 @@ Workloads (seeds: source 1000 + i, target 2000 + i):
   pixels in a 3802 x 3802 crop), gradient ``src``, GridSolver, 5000
   iterations;
 * ``grid8192`` 8192 x 8192, union of 6 random ellipses covering 60 +- 2 %
-  of the image, gradient ``max``, GridSolver, 10000 iterations.
+  of the image, gradient ``max``, GridSolver, 10000 iterations;
+* ``batch256`` 256 composites, side log-uniform in [128, 2048], aspect
+  ratio log-uniform in [0.5, 2], star polygon masks of 8-24 vertices
+  covering 20-80 % of the image, gradient ``src`` or ``max`` per item, each
+  stopped after 5000 sweeps or once a sweep changes no value by 1e-3.
 """
 
 from __future__ import annotations
@@ -83,6 +87,54 @@ def ellipse_mask(h: int, w: int, seed: int, n: int = 6,
   raise RuntimeError("no ellipse union within the coverage band")
 
 
+def _polygon_raster(h: int, w: int, py: np.ndarray,
+                    px: np.ndarray) -> np.ndarray:
+  """Even-odd fill of a closed polygon: pixel (y, x) is inside when an odd
+  number of edges cross row y to the right of x.  Each edge crossing of a
+  row becomes a count at column ceil(x_cross); a reverse cumulative sum
+  along the row gives, per pixel, the crossings to its right."""
+  y = np.arange(h, dtype=np.float64)[:, None]
+  y0, x0 = py[None, :], px[None, :]
+  y1, x1 = np.roll(py, -1)[None, :], np.roll(px, -1)[None, :]
+  dy = np.where(y1 == y0, 1.0, y1 - y0)
+  cross = (y >= np.minimum(y0, y1)) & (y < np.maximum(y0, y1)) & (y1 != y0)
+  xs = x0 + (y - y0) * (x1 - x0) / dy
+  col = np.clip(np.ceil(xs), 0, w).astype(np.int64)
+  cnt = np.zeros((h, w + 1), np.int32)
+  rows = np.broadcast_to(np.arange(h)[:, None], col.shape)
+  np.add.at(cnt, (rows[cross], col[cross]), 1)
+  right = np.cumsum(cnt[:, ::-1], axis=1)[:, ::-1]  # crossings at >= col
+  return (right[:, 1:] & 1).astype(bool)  # pixel x: crossings at > x
+
+
+def star_mask(h: int, w: int, rng: np.random.Generator,
+              coverage: tuple[float, float] = (0.2, 0.8)) -> np.ndarray:
+  """Filled star-shaped polygon (8-24 vertices at sorted random angles
+  around the centre, radii uniform in [0.35, 1] of the largest radius),
+  scaled so that it covers a fraction drawn uniformly from `coverage`."""
+  n = int(rng.integers(8, 25))
+  ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+  rad = rng.uniform(0.35, 1.0, n)
+  want = rng.uniform(*coverage)
+
+  def raster(hh, ww, scale):
+    py = hh / 2 + scale * (hh / 2) * rad * np.sin(ang)
+    px = ww / 2 + scale * (ww / 2) * rad * np.cos(ang)
+    return _polygon_raster(hh, ww, py, px)
+
+  # bisection on the star's size over a coarse copy of the image
+  f = max(1, max(h, w) // 256)
+  hc, wc = max(8, h // f), max(8, w // f)
+  lo, hi = 0.05, 3.0
+  for _ in range(24):
+    mid = (lo + hi) / 2
+    if raster(hc, wc, mid).mean() < want:
+      lo = mid
+    else:
+      hi = mid
+  return raster(h, w, (lo + hi) / 2).astype(np.uint8) * 255
+
+
 @dataclass
 class Workload:
   name: str
@@ -112,6 +164,26 @@ def grid8192(i: int = 0) -> Workload:
   return Workload("grid8192", "grid", blobs(8192, 8192, 1000 + i),
                   ellipse_mask(8192, 8192, 3000 + i),
                   blobs(8192, 8192, 2000 + i), "max", 10000)
+
+
+def batch(n: int = 256, seed: int = 0, side: tuple[int, int] = (128, 2048),
+          max_sweeps: int = 5000, tol: float = 1e-3) -> list[Workload]:
+  """The batched workload: n independent GridSolver composites."""
+  rng = np.random.default_rng(seed)
+  out = []
+  for i in range(n):
+    s = float(np.exp(rng.uniform(np.log(side[0]), np.log(side[1]))))
+    aspect = float(np.exp(rng.uniform(np.log(0.5), np.log(2.0))))
+    h = max(16, int(round(s * np.sqrt(aspect))))
+    w = max(16, int(round(s / np.sqrt(aspect))))
+    mode = "src" if rng.random() < 0.5 else "max"
+    item_rng = np.random.default_rng([seed, i])
+    wl = Workload(f"batch{n}[{i}]", "grid", blobs(h, w, 1000 + i),
+                  star_mask(h, w, item_rng), blobs(h, w, 2000 + i), mode,
+                  max_sweeps)
+    wl.tol = tol
+    out.append(wl)
+  return out
 
 
 WORKLOADS = {"equ512": equ512, "grid4096": grid4096, "grid8192": grid8192}

@@ -134,6 +134,36 @@ int pie_shape(pie_solver* s, int* h, int* w, int* n, long long* masked);
 /* Name and properties of the current device (cuda/utils.cu:5-22). */
 int pie_device_info(char* buf, int len);
 
+/* ---- batched GridSolver: no counterpart in the reference ---------------
+ * Many composites of uneven size and mask, each with its own stop rule,
+ * solved together by one persistent Jacobi sweep kernel over a shared work
+ * queue of tiles, R rounds per hipGraph replay.  An item stops after the
+ * first launch whose last sweep changed no value by `tol` or more, or after
+ * `max_sweeps` sweeps. */
+typedef struct pie_batch pie_batch;
+/* order: PIE_ORDER_*; tile: sweeps per launch, columns per lane, rows per
+ * task (0 = default 8, 1, 64). */
+int pie_batch_create(int order, int sweeps_per_launch, int cols_per_lane,
+                     int seg_rows, pie_batch** out);
+int pie_batch_destroy(pie_batch* b);
+/* Add one GridSolver problem (arguments as pie_reset_grid); *index gets its
+ * position in the batch. */
+int pie_batch_add(pie_batch* b, int h, int w, const int32_t* mask,
+                  const float* tgt, const float* grad, int max_sweeps,
+                  float tol, int* index);
+/* Solve every item from its initial state until all have stopped;
+ * *seconds = wall time of the solve, *rounds_run = rounds launched. */
+int pie_batch_solve(pie_batch* b, int rounds_per_graph, double* seconds,
+                    int* rounds_run);
+/* Sweeps done, max |update| of its last launch, and the time from the
+ * start of the solve until the item stopped (device clock). */
+int pie_batch_item(pie_batch* b, int i, int* sweeps, float* last_dmax,
+                   double* latency_s);
+/* Outputs of item i as pie_step / pie_get_state (any may be NULL). */
+int pie_batch_read(pie_batch* b, int i, uint8_t* img, float* err,
+                   float* state);
+int pie_batch_size(pie_batch* b, int* items, int* tasks, long long* masked);
+
 #ifdef __cplusplus
 }
 #endif

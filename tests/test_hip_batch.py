@@ -1,0 +1,107 @@
+"""Batched GridSolver (GPU): one persistent Jacobi kernel over many items.
+
+Every item must end bit-identical to the oracle's Jacobi loop run for the
+number of sweeps the item reports, and the stop rule must have fired at the
+first launch whose last sweep moved no value by `tol` or more: the oracle
+recomputes that last-sweep change exactly.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+  return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+class _Grab:
+  def reset(self, *a):
+    self.a = a
+
+
+def _items(n, seed, side=(24, 160)):
+  from fpie_amd import process, synth
+  process.register_backend("grab", lambda **kw: _Grab(), lambda **kw: _Grab())
+  out = []
+  for w in synth.batch(n, seed=seed, side=side):
+    p = process.GridProcessor(w.gradient, "grab")
+    p.reset(w.src, w.mask, w.tgt, (0, 0), (0, 0))
+    _, m, t, g = p.core.a
+    out.append((m, t, g))
+  return out
+
+
+@pytest.fixture(scope="module")
+def hip():
+  from fpie_amd import core_hip, native
+  native.load()
+  return core_hip
+
+
+def test_fixed_budget_items_match_oracle(hip, oracle):
+  """tol = 0 never stops early: every item runs exactly its budget."""
+  items = _items(12, 3)
+  budgets = [0, 1, 7, 8, 9, 16, 37, 64, 100, 5, 23, 50]
+  b = hip.BatchSolver(sweeps_per_launch=8)
+  for (m, t, g), k in zip(items, budgets):
+    b.add(m, t, g, max_sweeps=k, tol=0.0)
+  b.solve(rounds_per_graph=4)
+  for i, ((m, t, g), k) in enumerate(zip(items, budgets)):
+    info = b.item(i)
+    assert info["sweeps"] == k
+    img, err, st = b.read(i, state=True)
+    want = oracle.grid_jacobi(m, t, g, k, "ulrd")
+    assert np.array_equal(bits(st), bits(want)), (i, k)
+    np.testing.assert_array_equal(img, oracle.clamp_u8(want))
+    np.testing.assert_allclose(err, oracle.grid_terms(m, want, g).sum((0, 1)),
+                               rtol=1e-6, atol=1e-4)
+
+
+def test_stop_rule_fires_at_first_quiet_launch(hip, oracle):
+  items = _items(10, 7)
+  tol = 0.05
+  b = hip.BatchSolver(sweeps_per_launch=8)
+  for m, t, g in items:
+    b.add(m, t, g, max_sweeps=2000, tol=tol)
+  b.solve(rounds_per_graph=8)
+  for i, (m, t, g) in enumerate(items):
+    info = b.item(i)
+    k = info["sweeps"]
+    _, _, st = b.read(i, state=True)
+    want = oracle.grid_jacobi(m, t, g, k, "ulrd")
+    assert np.array_equal(bits(st), bits(want)), i
+    if k < 2000:
+      assert k % 8 == 0
+      before = oracle.grid_jacobi(m, t, g, k - 1, "ulrd")
+      change = float(np.abs(want - before).max())
+      assert info["last_dmax"] == change < tol
+      if k > 8:  # the previous launch's last sweep was still moving
+        a = oracle.grid_jacobi(m, t, g, k - 9, "ulrd")
+        bb = oracle.grid_jacobi(m, t, g, k - 8, "ulrd")
+        assert float(np.abs(bb - a).max()) >= tol
+    assert info["latency_s"] >= 0.0
+
+
+def test_solve_restarts_from_the_initial_state(hip):
+  items = _items(4, 11)
+  b = hip.BatchSolver()
+  for m, t, g in items:
+    b.add(m, t, g, max_sweeps=40, tol=0.0)
+  b.solve()
+  first = [bits(b.read(i, state=True)[2]).copy() for i in range(4)]
+  b.solve()
+  for i in range(4):
+    np.testing.assert_array_equal(bits(b.read(i, state=True)[2]), first[i])
+
+
+def test_batch_rejects_bad_items(hip):
+  b = hip.BatchSolver()
+  m = np.zeros((10, 10), np.int32)
+  m[0, 3] = 1
+  t = np.zeros((10, 10, 3), np.float32)
+  with pytest.raises(RuntimeError, match="outermost"):
+    b.add(m, t, t)
+  with pytest.raises(RuntimeError, match="empty batch"):
+    b.solve()
