@@ -215,7 +215,7 @@ def main():
   if a.tile:
     core.set_tile(*[int(v) for v in a.tile.split(",")])
   else:
-    core.autotune(3)
+    core.autotune(8)
   tile = core.tile()
   print(f"# tile S={tile[0]} C={tile[1]} rows={tile[2]}", file=sys.stderr,
         flush=True)
@@ -237,6 +237,8 @@ def main():
   alg_bytes = BYTES_PER_UPDATE * masked * sweeps_per_launch
   achieved = alg_bytes / (launch_ms * 1e-3) / 1e9
   traffic = traffic_from_profiles(w.name, a.schedule)
+  if traffic and traffic.get("tile") != list(tile):
+    traffic = None  # measured for another tile
   line = {
       "metric": "Jacobi pixel-updates/s + % HBM peak",
       "value": value,

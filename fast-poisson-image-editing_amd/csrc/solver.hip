@@ -532,7 +532,7 @@ int pie_iterate(pie_solver* s, int iterations) {
   if (iterations < 0) return fail(PIE_EINVAL, "negative iteration count");
   if (s->grid_layout && !s->tuned && !s->tile_fixed &&
       (long long)s->H * s->W >= (1LL << 22))
-    PIE_CHECK(autotune(s, 3));
+    PIE_CHECK(autotune(s, 6));
   return run_sweeps(s, iterations);
 }
 
@@ -549,6 +549,49 @@ int pie_step(pie_solver* s, int iterations, uint8_t* img, float* err) {
 int pie_get_state(pie_solver* s, float* out) {
   if (!s || !out) return fail(PIE_EINVAL, "NULL argument");
   return read_outputs(s, nullptr, nullptr, out);
+}
+
+int pie_set_state(pie_solver* s, const float* x) {
+  if (!s || !x) return fail(PIE_EINVAL, "NULL argument");
+  if (!s->loaded) return fail(PIE_ESTATE, "set_state before reset");
+  if (s->kind != PIE_GRID)
+    return fail(PIE_EUNSUPPORTED, "set_state is a GridSolver call");
+  const size_t px = (size_t)s->H * s->W;
+  DevBuf<float> stage;
+  PIE_HIP(stage.alloc(px * 3));
+  PIE_CHECK(upload(x, px * 12, stage.p, s->stream));
+  // both buffers: padding and unsolved pixels keep their values
+  PIE_HIP(pie::launch_deinterleave(stage.p, s->xa0(), s->plane, s->H, s->W,
+                                   s->P, s->stream));
+  PIE_HIP(hipMemcpyAsync(s->xb.p, s->xa.p, 3 * s->plane * sizeof(float),
+                         hipMemcpyDeviceToDevice, s->stream));
+  PIE_HIP(hipStreamSynchronize(s->stream));
+  s->cur = 0;
+  return PIE_OK;
+}
+
+int pie_solve_to_residual(pie_solver* s, double tol_mean, int check_every,
+                          long long max_sweeps, long long* sweeps,
+                          double* mean_residual) {
+  if (!s) return fail(PIE_EINVAL, "NULL solver");
+  if (!s->loaded) return fail(PIE_ESTATE, "solve before reset");
+  if (check_every < 1 || max_sweeps < 0)
+    return fail(PIE_EINVAL, "check_every >= 1 and max_sweeps >= 0");
+  const double denom = 3.0 * (double)std::max(1LL, s->masked);
+  long long done = 0;
+  float e[3];
+  PIE_CHECK(read_outputs(s, nullptr, e, nullptr));
+  double mean = ((double)e[0] + e[1] + e[2]) / denom;
+  while (mean > tol_mean && done < max_sweeps) {
+    const int k = (int)std::min<long long>(check_every, max_sweeps - done);
+    PIE_CHECK(pie_iterate(s, k));
+    done += k;
+    PIE_CHECK(read_outputs(s, nullptr, e, nullptr));
+    mean = ((double)e[0] + e[1] + e[2]) / denom;
+  }
+  if (sweeps) *sweeps = done;
+  if (mean_residual) *mean_residual = mean;
+  return PIE_OK;
 }
 
 int pie_set_tile(pie_solver* s, int sweeps, int cols_per_lane, int seg_rows) {

@@ -1,0 +1,424 @@
+// Warm-start U-Net runtime (C ABI pie_unet_*): owns the parameters, Adam
+// state, bf16 operand copies of the weights and every activation and
+// gradient buffer of one batch shape, and runs forward, backward and the
+// optimiser step with the kernels of unet_kernels.hip.
+//
+// Network (no counterpart in the reference): 4 resolution levels, base 32
+// channels, 3x3 convolutions with bias and ReLU, 2x2 average pooling,
+// nearest 2x upsampling, skip concatenation (upsampled channels first), a
+// 1x1 head to 3 channels.  Input: 8 channels (3 divergence / 64,
+// 3 (target - 128) / 64, mask, one zero channel); output: the correction
+// (x - target) / 64 on the solved pixels.
+//
+// Parameters are one fp32 array, layer by layer: conv w[tap][cin][cout]
+// then b[cout]; the head w[32][3] then b[3].
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "host_util.h"
+#include "unet_kernels.h"
+
+using pie::host::DevBuf;
+using pie::host::fail;
+using pie::unet::bf16_t;
+using pie::unet::Slice;
+
+namespace {
+
+struct ConvSpec {
+  const char* name;
+  int cin, cout, level;
+};
+
+// name, cin, cout, resolution level (0 = full)
+const ConvSpec kConvs[] = {
+    {"e0a", 8, 32, 0},    {"e0b", 32, 32, 0},   {"e1a", 32, 64, 1},
+    {"e1b", 64, 64, 1},   {"e2a", 64, 128, 2},  {"e2b", 128, 128, 2},
+    {"bna", 128, 256, 3}, {"bnb", 256, 256, 3}, {"d2a", 384, 128, 2},
+    {"d2b", 128, 128, 2}, {"d1a", 192, 64, 1},  {"d1b", 64, 64, 1},
+    {"d0a", 96, 32, 0},   {"d0b", 32, 32, 0},
+};
+constexpr int kNumConv = sizeof(kConvs) / sizeof(kConvs[0]);
+enum { E0A, E0B, E1A, E1B, E2A, E2B, BNA, BNB, D2A, D2B, D1A, D1B, D0A, D0B };
+
+}  // namespace
+
+struct pie_unet {
+  int B = 0, H = 0, W = 0;
+  long long P[4] = {0, 0, 0, 0};  // pixels per level
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  long long nparam = 0;
+  long long woff[kNumConv + 1] = {}, boff[kNumConv + 1] = {};
+  DevBuf<float> param, grad, adam_m, adam_v;
+  DevBuf<bf16_t> wk[kNumConv], wkt[kNumConv];
+  int step = 0;
+  // activations (bf16)
+  DevBuf<bf16_t> in, e0a, c0, p0, e1a, c1, p1, e2a, c2, p2, bna, bnb, d2a,
+      d2b, d1a, d1b, d0a, d0b;
+  // fp32 host-facing buffers and gradients
+  DevBuf<float> x32, out, dout, target;
+  DevBuf<uint8_t> mask;
+  DevBuf<float> gA, gB, gC0, gC1, gC2, gP;
+  DevBuf<bf16_t> dyb;
+  DevBuf<double> acc;
+  double ms_total = 0.0, flops_total = 0.0;
+};
+
+namespace {
+
+Slice sl(DevBuf<bf16_t>& b, int stride, int off, int ch) {
+  return Slice{b.p, stride, off, ch};
+}
+
+int alloc_all(pie_unet* u) {
+  const long long P0 = u->P[0], P1 = u->P[1], P2 = u->P[2], P3 = u->P[3];
+  PIE_HIP(u->in.alloc(P0 * 8));
+  PIE_HIP(u->e0a.alloc(P0 * 32));
+  PIE_HIP(u->c0.alloc(P0 * 96));
+  PIE_HIP(u->p0.alloc(P1 * 32));
+  PIE_HIP(u->e1a.alloc(P1 * 64));
+  PIE_HIP(u->c1.alloc(P1 * 192));
+  PIE_HIP(u->p1.alloc(P2 * 64));
+  PIE_HIP(u->e2a.alloc(P2 * 128));
+  PIE_HIP(u->c2.alloc(P2 * 384));
+  PIE_HIP(u->p2.alloc(P3 * 128));
+  PIE_HIP(u->bna.alloc(P3 * 256));
+  PIE_HIP(u->bnb.alloc(P3 * 256));
+  PIE_HIP(u->d2a.alloc(P2 * 128));
+  PIE_HIP(u->d2b.alloc(P2 * 128));
+  PIE_HIP(u->d1a.alloc(P1 * 64));
+  PIE_HIP(u->d1b.alloc(P1 * 64));
+  PIE_HIP(u->d0a.alloc(P0 * 32));
+  PIE_HIP(u->d0b.alloc(P0 * 32));
+  PIE_HIP(u->x32.alloc(P0 * 8));
+  PIE_HIP(u->out.alloc(P0 * 3));
+  PIE_HIP(u->dout.alloc(P0 * 3));
+  PIE_HIP(u->target.alloc(P0 * 3));
+  PIE_HIP(u->mask.alloc(P0));
+  // gradient scratch: the largest per-level working set
+  long long ga = std::max({P0 * 32, P1 * 64, P2 * 128, P3 * 256});
+  PIE_HIP(u->gA.alloc(ga));
+  PIE_HIP(u->gB.alloc(ga));
+  PIE_HIP(u->gC0.alloc(P0 * 96));
+  PIE_HIP(u->gC1.alloc(P1 * 192));
+  PIE_HIP(u->gC2.alloc(P2 * 384));
+  PIE_HIP(u->gP.alloc(std::max({P1 * 32, P2 * 64, P3 * 128})));
+  PIE_HIP(u->dyb.alloc(ga));
+  PIE_HIP(u->acc.alloc(2));
+  return PIE_OK;
+}
+
+int pack_all(pie_unet* u) {
+  for (int l = 0; l < kNumConv; ++l)
+    PIE_HIP(pie::unet::pack_weights(u->param.p + u->woff[l], kConvs[l].cin,
+                                    kConvs[l].cout, u->wk[l].p, u->wkt[l].p,
+                                    u->stream));
+  return PIE_OK;
+}
+
+double conv_flops(const pie_unet* u, bool backward) {
+  double f = 0.0;
+  for (int l = 0; l < kNumConv; ++l) {
+    const double one = 2.0 * u->P[kConvs[l].level] * 9.0 * kConvs[l].cin *
+                       kConvs[l].cout;
+    f += one;
+    if (backward) f += one * (l == E0A ? 1.0 : 2.0);  // wgrad (+ dgrad)
+  }
+  return f;
+}
+
+int conv_fwd(pie_unet* u, int l, const Slice& x, const Slice& y) {
+  const ConvSpec& c = kConvs[l];
+  const int s = 1 << c.level;
+  PIE_HIP(pie::unet::conv3x3(x, u->B, u->H / s, u->W / s, u->wk[l].p, c.cout,
+                             u->param.p + u->boff[l], true, &y, nullptr, 0,
+                             false, u->stream));
+  return PIE_OK;
+}
+
+int forward(pie_unet* u) {
+  hipStream_t st = u->stream;
+  const int B = u->B, H = u->H, W = u->W;
+  PIE_HIP(pie::unet::to_bf16(u->x32.p, (int)u->P[0], 8, sl(u->in, 8, 0, 8),
+                             st));
+  PIE_CHECK(conv_fwd(u, E0A, sl(u->in, 8, 0, 8), sl(u->e0a, 32, 0, 32)));
+  PIE_CHECK(conv_fwd(u, E0B, sl(u->e0a, 32, 0, 32), sl(u->c0, 96, 64, 32)));
+  PIE_HIP(pie::unet::pool_fwd(sl(u->c0, 96, 64, 32), B, H, W,
+                              sl(u->p0, 32, 0, 32), st));
+  PIE_CHECK(conv_fwd(u, E1A, sl(u->p0, 32, 0, 32), sl(u->e1a, 64, 0, 64)));
+  PIE_CHECK(conv_fwd(u, E1B, sl(u->e1a, 64, 0, 64), sl(u->c1, 192, 128, 64)));
+  PIE_HIP(pie::unet::pool_fwd(sl(u->c1, 192, 128, 64), B, H / 2, W / 2,
+                              sl(u->p1, 64, 0, 64), st));
+  PIE_CHECK(conv_fwd(u, E2A, sl(u->p1, 64, 0, 64), sl(u->e2a, 128, 0, 128)));
+  PIE_CHECK(conv_fwd(u, E2B, sl(u->e2a, 128, 0, 128),
+                     sl(u->c2, 384, 256, 128)));
+  PIE_HIP(pie::unet::pool_fwd(sl(u->c2, 384, 256, 128), B, H / 4, W / 4,
+                              sl(u->p2, 128, 0, 128), st));
+  PIE_CHECK(conv_fwd(u, BNA, sl(u->p2, 128, 0, 128), sl(u->bna, 256, 0, 256)));
+  PIE_CHECK(conv_fwd(u, BNB, sl(u->bna, 256, 0, 256), sl(u->bnb, 256, 0, 256)));
+  PIE_HIP(pie::unet::up_fwd(sl(u->bnb, 256, 0, 256), B, H / 8, W / 8,
+                            sl(u->c2, 384, 0, 256), st));
+  PIE_CHECK(conv_fwd(u, D2A, sl(u->c2, 384, 0, 384), sl(u->d2a, 128, 0, 128)));
+  PIE_CHECK(conv_fwd(u, D2B, sl(u->d2a, 128, 0, 128), sl(u->d2b, 128, 0, 128)));
+  PIE_HIP(pie::unet::up_fwd(sl(u->d2b, 128, 0, 128), B, H / 4, W / 4,
+                            sl(u->c1, 192, 0, 128), st));
+  PIE_CHECK(conv_fwd(u, D1A, sl(u->c1, 192, 0, 192), sl(u->d1a, 64, 0, 64)));
+  PIE_CHECK(conv_fwd(u, D1B, sl(u->d1a, 64, 0, 64), sl(u->d1b, 64, 0, 64)));
+  PIE_HIP(pie::unet::up_fwd(sl(u->d1b, 64, 0, 64), B, H / 2, W / 2,
+                            sl(u->c0, 96, 0, 64), st));
+  PIE_CHECK(conv_fwd(u, D0A, sl(u->c0, 96, 0, 96), sl(u->d0a, 32, 0, 32)));
+  PIE_CHECK(conv_fwd(u, D0B, sl(u->d0a, 32, 0, 32), sl(u->d0b, 32, 0, 32)));
+  const long long hw = u->woff[kNumConv];  // head weights
+  PIE_HIP(pie::unet::head_fwd(sl(u->d0b, 32, 0, 32), (int)u->P[0],
+                              u->param.p + hw, u->param.p + hw + 96,
+                              u->out.p, st));
+  return PIE_OK;
+}
+
+// Backward of conv l: upstream gradient g (fp32 slice), output y (bf16
+// slice), input x (bf16 slice); writes dX (fp32, Cin channels) into dx
+// unless dx is null.
+int conv_bwd(pie_unet* u, int l, const float* g, int g_stride, int g_off,
+             const Slice& y, const Slice& x, float* dx) {
+  const ConvSpec& c = kConvs[l];
+  const int s = 1 << c.level;
+  const int h = u->H / s, w = u->W / s;
+  const long long P = u->P[c.level];
+  hipStream_t st = u->stream;
+  PIE_HIP(pie::unet::relu_grad(g, g_stride, g_off, y, (int)P, u->dyb.p, st));
+  PIE_HIP(pie::unet::conv3x3_wgrad(x, u->B, h, w, u->dyb.p, c.cout,
+                                   u->grad.p + u->woff[l],
+                                   u->grad.p + u->boff[l], st));
+  if (dx) {
+    Slice dys{u->dyb.p, c.cout, 0, c.cout};
+    PIE_HIP(pie::unet::conv3x3(dys, u->B, h, w, u->wkt[l].p, c.cin, nullptr,
+                               false, nullptr, dx, c.cin, false, st));
+  }
+  return PIE_OK;
+}
+
+int backward(pie_unet* u) {
+  hipStream_t st = u->stream;
+  const int B = u->B, H = u->H, W = u->W;
+  const long long hw = u->woff[kNumConv];
+  PIE_HIP(hipMemsetAsync(u->grad.p, 0, u->nparam * sizeof(float), st));
+  // head: dout -> gA [P0][32]
+  PIE_HIP(pie::unet::head_bwd(sl(u->d0b, 32, 0, 32), (int)u->P[0],
+                              u->param.p + hw, u->dout.p, u->grad.p + hw,
+                              u->grad.p + hw + 96, u->gA.p, st));
+  PIE_CHECK(conv_bwd(u, D0B, u->gA.p, 32, 0, sl(u->d0b, 32, 0, 32),
+                     sl(u->d0a, 32, 0, 32), u->gB.p));
+  PIE_CHECK(conv_bwd(u, D0A, u->gB.p, 32, 0, sl(u->d0a, 32, 0, 32),
+                     sl(u->c0, 96, 0, 96), u->gC0.p));
+  PIE_HIP(pie::unet::up_bwd(u->gC0.p, 96, B, H / 2, W / 2, 64, u->gA.p, st));
+  PIE_CHECK(conv_bwd(u, D1B, u->gA.p, 64, 0, sl(u->d1b, 64, 0, 64),
+                     sl(u->d1a, 64, 0, 64), u->gB.p));
+  PIE_CHECK(conv_bwd(u, D1A, u->gB.p, 64, 0, sl(u->d1a, 64, 0, 64),
+                     sl(u->c1, 192, 0, 192), u->gC1.p));
+  PIE_HIP(pie::unet::up_bwd(u->gC1.p, 192, B, H / 4, W / 4, 128, u->gA.p, st));
+  PIE_CHECK(conv_bwd(u, D2B, u->gA.p, 128, 0, sl(u->d2b, 128, 0, 128),
+                     sl(u->d2a, 128, 0, 128), u->gB.p));
+  PIE_CHECK(conv_bwd(u, D2A, u->gB.p, 128, 0, sl(u->d2a, 128, 0, 128),
+                     sl(u->c2, 384, 0, 384), u->gC2.p));
+  PIE_HIP(pie::unet::up_bwd(u->gC2.p, 384, B, H / 8, W / 8, 256, u->gA.p, st));
+  PIE_CHECK(conv_bwd(u, BNB, u->gA.p, 256, 0, sl(u->bnb, 256, 0, 256),
+                     sl(u->bna, 256, 0, 256), u->gB.p));
+  PIE_CHECK(conv_bwd(u, BNA, u->gB.p, 256, 0, sl(u->bna, 256, 0, 256),
+                     sl(u->p2, 128, 0, 128), u->gP.p));
+  PIE_HIP(pie::unet::pool_bwd(u->gP.p, B, H / 8, W / 8, 128, u->gC2.p + 256,
+                              384, st));
+  PIE_CHECK(conv_bwd(u, E2B, u->gC2.p, 384, 256, sl(u->c2, 384, 256, 128),
+                     sl(u->e2a, 128, 0, 128), u->gB.p));
+  PIE_CHECK(conv_bwd(u, E2A, u->gB.p, 128, 0, sl(u->e2a, 128, 0, 128),
+                     sl(u->p1, 64, 0, 64), u->gP.p));
+  PIE_HIP(pie::unet::pool_bwd(u->gP.p, B, H / 4, W / 4, 64, u->gC1.p + 128,
+                              192, st));
+  PIE_CHECK(conv_bwd(u, E1B, u->gC1.p, 192, 128, sl(u->c1, 192, 128, 64),
+                     sl(u->e1a, 64, 0, 64), u->gB.p));
+  PIE_CHECK(conv_bwd(u, E1A, u->gB.p, 64, 0, sl(u->e1a, 64, 0, 64),
+                     sl(u->p0, 32, 0, 32), u->gP.p));
+  PIE_HIP(pie::unet::pool_bwd(u->gP.p, B, H / 2, W / 2, 32, u->gC0.p + 64, 96,
+                              st));
+  PIE_CHECK(conv_bwd(u, E0B, u->gC0.p, 96, 64, sl(u->c0, 96, 64, 32),
+                     sl(u->e0a, 32, 0, 32), u->gB.p));
+  PIE_CHECK(conv_bwd(u, E0A, u->gB.p, 32, 0, sl(u->e0a, 32, 0, 32),
+                     sl(u->in, 8, 0, 8), nullptr));
+  return PIE_OK;
+}
+
+int load_batch(pie_unet* u, const float* x, const float* target,
+               const uint8_t* mask) {
+  const long long P0 = u->P[0];
+  PIE_HIP(hipMemcpyAsync(u->x32.p, x, P0 * 8 * sizeof(float),
+                         hipMemcpyHostToDevice, u->stream));
+  if (target)
+    PIE_HIP(hipMemcpyAsync(u->target.p, target, P0 * 3 * sizeof(float),
+                           hipMemcpyHostToDevice, u->stream));
+  if (mask)
+    PIE_HIP(hipMemcpyAsync(u->mask.p, mask, P0, hipMemcpyHostToDevice,
+                           u->stream));
+  return PIE_OK;
+}
+
+int loss_and_grad(pie_unet* u, float* loss) {
+  PIE_HIP(hipMemsetAsync(u->acc.p, 0, 2 * sizeof(double), u->stream));
+  PIE_HIP(pie::unet::mse(u->out.p, u->target.p, u->mask.p, (int)u->P[0],
+                         u->dout.p, u->acc.p, u->stream));
+  PIE_HIP(pie::unet::scale(u->dout.p, u->P[0] * 3, u->acc.p, u->stream));
+  PIE_CHECK(backward(u));
+  double acc[2];
+  PIE_HIP(hipMemcpyAsync(acc, u->acc.p, sizeof(acc), hipMemcpyDeviceToHost,
+                         u->stream));
+  PIE_HIP(hipStreamSynchronize(u->stream));
+  if (loss) *loss = (float)(acc[0] / std::max(1.0, acc[1]));
+  return PIE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pie_unet_create(int batch, int h, int w, pie_unet** out) {
+  if (!out) return fail(PIE_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (batch < 1 || h < 8 || w < 8 || h % 8 || w % 8)
+    return fail(PIE_EINVAL, "batch >= 1 and h, w positive multiples of 8");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
+    return fail(PIE_EHIP, "no HIP device");
+  std::unique_ptr<pie_unet> u(new pie_unet());
+  u->B = batch;
+  u->H = h;
+  u->W = w;
+  for (int l = 0; l < 4; ++l)
+    u->P[l] = (long long)batch * (h >> l) * (w >> l);
+  long long off = 0;
+  for (int l = 0; l < kNumConv; ++l) {
+    u->woff[l] = off;
+    off += 9LL * kConvs[l].cin * kConvs[l].cout;
+    u->boff[l] = off;
+    off += kConvs[l].cout;
+  }
+  u->woff[kNumConv] = off;  // head: w[32][3], b[3]
+  u->boff[kNumConv] = off + 96;
+  off += 99;
+  u->nparam = off;
+  PIE_HIP(hipStreamCreateWithFlags(&u->stream, hipStreamNonBlocking));
+  PIE_HIP(hipEventCreate(&u->ev0));
+  PIE_HIP(hipEventCreate(&u->ev1));
+  PIE_HIP(u->param.alloc(off));
+  PIE_HIP(u->grad.alloc(off));
+  PIE_HIP(u->adam_m.alloc(off));
+  PIE_HIP(u->adam_v.alloc(off));
+  PIE_HIP(hipMemsetAsync(u->param.p, 0, off * sizeof(float), u->stream));
+  PIE_HIP(hipMemsetAsync(u->adam_m.p, 0, off * sizeof(float), u->stream));
+  PIE_HIP(hipMemsetAsync(u->adam_v.p, 0, off * sizeof(float), u->stream));
+  for (int l = 0; l < kNumConv; ++l) {
+    PIE_HIP(u->wk[l].alloc(9LL * kConvs[l].cin * kConvs[l].cout));
+    PIE_HIP(u->wkt[l].alloc(9LL * kConvs[l].cin * kConvs[l].cout));
+  }
+  PIE_CHECK(alloc_all(u.get()));
+  PIE_HIP(hipStreamSynchronize(u->stream));
+  *out = u.release();
+  return PIE_OK;
+}
+
+int pie_unet_destroy(pie_unet* u) {
+  if (!u) return PIE_OK;
+  if (u->stream) (void)hipStreamSynchronize(u->stream);
+  if (u->ev0) (void)hipEventDestroy(u->ev0);
+  if (u->ev1) (void)hipEventDestroy(u->ev1);
+  hipStream_t st = u->stream;
+  delete u;
+  if (st) (void)hipStreamDestroy(st);
+  return PIE_OK;
+}
+
+int pie_unet_param_count(pie_unet* u, long long* n) {
+  if (!u || !n) return fail(PIE_EINVAL, "NULL argument");
+  *n = u->nparam;
+  return PIE_OK;
+}
+
+int pie_unet_set_params(pie_unet* u, const float* p) {
+  if (!u || !p) return fail(PIE_EINVAL, "NULL argument");
+  PIE_HIP(hipMemcpyAsync(u->param.p, p, u->nparam * sizeof(float),
+                         hipMemcpyHostToDevice, u->stream));
+  PIE_HIP(hipMemsetAsync(u->adam_m.p, 0, u->nparam * sizeof(float),
+                         u->stream));
+  PIE_HIP(hipMemsetAsync(u->adam_v.p, 0, u->nparam * sizeof(float),
+                         u->stream));
+  u->step = 0;
+  PIE_CHECK(pack_all(u));
+  PIE_HIP(hipStreamSynchronize(u->stream));
+  return PIE_OK;
+}
+
+int pie_unet_get_params(pie_unet* u, float* p) {
+  if (!u || !p) return fail(PIE_EINVAL, "NULL argument");
+  PIE_HIP(hipMemcpyAsync(p, u->param.p, u->nparam * sizeof(float),
+                         hipMemcpyDeviceToHost, u->stream));
+  PIE_HIP(hipStreamSynchronize(u->stream));
+  return PIE_OK;
+}
+
+int pie_unet_forward(pie_unet* u, const float* x, float* out) {
+  if (!u || !x || !out) return fail(PIE_EINVAL, "NULL argument");
+  PIE_CHECK(load_batch(u, x, nullptr, nullptr));
+  PIE_CHECK(forward(u));
+  PIE_HIP(hipMemcpyAsync(out, u->out.p, u->P[0] * 3 * sizeof(float),
+                         hipMemcpyDeviceToHost, u->stream));
+  PIE_HIP(hipStreamSynchronize(u->stream));
+  return PIE_OK;
+}
+
+int pie_unet_loss_grad(pie_unet* u, const float* x, const float* target,
+                       const uint8_t* mask, float* loss, float* grads) {
+  if (!u || !x || !target || !mask) return fail(PIE_EINVAL, "NULL argument");
+  PIE_CHECK(load_batch(u, x, target, mask));
+  PIE_CHECK(forward(u));
+  PIE_CHECK(loss_and_grad(u, loss));
+  if (grads) {
+    PIE_HIP(hipMemcpyAsync(grads, u->grad.p, u->nparam * sizeof(float),
+                           hipMemcpyDeviceToHost, u->stream));
+    PIE_HIP(hipStreamSynchronize(u->stream));
+  }
+  return PIE_OK;
+}
+
+int pie_unet_train_step(pie_unet* u, const float* x, const float* target,
+                        const uint8_t* mask, float lr, float* loss) {
+  if (!u || !x || !target || !mask) return fail(PIE_EINVAL, "NULL argument");
+  PIE_CHECK(load_batch(u, x, target, mask));
+  PIE_HIP(hipEventRecord(u->ev0, u->stream));
+  PIE_CHECK(forward(u));
+  PIE_CHECK(loss_and_grad(u, loss));  // synchronises
+  PIE_HIP(hipEventRecord(u->ev1, u->stream));
+  ++u->step;
+  PIE_HIP(pie::unet::adam(u->param.p, u->grad.p, u->adam_m.p, u->adam_v.p,
+                          u->nparam, lr, 0.9f, 0.999f, 1e-8f, u->step,
+                          u->stream));
+  PIE_CHECK(pack_all(u));
+  PIE_HIP(hipStreamSynchronize(u->stream));
+  float ms = 0.0f;
+  PIE_HIP(hipEventElapsedTime(&ms, u->ev0, u->ev1));
+  u->ms_total += ms;
+  u->flops_total += conv_flops(u, true);
+  return PIE_OK;
+}
+
+int pie_unet_stats(pie_unet* u, double* ms, double* conv_flops_done) {
+  if (!u) return fail(PIE_EINVAL, "NULL argument");
+  if (ms) *ms = u->ms_total;
+  if (conv_flops_done) *conv_flops_done = u->flops_total;
+  return PIE_OK;
+}
+
+}  // extern "C"

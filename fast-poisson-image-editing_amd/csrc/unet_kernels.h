@@ -1,0 +1,80 @@
+// Kernels of the warm-start U-Net (bf16 activations and weights, fp32
+// accumulation on MFMA).  Activation tensors are NHWC bf16 with an explicit
+// channel stride and offset, so a convolution can read from or write into a
+// slice of a concatenation buffer.  Gradient tensors are NHWC fp32.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pie {
+namespace unet {
+
+typedef uint16_t bf16_t;  // raw bfloat16 bits
+
+// A channel slice of an NHWC tensor: element (pixel p, channel c) at
+// base[p * stride + off + c].
+struct Slice {
+  void* base;
+  int stride, off, ch;
+};
+
+// 3x3 convolution with zero padding as an implicit GEMM on
+// v_mfma_f32_16x16x32_bf16: out[p][co] = sum_k A[p][k] B[k][co] with
+// k = 8 * kb + e, kb = tap * (Cin / 8) + cb, A = in[p + offset(tap)][8cb+e].
+// wk holds B as [kb][Cout][8] bf16.  Output: bf16 (+bias, optional ReLU)
+// into out_bf, and/or fp32 into out_f32 (optionally accumulated).
+hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
+                   int cout, const float* bias, bool relu, const Slice* out_bf,
+                   float* out_f32, int out_f32_stride, bool accumulate,
+                   hipStream_t st);
+
+// dW[tap][ci][co] += sum_p x[p + offset(tap)][ci] * dy[p][co] (fp32
+// atomics) and db[co] += sum_p dy[p][co]; x is a bf16 slice, dy bf16 with
+// Cout contiguous channels.
+hipError_t conv3x3_wgrad(const Slice& x, int n, int h, int w,
+                         const bf16_t* dy, int cout, float* dw, float* db,
+                         hipStream_t st);
+
+// Pack fp32 master weights w[tap][ci][co] into the forward operand
+// wk[kb][co][8] (kb = tap*(ci/8)+cb) and the data-gradient operand
+// wkt[kb'][ci][8] (kb' = tap'*(co/8)+ob, value w[8-tap'][ci][8ob+e]).
+hipError_t pack_weights(const float* w, int cin, int cout, bf16_t* wk,
+                        bf16_t* wkt, hipStream_t st);
+
+// input fp32 [P][8] -> bf16 slice
+hipError_t to_bf16(const float* x, int p, int ch, const Slice& out,
+                   hipStream_t st);
+// 2x2 average pool bf16 -> bf16 (h, w: input size, even)
+hipError_t pool_fwd(const Slice& in, int n, int h, int w, const Slice& out,
+                    hipStream_t st);
+// dx[n][2i+a][2j+b][c] += dy[n][i][j][c] / 4   (fp32, dx stride cs)
+hipError_t pool_bwd(const float* dy, int n, int h2, int w2, int ch,
+                    float* dx, int dx_stride, hipStream_t st);
+// 2x nearest upsample bf16 -> bf16 slice (h, w: input size)
+hipError_t up_fwd(const Slice& in, int n, int h, int w, const Slice& out,
+                  hipStream_t st);
+// dx[n][i][j][c] = sum_ab dy[n][2i+a][2j+b][c] (fp32 slice of dy)
+hipError_t up_bwd(const float* dy, int dy_stride, int n, int h, int w, int ch,
+                  float* dx, hipStream_t st);
+// g[p][c] * (y[p][c] > 0) -> bf16 [P][ch] (ReLU gradient, ready for MFMA)
+hipError_t relu_grad(const float* g, int g_stride, int g_off,
+                     const Slice& y, int p, bf16_t* out, hipStream_t st);
+// head: out[p][0..3) = sum_c x[p][c] * bf16(w[c][o]) + b[o]  (x bf16 32 ch)
+hipError_t head_fwd(const Slice& x, int p, const float* w, const float* b,
+                    float* out, hipStream_t st);
+// head backward: dw, db (atomics), dx[p][c] = sum_o dout[p][o] bf16(w[c][o])
+hipError_t head_bwd(const Slice& x, int p, const float* w, const float* dout,
+                    float* dw, float* db, float* dx, hipStream_t st);
+// masked mean squared error over solved pixels: loss_sum[0] += sum d^2,
+// count[0] += solved * 3, dout = 2 d (to be scaled by 1/count)
+hipError_t mse(const float* out, const float* target, const uint8_t* mask,
+               int p, float* dout, double* acc, hipStream_t st);
+hipError_t scale(float* x, long long n, const double* acc, hipStream_t st);
+// Adam on fp32 master weights
+hipError_t adam(float* p, const float* g, float* m, float* v, long long n,
+                float lr, float b1, float b2, float eps, int t,
+                hipStream_t st);
+
+}  // namespace unet
+}  // namespace pie

@@ -184,6 +184,24 @@ class GridSolver(_Core):
     native.check(self._lib.pie_get_state(self._h, _ptr(out)))
     return out
 
+  def set_state(self, x: np.ndarray) -> None:
+    """Replace the state (a warm start) by float32 x[h][w][3] (extension)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    if x.shape != self.hw + (3,):
+      raise ValueError(f"expected state of shape {self.hw + (3,)}")
+    native.check(self._lib.pie_set_state(self._h, _ptr(x)))
+
+  def solve_to_residual(self, tol_mean: float, check_every: int = 256,
+                        max_sweeps: int = 10**7) -> tuple[int, float]:
+    """Sweep until the mean absolute residual is <= tol_mean (extension).
+    Returns (sweeps run, final mean residual)."""
+    n = ctypes.c_longlong()
+    r = ctypes.c_double()
+    native.check(self._lib.pie_solve_to_residual(
+        self._h, float(tol_mean), int(check_every), int(max_sweeps),
+        ctypes.byref(n), ctypes.byref(r)))
+    return n.value, r.value
+
 
 class BatchSolver:
   """Batched GridSolver (extension): composites of uneven size and mask,

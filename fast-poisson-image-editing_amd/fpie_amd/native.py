@@ -49,6 +49,22 @@ _SIGNATURES = {
                    ctypes.POINTER(_ip), ctypes.POINTER(ctypes.c_longlong)],
                   _ip),
     "pie_device_info": ([ctypes.c_char_p, _ip], _ip),
+    "pie_set_state": ([_vp, _vp], _ip),
+    "pie_solve_to_residual": ([_vp, ctypes.c_double, _ip, ctypes.c_longlong,
+                               ctypes.POINTER(ctypes.c_longlong),
+                               ctypes.POINTER(ctypes.c_double)], _ip),
+    "pie_unet_create": ([_ip, _ip, _ip, ctypes.POINTER(_vp)], _ip),
+    "pie_unet_destroy": ([_vp], _ip),
+    "pie_unet_param_count": ([_vp, ctypes.POINTER(ctypes.c_longlong)], _ip),
+    "pie_unet_set_params": ([_vp, _vp], _ip),
+    "pie_unet_get_params": ([_vp, _vp], _ip),
+    "pie_unet_forward": ([_vp, _vp, _vp], _ip),
+    "pie_unet_loss_grad": ([_vp, _vp, _vp, _vp,
+                            ctypes.POINTER(ctypes.c_float), _vp], _ip),
+    "pie_unet_train_step": ([_vp, _vp, _vp, _vp, ctypes.c_float,
+                             ctypes.POINTER(ctypes.c_float)], _ip),
+    "pie_unet_stats": ([_vp, ctypes.POINTER(ctypes.c_double),
+                        ctypes.POINTER(ctypes.c_double)], _ip),
     "pie_batch_create": ([_ip, _ip, _ip, _ip, ctypes.POINTER(_vp)], _ip),
     "pie_batch_destroy": ([_vp], _ip),
     "pie_batch_add": ([_vp, _ip, _ip, _vp, _vp, _vp, _ip, ctypes.c_float,

@@ -105,8 +105,39 @@ def prepare_mask(mask: np.ndarray):
   return m[x0:x1, y0:y1], x0, x1, y0, y1
 
 
+def mixgrad(a: np.ndarray, b: np.ndarray, gradient: str) -> np.ndarray:
+  """Source gradient ('src'), mean of both ('avg'), or the larger in
+  magnitude ('max', ties keep the source), elementwise in float32
+  (fpie/process.py:113-122)."""
+  if gradient == "src":
+    return a
+  if gradient == "avg":
+    return (a + b) / 2
+  return np.where(np.abs(a) < np.abs(b), b, a)
+
+
 # neighbour offsets (drow, dcol)
 _UP, _DOWN, _LEFT, _RIGHT = (-1, 0), (1, 0), (0, -1), (0, 1)
+
+
+def divergence(src: np.ndarray, tgt: np.ndarray, mask: np.ndarray,
+               mixgrad) -> np.ndarray:
+  """GridSolver guidance (fpie/process.py:362-378): at every solved pixel
+  the sum over its four neighbours of mixgrad(src_c - src_n, tgt_c - tgt_n),
+  accumulated up, down, left, right; zero off the mask.  src and tgt are
+  float32 [h, w, 3]; pixels without a neighbour in a direction get no term
+  for it."""
+  h, w = mask.shape
+  grad = np.zeros((h, w, 3), np.float32)
+  for d in (_UP, _DOWN, _LEFT, _RIGHT):
+    rs = slice(max(0, -d[0]), h - max(0, d[0]))
+    cs = slice(max(0, -d[1]), w - max(0, d[1]))
+    rn = slice(rs.start + d[0], rs.stop + d[0])
+    cn = slice(cs.start + d[1], cs.stop + d[1])
+    grad[rs, cs] += mixgrad(src[rs, cs] - src[rn, cn],
+                            tgt[rs, cs] - tgt[rn, cn])
+  grad[mask == 0] = 0
+  return grad
 
 
 class BaseProcessor(ABC):
@@ -133,13 +164,7 @@ class BaseProcessor(ABC):
     return entry[which](**kw) if entry is not None else None
 
   def mixgrad(self, a: np.ndarray, b: np.ndarray) -> np.ndarray:
-    """Source gradient, mean of both, or the larger in magnitude (ties keep
-    the source), elementwise in float32."""
-    if self.gradient == "src":
-      return a
-    if self.gradient == "avg":
-      return (a + b) / 2
-    return np.where(np.abs(a) < np.abs(b), b, a)
+    return mixgrad(a, b, self.gradient)
 
   @abstractmethod
   def reset(self, src: np.ndarray, mask: np.ndarray, tgt: np.ndarray,
@@ -235,17 +260,7 @@ class GridProcessor(BaseProcessor):
              mask_on_src[1] + y0:mask_on_src[1] + y1].astype(np.float32)
     tc = tgt[mask_on_tgt[0] + x0:mask_on_tgt[0] + x1,
              mask_on_tgt[1] + y0:mask_on_tgt[1] + y1].astype(np.float32)
-    grad = np.zeros((h, w, 3), np.float32)
-    # centre minus neighbour, accumulated up, down, left, right
-    # (process.py:363-376); pixels without that neighbour get nothing
-    for d in (_UP, _DOWN, _LEFT, _RIGHT):
-      rs = slice(max(0, -d[0]), h - max(0, d[0]))
-      cs = slice(max(0, -d[1]), w - max(0, d[1]))
-      rn = slice(rs.start + d[0], rs.stop + d[0])
-      cn = slice(cs.start + d[1], cs.stop + d[1])
-      grad[rs, cs] += self.mixgrad(sc[rs, cs] - sc[rn, cn],
-                                   tc[rs, cs] - tc[rn, cn])
-    grad[m == 0] = 0
+    grad = divergence(sc, tc, m, self.mixgrad)
     self.x0, self.x1 = mask_on_tgt[0] + x0, mask_on_tgt[0] + x1
     self.y0, self.y1 = mask_on_tgt[1] + y0, mask_on_tgt[1] + y1
     self.tgt = tgt.copy()

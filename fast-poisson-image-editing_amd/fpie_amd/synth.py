@@ -66,22 +66,35 @@ def square_mask(h: int, w: int, side: int) -> np.ndarray:
   return m
 
 
+def _ellipses(h, w, params):
+  yy = np.arange(h, dtype=np.float32)[:, None]
+  xx = np.arange(w, dtype=np.float32)[None, :]
+  m = np.zeros((h, w), bool)
+  for cy, cx, ay, ax, th in params:
+    c, s = np.float32(np.cos(th)), np.float32(np.sin(th))
+    dy, dx = yy - np.float32(cy * h), xx - np.float32(cx * w)
+    u = (dy * c + dx * s) / np.float32(ay * h)
+    v = (dx * c - dy * s) / np.float32(ax * w)
+    m |= u * u + v * v <= 1.0
+  return m
+
+
 def ellipse_mask(h: int, w: int, seed: int, n: int = 6,
                  coverage: float = 0.60, tol: float = 0.02) -> np.ndarray:
-  """Union of n random ellipses, redrawn until the covered fraction is
-  within coverage +- tol."""
+  """Union of n random ellipses (centres in the middle 70 %, semi-axes 12-45
+  % of the side, any orientation), parameters redrawn until the covered
+  fraction, measured on a 256-px copy and then on the full grid, is within
+  coverage +- tol."""
   rng = np.random.default_rng(seed)
-  yy, xx = np.ogrid[0:h, 0:w]
+  f = max(1, max(h, w) // 256)
   for _ in range(1000):
-    m = np.zeros((h, w), bool)
-    for _ in range(n):
-      cy, cx = rng.uniform(0.15, 0.85) * h, rng.uniform(0.15, 0.85) * w
-      ay, ax = rng.uniform(0.12, 0.45) * h, rng.uniform(0.12, 0.45) * w
-      th = rng.uniform(0, np.pi)
-      c, s = np.cos(th), np.sin(th)
-      u = (yy - cy) * c + (xx - cx) * s
-      v = -(yy - cy) * s + (xx - cx) * c
-      m |= (u / ay)**2 + (v / ax)**2 <= 1.0
+    params = [(rng.uniform(0.15, 0.85), rng.uniform(0.15, 0.85),
+               rng.uniform(0.12, 0.45), rng.uniform(0.12, 0.45),
+               rng.uniform(0, np.pi)) for _ in range(n)]
+    if abs(_ellipses(max(8, h // f), max(8, w // f), params).mean() -
+           coverage) > tol:
+      continue
+    m = _ellipses(h, w, params)
     if abs(m.mean() - coverage) <= tol:
       return m.astype(np.uint8) * 255
   raise RuntimeError("no ellipse union within the coverage band")

@@ -133,6 +133,15 @@ int pie_get_timing(pie_solver* s, double* kernel_ms, long long* launches,
 int pie_shape(pie_solver* s, int* h, int* w, int* n, long long* masked);
 /* Name and properties of the current device (cuda/utils.cu:5-22). */
 int pie_device_info(char* buf, int len);
+/* GridSolver: replace the state by x float[h][w][3] (a warm start; solved
+ * pixels take x, unsolved pixels should hold the target values). */
+int pie_set_state(pie_solver* s, const float* x);
+/* Sweep in chunks of check_every until the mean absolute residual
+ * err / (3 * solved pixels) is <= tol_mean or max_sweeps sweeps are done;
+ * returns the sweeps run and the final mean residual. */
+int pie_solve_to_residual(pie_solver* s, double tol_mean, int check_every,
+                          long long max_sweeps, long long* sweeps,
+                          double* mean_residual);
 
 /* ---- batched GridSolver: no counterpart in the reference ---------------
  * Many composites of uneven size and mask, each with its own stop rule,
@@ -163,6 +172,33 @@ int pie_batch_item(pie_batch* b, int i, int* sweeps, float* last_dmax,
 int pie_batch_read(pie_batch* b, int i, uint8_t* img, float* err,
                    float* state);
 int pie_batch_size(pie_batch* b, int* items, int* tasks, long long* masked);
+
+/* ---- warm-start U-Net: no counterpart in the reference -----------------
+ * 4-level U-Net (base 32 channels, 3x3 convolutions, bf16 weights and
+ * activations, fp32 accumulation on MFMA) predicting the solution's
+ * correction from 8 input channels: divergence / 64, (target - 128) / 64,
+ * mask, 0.  Tensors are NHWC float32 on the host: x [B][H][W][8], target
+ * and output [B][H][W][3], mask uint8 [B][H][W].  Parameters: one float32
+ * array (pie_unet_param_count), per conv layer w[tap][cin][cout] then
+ * b[cout], then the 1x1 head w[32][3], b[3]. */
+typedef struct pie_unet pie_unet;
+/* batch >= 1; h, w multiples of 8.  Parameters start at zero. */
+int pie_unet_create(int batch, int h, int w, pie_unet** out);
+int pie_unet_destroy(pie_unet* u);
+int pie_unet_param_count(pie_unet* u, long long* n);
+int pie_unet_set_params(pie_unet* u, const float* p);
+int pie_unet_get_params(pie_unet* u, float* p);
+int pie_unet_forward(pie_unet* u, const float* x, float* out);
+/* Masked mean squared error over solved pixels and its parameter gradient
+ * (grads may be NULL). */
+int pie_unet_loss_grad(pie_unet* u, const float* x, const float* target,
+                       const uint8_t* mask, float* loss, float* grads);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the batch. */
+int pie_unet_train_step(pie_unet* u, const float* x, const float* target,
+                        const uint8_t* mask, float lr, float* loss);
+/* Device time of the forward + backward passes so far and the convolution
+ * FLOPs they did (for MFMA utilisation). */
+int pie_unet_stats(pie_unet* u, double* ms, double* conv_flops);
 
 #ifdef __cplusplus
 }
