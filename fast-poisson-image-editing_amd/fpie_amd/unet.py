@@ -205,43 +205,50 @@ def composite(size: tuple[int, int], seed: int, gradient: str | None = None):
 
 
 def _composite_job(args):
-  size, seed = args
-  m, t, g, _ = composite(size, seed)
-  return m.astype(np.uint8), t.astype(np.uint8), (2 * g).astype(np.int16)
+  size, render, seed = args
+  m, t, g, _ = composite((render, render), seed)
+  if render != size:
+    m, t, g = coarsen(m, t, g, render // size)
+  return m.astype(np.uint8), t.astype(np.float16), g.astype(np.float32)
 
 
 def training_set(n: int, size: int = 256, seed0: int = 0,
                  sweeps: int = 20000, workers: int = 16, chunk: int = 512,
-                 log=print):
-  """n composites and their Jacobi labels (sweeps sweeps from the target
-  state, solved by the batched GPU solver).  Stored compactly: mask uint8,
-  target uint8, 2 * divergence int16 (divergences are multiples of 1/2),
-  label correction float16 ((x - target) / 64)."""
+                 render: int | None = None, log=print):
+  """n composites of size x size and their Jacobi labels (sweeps sweeps from
+  the target state, solved by the batched GPU solver).  With render > size
+  each composite is drawn at render x render and coarsened (coarsen()), the
+  form in which warm_start() shows large problems to the network.  Stored
+  compactly: mask uint8, target float16, divergence float32, label
+  correction float16 ((x - target) / 64)."""
+  render = render or size
   from concurrent.futures import ProcessPoolExecutor
 
   from .core_hip import BatchSolver
 
   t0 = time.perf_counter()
-  jobs = [((size, size), seed0 + i) for i in range(n)]
+  jobs = [(size, render, seed0 + i) for i in range(n)]
   with ProcessPoolExecutor(max_workers=workers) as ex:
     parts = list(ex.map(_composite_job, jobs, chunksize=16))
   M = np.stack([p[0] for p in parts])
   T = np.stack([p[1] for p in parts])
   G2 = np.stack([p[2] for p in parts])
   del parts
-  log(f"composites: {n} x {size}^2 in {time.perf_counter() - t0:.1f} s")
+  log(f"composites: {n} x {size}^2 (drawn at {render}^2) in "
+      f"{time.perf_counter() - t0:.1f} s")
   Y = np.zeros((n, size, size, 3), np.float16)
   t1 = time.perf_counter()
   for c0 in range(0, n, chunk):
     b = BatchSolver(sweeps_per_launch=8, rows_per_task=64)
     idx = range(c0, min(n, c0 + chunk))
     for i in idx:
-      b.add(M[i], T[i].astype(np.float32), G2[i].astype(np.float32) / 2,
-            max_sweeps=sweeps, tol=0.0)
+      b.add(M[i], T[i].astype(np.float32), G2[i], max_sweeps=sweeps,
+            tol=0.0)
     b.solve(32)
     for k, i in enumerate(idx):
       _, _, st = b.read(k, state=True)
-      Y[i] = ((st - T[i]) / SCALE * (M[i][..., None] > 0)).astype(np.float16)
+      Y[i] = ((st - T[i].astype(np.float32)) / SCALE *
+              (M[i][..., None] > 0)).astype(np.float16)
     del b
   log(f"labels: {sweeps} Jacobi sweeps each in {time.perf_counter() - t1:.1f} s")
   return M, T, G2, Y
@@ -249,7 +256,7 @@ def training_set(n: int, size: int = 256, seed0: int = 0,
 
 def batch_arrays(M, T, G2, Y, idx):
   x = np.zeros((len(idx), M.shape[1], M.shape[2], 8), np.float32)
-  x[..., 0:3] = G2[idx].astype(np.float32) / (2 * SCALE)
+  x[..., 0:3] = G2[idx] / SCALE
   x[..., 3:6] = (T[idx].astype(np.float32) - 128.0) / SCALE
   x[..., 6] = M[idx]
   return x, Y[idx].astype(np.float32), M[idx]
@@ -277,18 +284,68 @@ def train(M, T, G2, Y, epochs: int = 4, batch: int = 64, lr: float = 1e-3,
 # ----------------------------------------------------------- warm start ---
 
 
-def warm_start(params: np.ndarray, mask, tgt, grad,
-               net: UNet | None = None) -> np.ndarray:
-  """Predicted start state x0 [h, w, 3] for one problem."""
+def coarsen(mask, tgt, grad, f: int):
+  """The problem on a grid f times coarser: the mask where most of a block
+  is solved, the target averaged, the divergence summed over the block (the
+  discrete Laplacian of a field sampled every f pixels is f^2 times larger,
+  which the block sum matches).  Sizes are cut to multiples of f."""
+  h, w = (mask.shape[0] // f) * f, (mask.shape[1] // f) * f
+  def blocks(a):
+    return a[:h, :w].reshape(h // f, f, w // f, f, *a.shape[2:])
+  m = (blocks(mask != 0).mean((1, 3)) > 0.5).astype(np.int32)
+  m[[0, -1], :] = 0
+  m[:, [0, -1]] = 0
+  t = blocks(tgt).mean((1, 3)).astype(np.float32)
+  g = (blocks(grad).sum((1, 3)) * (m[..., None] != 0)).astype(np.float32)
+  return m, t, g
+
+
+def _upsample(a: np.ndarray, f: int, h: int, w: int) -> np.ndarray:
+  """Bilinear upsampling of [hc, wc, c] by f (cell-centred), cut to h x w."""
+  hc, wc = a.shape[:2]
+  ys = (np.arange(h) + 0.5) / f - 0.5
+  xs = (np.arange(w) + 0.5) / f - 0.5
+  y0 = np.clip(np.floor(ys).astype(int), 0, hc - 1)
+  x0 = np.clip(np.floor(xs).astype(int), 0, wc - 1)
+  y1, x1 = np.minimum(y0 + 1, hc - 1), np.minimum(x0 + 1, wc - 1)
+  fy = np.clip(ys - y0, 0, 1)[:, None, None]
+  fx = np.clip(xs - x0, 0, 1)[None, :, None]
+  top = a[y0][:, x0] * (1 - fx) + a[y0][:, x1] * fx
+  bot = a[y1][:, x0] * (1 - fx) + a[y1][:, x1] * fx
+  return (top * (1 - fy) + bot * fy).astype(np.float32)
+
+
+def predict(params: np.ndarray, mask, tgt, grad, net: UNet | None = None):
+  """Network correction (x - target) in pixel units, [h, w, 3]."""
   h, w = mask.shape
   x = pad8(make_input(mask, tgt, grad), "edge")
   x[h:, :, 6] = 0
   x[:, w:, 6] = 0
   if net is None or net.shape != (1,) + x.shape[:2]:
     net = UNet(1, x.shape[0], x.shape[1], params=params)
-  out = net.forward(x[None])[0, :h, :w]
-  return np.where(mask[..., None] != 0, tgt + SCALE * out, tgt).astype(
-      np.float32)
+  return SCALE * net.forward(x[None])[0, :h, :w]
+
+
+def warm_start(params: np.ndarray, mask, tgt, grad, coarse: int = 256,
+               net: UNet | None = None) -> np.ndarray:
+  """Predicted start state x0 [h, w, 3] for one problem.
+
+  Problems larger than `coarse` pixels on a side are predicted on a grid
+  coarsened by the power of two that brings them to about that size (the
+  scale the network is trained at), and the predicted correction is
+  upsampled bilinearly: the network supplies the smooth, image-wide part of
+  the solution, which is what Jacobi removes slowest, and Jacobi removes the
+  local remainder quickly."""
+  h, w = mask.shape
+  f = 1
+  while max(h, w) // (2 * f) >= coarse:
+    f *= 2
+  if f == 1:
+    corr = predict(params, mask, tgt, grad, net)
+  else:
+    mc, tc, gc = coarsen(mask, tgt, grad, f)
+    corr = _upsample(predict(params, mc, tc, gc, net), f, h, w)
+  return np.where(mask[..., None] != 0, tgt + corr, tgt).astype(np.float32)
 
 
 def time_to_residual(mask, tgt, grad, x0=None, tol: float = 1e-3,

@@ -74,7 +74,8 @@ def test_gradients_match_oracle(unet):
 
 
 def test_adam_steps_reduce_the_loss(unet):
-  x, y, m = _problem(4, 32, 32, 4)
+  x, _, m = _problem(4, 32, 32, 4)
+  y = (0.5 * x[..., 3:6] - 0.25 * x[..., 0:3]).astype(np.float32)  # learnable
   net = unet.UNet(4, 32, 32, seed=1)
   first = net.train_step(x, y, m, 1e-3)
   for _ in range(40):
