@@ -141,6 +141,21 @@ hipError_t launch_finalize(int order, const float* x, const float* g,
 hipError_t launch_reduce(const double* partial, int n, float* err,
                          hipStream_t st);
 
+// ---- coarse-grid correction (warm start) -------------------------------
+// out[hc][wc][3] = per f x f block, the sum over solved pixels of the
+// signed residual (g + neighbours) - 4 x.
+hipError_t launch_restrict(int order, const float* x, const float* g,
+                           const uint8_t* m, long long plane, int P, int f,
+                           int hc, int wc, float* out, hipStream_t st);
+// x += cell-centred bilinear upsampling of corr[hc][wc][3] at solved pixels
+hipError_t launch_prolong(float* x, const uint8_t* m, long long plane, int H,
+                          int W, int P, int f, int hc, int wc,
+                          const float* corr, hipStream_t st);
+// one weighted Jacobi sweep xin -> xout (weight omega)
+hipError_t launch_weighted(int order, const float* xin, float* xout,
+                           const float* g, const uint8_t* m, long long plane,
+                           int H, int W, int P, float omega, hipStream_t st);
+
 // ---- EquSolver on an arbitrary A (no grid geometry) -------------------
 hipError_t launch_equ_pack(const float* X, float4* out, int N,
                            hipStream_t st);

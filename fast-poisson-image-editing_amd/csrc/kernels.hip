@@ -309,6 +309,120 @@ __global__ void clock_kernel(unsigned long long* out) {
   if (threadIdx.x == 0) *out = wall_clock64();
 }
 
+// ------------------------------------------- coarse-grid correction ----
+
+// float sum g + four neighbours in the solver's order (no rounding other
+// than the float adds, as the sweep kernel)
+template <int ORDER>
+__device__ __forceinline__ float neighbour_sum(const float* __restrict__ xc,
+                                               float gv, long long o, int P) {
+  float s = gv;
+  if (ORDER == ORDER_UDLR) {
+    s = s + xc[o - P];
+    s = s + xc[o + P];
+    s = s + xc[o - 1];
+    s = s + xc[o + 1];
+  } else {
+    s = s + xc[o - P];
+    s = s + xc[o - 1];
+    s = s + xc[o + 1];
+    s = s + xc[o + P];
+  }
+  return s;
+}
+
+// out[I][J][c] = sum over the solved pixels of the f x f block (I, J) of the
+// signed residual (float neighbour sum) - 4 x, accumulated in double in
+// row-major order inside the block.  One thread per coarse cell.
+template <int ORDER>
+__global__ void __launch_bounds__(kBlock)
+    restrict_kernel(const float* __restrict__ x, const float* __restrict__ g,
+                    const uint8_t* __restrict__ m, long long plane, int P,
+                    int f, int hc, int wc, float* __restrict__ out) {
+  const int id = blockIdx.x * kBlock + threadIdx.x;
+  if (id >= hc * wc) return;
+  const int I = id / wc, J = id % wc;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int di = 0; di < f; ++di) {
+    const long long row = (long long)(I * f + di) * P + (long long)J * f;
+    for (int dj = 0; dj < f; ++dj) {
+      const long long o = row + dj;
+      if (!m[o]) continue;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float* xc = x + c * plane;
+        const float s = neighbour_sum<ORDER>(xc, g[c * plane + o], o, P);
+        acc[c] += (double)s - (double)xc[o] * 4.0;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out[id * 3 + c] = (float)acc[c];
+}
+
+// x[i][j] += bilinear sample of corr[hc][wc][3] (cell-centred: fine pixel
+// (i, j) sits at coarse coordinate ((i + 0.5) / f - 0.5, (j + 0.5) / f - 0.5),
+// clamped to the coarse grid) at the solved pixels.
+__global__ void __launch_bounds__(kBlock)
+    prolong_kernel(float* __restrict__ x, const uint8_t* __restrict__ m,
+                   long long plane, int H, int W, int P, int f, int hc, int wc,
+                   const float* __restrict__ corr) {
+  const long long n = (long long)H * W;
+  const float inv = 1.0f / (float)f;
+  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < n;
+       id += (long long)gridDim.x * kBlock) {
+    const int i = (int)(id / W), j = (int)(id % W);
+    const long long o = (long long)i * P + j;
+    if (!m[o]) continue;
+    const float ys = ((float)i + 0.5f) * inv - 0.5f;
+    const float xs = ((float)j + 0.5f) * inv - 0.5f;
+    const int y0 = min(max((int)floorf(ys), 0), hc - 1);
+    const int x0 = min(max((int)floorf(xs), 0), wc - 1);
+    const int y1 = min(y0 + 1, hc - 1), x1 = min(x0 + 1, wc - 1);
+    const float fy = fminf(fmaxf(ys - (float)y0, 0.0f), 1.0f);
+    const float fx = fminf(fmaxf(xs - (float)x0, 0.0f), 1.0f);
+    const float* c00 = corr + ((long long)y0 * wc + x0) * 3;
+    const float* c01 = corr + ((long long)y0 * wc + x1) * 3;
+    const float* c10 = corr + ((long long)y1 * wc + x0) * 3;
+    const float* c11 = corr + ((long long)y1 * wc + x1) * 3;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float top = c00[c] * (1.0f - fx) + c01[c] * fx;
+      const float bot = c10[c] * (1.0f - fx) + c11[c] * fx;
+      x[c * plane + o] += top * (1.0f - fy) + bot * fy;
+    }
+  }
+}
+
+// One weighted Jacobi sweep: xout = xin + omega * ((sum) * 0.25 - xin) at
+// the solved pixels (exactly (sum) * 0.25 when omega is 1, the plain sweep),
+// xout = xin elsewhere (every pixel of the H x W crop).
+template <int ORDER>
+__global__ void __launch_bounds__(kBlock)
+    weighted_kernel(const float* __restrict__ xin, float* __restrict__ xout,
+                    const float* __restrict__ g, const uint8_t* __restrict__ m,
+                    long long plane, int H, int W, int P, float omega) {
+  const long long n = (long long)H * W;
+  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < n;
+       id += (long long)gridDim.x * kBlock) {
+    const int i = (int)(id / W), j = (int)(id % W);
+    const long long o = (long long)i * P + j;
+    const bool solved = m[o] != 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float* xc = xin + c * plane;
+      const float v = xc[o];
+      float r = v;
+      if (solved) {
+        const float jv = neighbour_sum<ORDER>(xc, g[c * plane + o], o, P) *
+                         0.25f;
+        r = omega == 1.0f ? jv : v + omega * (jv - v);
+      }
+      xout[c * plane + o] = r;
+    }
+  }
+}
+
 int grid_blocks(long long n, int cap) {
   long long b = (n + kBlock - 1) / kBlock;
   if (b < 1) b = 1;
@@ -446,6 +560,41 @@ hipError_t launch_finalize(int order, const float* x, const float* g,
   else
     hipLaunchKernelGGL(finalize_kernel<ORDER_ULRD>, dim3(nblocks), dim3(kBlock),
                        0, st, x, g, m, plane, H, W, P, img, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_restrict(int order, const float* x, const float* g,
+                           const uint8_t* m, long long plane, int P, int f,
+                           int hc, int wc, float* out, hipStream_t st) {
+  const int blocks = (int)(((long long)hc * wc + kBlock - 1) / kBlock);
+  if (order == ORDER_UDLR)
+    hipLaunchKernelGGL(restrict_kernel<ORDER_UDLR>, dim3(blocks), dim3(kBlock),
+                       0, st, x, g, m, plane, P, f, hc, wc, out);
+  else
+    hipLaunchKernelGGL(restrict_kernel<ORDER_ULRD>, dim3(blocks), dim3(kBlock),
+                       0, st, x, g, m, plane, P, f, hc, wc, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_prolong(float* x, const uint8_t* m, long long plane, int H,
+                          int W, int P, int f, int hc, int wc,
+                          const float* corr, hipStream_t st) {
+  hipLaunchKernelGGL(prolong_kernel,
+                     dim3(grid_blocks((long long)H * W, 8192)), dim3(kBlock),
+                     0, st, x, m, plane, H, W, P, f, hc, wc, corr);
+  return hipGetLastError();
+}
+
+hipError_t launch_weighted(int order, const float* xin, float* xout,
+                           const float* g, const uint8_t* m, long long plane,
+                           int H, int W, int P, float omega, hipStream_t st) {
+  const dim3 grid(grid_blocks((long long)H * W, 8192));
+  if (order == ORDER_UDLR)
+    hipLaunchKernelGGL(weighted_kernel<ORDER_UDLR>, grid, dim3(kBlock), 0, st,
+                       xin, xout, g, m, plane, H, W, P, omega);
+  else
+    hipLaunchKernelGGL(weighted_kernel<ORDER_ULRD>, grid, dim3(kBlock), 0, st,
+                       xin, xout, g, m, plane, H, W, P, omega);
   return hipGetLastError();
 }
 

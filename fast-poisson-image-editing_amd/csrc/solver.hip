@@ -54,6 +54,7 @@ struct pie_solver {
   DevBuf<uint8_t> m, img;
   DevBuf<double> partial;
   DevBuf<float> err;
+  DevBuf<float> coarse;  // restriction / prolongation staging
 
   // EquSolver
   int N = 0;
@@ -95,6 +96,7 @@ void free_problem(pie_solver* s) {
   s->img.release();
   s->partial.release();
   s->err.release();
+  s->coarse.release();
   s->lin.release();
   s->A4.release();
   s->B4.release();
@@ -591,6 +593,72 @@ int pie_solve_to_residual(pie_solver* s, double tol_mean, int check_every,
   }
   if (sweeps) *sweeps = done;
   if (mean_residual) *mean_residual = mean;
+  return PIE_OK;
+}
+
+namespace {
+
+// Checks shared by the coarse-grid correction calls; *hc, *wc = the coarse
+// grid (crop size // factor).
+int coarse_grid(pie_solver* s, int factor, int* hc, int* wc) {
+  if (!s) return fail(PIE_EINVAL, "NULL solver");
+  if (!s->loaded) return fail(PIE_ESTATE, "coarse-grid call before reset");
+  if (s->kind != PIE_GRID)
+    return fail(PIE_EUNSUPPORTED, "coarse-grid calls are GridSolver calls");
+  if (factor < 1) return fail(PIE_EINVAL, "factor must be >= 1");
+  *hc = s->H / factor;
+  *wc = s->W / factor;
+  if (*hc < 1 || *wc < 1)
+    return fail(PIE_EINVAL, "factor exceeds the crop size");
+  const size_t n = (size_t)*hc * *wc * 3;
+  if (s->coarse.n < n) PIE_HIP(s->coarse.alloc(n));
+  return PIE_OK;
+}
+
+}  // namespace
+
+int pie_restrict(pie_solver* s, int factor, float* out) {
+  int hc = 0, wc = 0;
+  if (!out) return fail(PIE_EINVAL, "NULL argument");
+  PIE_CHECK(coarse_grid(s, factor, &hc, &wc));
+  PIE_HIP(pie::launch_restrict(s->eff_order(), s->state(), s->g0(), s->m0(),
+                               s->plane, s->P, factor, hc, wc, s->coarse.p,
+                               s->stream));
+  PIE_HIP(hipMemcpyAsync(out, s->coarse.p, (size_t)hc * wc * 3 * sizeof(float),
+                         hipMemcpyDeviceToHost, s->stream));
+  PIE_HIP(hipStreamSynchronize(s->stream));
+  return PIE_OK;
+}
+
+int pie_prolong(pie_solver* s, int factor, int hc, int wc, const float* corr) {
+  int ec = 0, ew = 0;
+  if (!corr) return fail(PIE_EINVAL, "NULL argument");
+  PIE_CHECK(coarse_grid(s, factor, &ec, &ew));
+  if (hc != ec || wc != ew)
+    return fail(PIE_EINVAL, "coarse grid must be (h // factor, w // factor)");
+  PIE_CHECK(upload(corr, (size_t)hc * wc * 3 * sizeof(float), s->coarse.p,
+                   s->stream));
+  PIE_HIP(pie::launch_prolong(s->state(), s->m0(), s->plane, s->H, s->W, s->P,
+                              factor, hc, wc, s->coarse.p, s->stream));
+  // the upload reads pageable host memory: finish before returning
+  PIE_HIP(hipStreamSynchronize(s->stream));
+  return PIE_OK;
+}
+
+int pie_relax(pie_solver* s, int sweeps, float omega) {
+  if (!s) return fail(PIE_EINVAL, "NULL solver");
+  if (!s->loaded) return fail(PIE_ESTATE, "relax before reset");
+  if (s->kind != PIE_GRID)
+    return fail(PIE_EUNSUPPORTED, "relax is a GridSolver call");
+  if (sweeps < 0) return fail(PIE_EINVAL, "negative sweep count");
+  if (!(omega > 0.0f && omega <= 1.0f))
+    return fail(PIE_EINVAL, "omega must be in (0, 1]");
+  for (int k = 0; k < sweeps; ++k) {
+    PIE_HIP(pie::launch_weighted(s->eff_order(), s->state(), s->other(),
+                                 s->g0(), s->m0(), s->plane, s->H, s->W, s->P,
+                                 omega, s->stream));
+    s->cur ^= 1;
+  }
   return PIE_OK;
 }
 

@@ -202,6 +202,36 @@ class GridSolver(_Core):
         ctypes.byref(n), ctypes.byref(r)))
     return n.value, r.value
 
+  # coarse-grid correction (extensions used by the learned warm start)
+
+  def coarse_shape(self, factor: int) -> tuple[int, int]:
+    if int(factor) < 1:
+      raise ValueError("factor must be >= 1")
+    h, w = self.hw
+    return h // factor, w // factor
+
+  def restrict(self, factor: int) -> np.ndarray:
+    """float32 [h // factor, w // factor, 3]: per factor x factor block, the
+    sum of the signed residual (g + neighbours - 4 x) of the current state
+    over the block's solved pixels."""
+    out = np.empty(self.coarse_shape(factor) + (3,), np.float32)
+    native.check(self._lib.pie_restrict(self._h, int(factor), _ptr(out)))
+    return out
+
+  def prolong(self, factor: int, corr: np.ndarray) -> None:
+    """state += bilinear upsampling of corr [h // factor, w // factor, 3] at
+    the solved pixels."""
+    corr = np.ascontiguousarray(corr, dtype=np.float32)
+    hc, wc = self.coarse_shape(factor)
+    if corr.shape != (hc, wc, 3):
+      raise ValueError(f"correction must be {(hc, wc, 3)}, not {corr.shape}")
+    native.check(self._lib.pie_prolong(self._h, int(factor), hc, wc,
+                                       _ptr(corr)))
+
+  def relax(self, sweeps: int = 1, omega: float = 0.5) -> None:
+    """`sweeps` weighted Jacobi sweeps x <- x + omega ((g + sum) / 4 - x)."""
+    native.check(self._lib.pie_relax(self._h, int(sweeps), float(omega)))
+
 
 class BatchSolver:
   """Batched GridSolver (extension): composites of uneven size and mask,

@@ -199,4 +199,25 @@ def batch(n: int = 256, seed: int = 0, side: tuple[int, int] = (128, 2048),
   return out
 
 
+def composite(size: tuple[int, int], seed: int, gradient: str | None = None):
+  """One GridSolver problem on a whole image (no crop): blob source and
+  target, a star-polygon mask inside a 1-px border, gradient mode drawn per
+  seed when not given.  Returns (mask int32 [h, w], target float32
+  [h, w, 3], divergence float32 [h, w, 3], gradient mode); the divergence is
+  the GridProcessor guidance (fpie/process.py:362-378)."""
+  from . import process
+  h, w = size
+  rng = np.random.default_rng([7, seed])
+  if gradient is None:
+    gradient = "src" if rng.random() < 0.5 else "max"
+  src = blobs(h, w, 10_000 + seed).astype(np.float32)
+  tgt = blobs(h, w, 20_000 + seed).astype(np.float32)
+  mask = (star_mask(h, w, rng) > 0).astype(np.int32)
+  mask[[0, -1], :] = 0
+  mask[:, [0, -1]] = 0
+  grad = process.divergence(src, tgt, mask,
+                            lambda a, b: process.mixgrad(a, b, gradient))
+  return mask, tgt, grad, gradient
+
+
 WORKLOADS = {"equ512": equ512, "grid4096": grid4096, "grid8192": grid8192}

@@ -1,130 +1,562 @@
-"""Train the warm-start U-Net and measure what it gains (GPU).
+"""Learned warm start for the GridSolver (no counterpart in the reference).
 
-    python -m fpie_amd.warmstart [--train N] [--epochs E] [--sizes 1024,2048]
+Why plain Jacobi is slow, and what the warm start removes.  A Jacobi sweep
+(fpie/core/openmp/grid.cc:75-103) multiplies an error mode of the masked
+5-point system by its eigenvalue mu; the modes with |mu| close to 1 are the
+smooth ones (mu near +1) and their odd-even (checkerboard) partners
+(mu near -1), both decaying in about L^2 sweeps on an L-pixel-wide mask.
+The mean absolute residual sees a checkerboard mode 8 / lambda_min times
+more strongly than the smooth mode of the same amplitude, so a stop rule on
+the residual waits for the checkerboard modes.  A start state that is only
+smooth-accurate (an upsampled prediction) therefore gains nothing; it has to
+be free of both.
 
-1. N seeded training composites (blob images, star-polygon masks) drawn at
-   1024 x 1024 and coarsened to 256 x 256 (the form in which large problems
-   are shown to the network), labelled by the batched GPU Jacobi solver
-   (20000 sweeps from the target).
-2. Adam training (lr 1e-3, batch 64, masked mean squared error) with the
-   hand-written forward/backward kernels.
-3. On composites with unseen seeds at each evaluation size: plain Jacobi from
-   the target state until the mean absolute residual is <= 1e-3 (cold), and
-   the same from the network's prediction (warm; network inference, the
-   residual comparison that decides the fallback and the state upload are
-   inside the timed region).  Problems larger than the training size are
-   predicted on a coarsened grid (fpie_amd.unet.warm_start).  Both final
-   states are compared with a reference: the cold run continued for as many
-   sweeps again.
+Algorithm (device work through the GridSolver calls restrict / prolong /
+relax / iterate; the coarse problem on the host with the network on the
+GPU):
 
-Prints one JSON line per evaluation and a summary line.
+  smooth: 6 f^2 plain Jacobi sweeps, then one weighted sweep with omega 1/2
+          (the mean of two consecutive Jacobi iterates), which removes the
+          fine error the coarse grid cannot carry and the odd-even modes
+  smooth once (the first coarse residual is then free of pixel noise)
+  for each cycle:
+    r_c = block sums (factor f) of the residual b - A x of the state
+    solve A_c c = r_c (zero on the coarse boundary) approximately by
+      `coarse_iters` steps of flexible conjugate gradients with two search
+      directions per step:
+        - the U-Net's answer to the current coarse residual (per-channel
+          rms-normalised), low-passed by 4 damped smoothing sweeps of the
+          coarse Laplacian (bf16 noise in the output would otherwise
+          dominate <p, A_c p> and shrink every step);
+        - the exact solution of the residual's restriction to a grid 4x
+          coarser again (sparse LU, factorised once per problem; at most
+          64 x 64 cells), upsampled: the few smoothest modes, which carry
+          the largest error and the smallest residual;
+      each direction is A_c-orthogonalised against the cycle's earlier ones
+      and taken with the energy-optimal step, so the coarse error's energy
+      norm never grows whatever the network returns
+    x += bilinear upsampling of c (solved pixels); smooth
+    stop once max |c| < `stop` pixel units, or when the mean residual of the
+    state did not fall over the cycle
+  fallback: if the result's mean residual is not below the target state's
+  (the cold start's), the state is reset to the target.
+
+Plain Jacobi then finishes to the residual tolerance.  f is the power of two
+that brings the larger side to about `coarse` (256) pixels.
+
+Training data (correction form; labels from the GPU Jacobi solver itself):
+composites drawn at `render` x `render` (blob source and target, star mask,
+fpie_amd.synth.composite) are smoothed from the target state as above and
+their residual restricted to `size` x `size` -- the coarse problem of the
+first cycle -- and labelled with `label_sweeps` batched Jacobi sweeps of
+that coarse correction problem; a share of smoothed random residuals (half
+of them confined to a band along the mask edge, where later cycles leave
+their residual) widens the input distribution.  Inputs and labels are
+normalised per sample and channel by their rms.
+
+  python -m fpie_amd.warmstart [--train N] [--epochs E] [--sizes 1024,2048]
+
+trains, evaluates on unseen seeds and prints one JSON line per evaluation
+plus a summary.
 """
 
 from __future__ import annotations
 
 import argparse
 import json
+import os
 import sys
 import time
 
 import numpy as np
 
-from . import unet
-from .core_hip import GridSolver
+from . import synth, unet
+from .core_hip import BatchSolver, GridSolver
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                       "warmstart_unet.npz")
+
+# ------------------------------------------------------- coarse algebra ---
 
 
-def evaluate(params: np.ndarray, size: int, seed: int, tol: float = 1e-3,
-             check_every: int = 512, max_sweeps: int = 4_000_000,
-             log=print) -> dict:
-  mask, tgt, grad, mode = unet.composite((size, size), seed)
-  solved = int(np.count_nonzero(mask))
-  unet.warm_start(params, mask, tgt, grad)  # kernel loading, not timed
+def coarse_mask(mask: np.ndarray, f: int) -> np.ndarray:
+  """bool [h // f, w // f]: blocks whose majority is solved, outer ring
+  cleared (the coarse system's Dirichlet boundary)."""
+  h, w = (mask.shape[0] // f) * f, (mask.shape[1] // f) * f
+  m = (np.asarray(mask)[:h, :w] != 0).reshape(h // f, f, w // f, f)
+  mc = m.mean((1, 3)) > 0.5
+  mc[[0, -1], :] = False
+  mc[:, [0, -1]] = False
+  return mc
 
-  def solver():
-    s = GridSolver(schedule="jacobi")
-    s.reset(0, mask, tgt, grad)
-    s.set_tile(8, 1, 128 if size >= 2048 else 64)
-    s.iterate(0)
-    s.sync()
-    return s
 
-  def solve(s, tol_):
-    # in chunks, with a progress line per chunk
-    n, r, chunk = 0, float("inf"), 200_000
-    while n < max_sweeps:
-      k, r = s.solve_to_residual(tol_, check_every, min(chunk, max_sweeps - n))
-      n += k
-      log(f"#   {size}^2 seed {seed}: {n} sweeps, mean residual {r:.3g}")
-      if r <= tol_ or k == 0:
+def signed_residual(mask, x, grad) -> np.ndarray:
+  """float64 [h, w, 3]: g + four neighbours - 4 x at solved pixels."""
+  t = np.asarray(x, np.float64)
+  r = np.zeros_like(t)
+  r[1:-1, 1:-1] = (np.asarray(grad, np.float64)[1:-1, 1:-1] + t[:-2, 1:-1] +
+                   t[2:, 1:-1] + t[1:-1, :-2] + t[1:-1, 2:] -
+                   4.0 * t[1:-1, 1:-1])
+  return r * (np.asarray(mask)[..., None] != 0)
+
+
+def block_sum(a: np.ndarray, f: int) -> np.ndarray:
+  h, w = (a.shape[0] // f) * f, (a.shape[1] // f) * f
+  return a[:h, :w].reshape(h // f, f, w // f, f, -1).sum((1, 3))
+
+
+def coarse_apply(mc: np.ndarray, p: np.ndarray) -> np.ndarray:
+  """A_c p: 4 p - sum of the solved neighbours' p, on the solved cells."""
+  q = np.where(mc[..., None], p, 0.0).astype(np.float64)
+  z = np.pad(q, ((1, 1), (1, 1), (0, 0)))
+  out = 4.0 * q - (z[:-2, 1:-1] + z[2:, 1:-1] + z[1:-1, :-2] + z[1:-1, 2:])
+  return out * mc[..., None]
+
+
+def lowpass(mc: np.ndarray, p: np.ndarray, sweeps: int = 4) -> np.ndarray:
+  """`sweeps` damped Jacobi sweeps (weight 1/2) of the homogeneous coarse
+  problem (p = 0 off the solved cells): keeps the smooth part of p and
+  removes its cell-scale oscillation."""
+  m = mc[..., None]
+  q = np.where(m, p, 0.0).astype(np.float64)
+  for _ in range(sweeps):
+    z = np.pad(q, ((1, 1), (1, 1), (0, 0)))
+    avg = 0.25 * (z[:-2, 1:-1] + z[2:, 1:-1] + z[1:-1, :-2] + z[1:-1, 2:])
+    q = np.where(m, 0.5 * (q + avg), 0.0)
+  return q
+
+
+def upsample(a: np.ndarray, f: int, h: int, w: int) -> np.ndarray:
+  """Cell-centred bilinear upsampling of a [hc, wc, 3] by f to [h, w, 3]
+  (the pie_prolong geometry, in float64)."""
+  hc, wc = a.shape[:2]
+  ys = (np.arange(h) + 0.5) / f - 0.5
+  xs = (np.arange(w) + 0.5) / f - 0.5
+  y0 = np.clip(np.floor(ys).astype(np.int64), 0, hc - 1)
+  x0 = np.clip(np.floor(xs).astype(np.int64), 0, wc - 1)
+  y1, x1 = np.minimum(y0 + 1, hc - 1), np.minimum(x0 + 1, wc - 1)
+  fy = np.clip(ys - y0, 0.0, 1.0)[:, None, None]
+  fx = np.clip(xs - x0, 0.0, 1.0)[None, :, None]
+  top = a[y0][:, x0] * (1 - fx) + a[y0][:, x1] * fx
+  bot = a[y1][:, x0] * (1 - fx) + a[y1][:, x1] * fx
+  return top * (1 - fy) + bot * fy
+
+
+class CoarsestSolve:
+  """Exact solve of the coarse problem restricted by `g` (block sums,
+  majority mask): sparse LU of the 5-point system, factorised once."""
+
+  def __init__(self, mc: np.ndarray, g: int):
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    self.g, self.shape = g, mc.shape
+    self.mcc = coarse_mask(mc, g)
+    ii, jj = np.nonzero(self.mcc)
+    self.ii, self.jj = ii, jj
+    n = ii.size
+    self.lu = None
+    if n == 0:
+      return
+    ids = -np.ones(self.mcc.shape, np.int64)
+    ids[ii, jj] = np.arange(n)
+    rows, cols = [np.arange(n)], [np.arange(n)]
+    vals = [np.full(n, 4.0)]
+    for di, dj in ((-1, 0), (1, 0), (0, -1), (0, 1)):
+      nid = ids[ii + di, jj + dj]
+      ok = nid >= 0
+      rows.append(np.arange(n)[ok])
+      cols.append(nid[ok])
+      vals.append(-np.ones(int(ok.sum())))
+    A = sp.csc_matrix((np.concatenate(vals),
+                       (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(n, n))
+    self.lu = spl.splu(A)
+
+  def direction(self, r: np.ndarray, mc: np.ndarray) -> np.ndarray:
+    if self.lu is None:
+      return np.zeros(r.shape)
+    rcc = block_sum(r, self.g)[self.ii, self.jj]
+    e = np.zeros(self.mcc.shape + (3,))
+    e[self.ii, self.jj] = self.lu.solve(rcc)
+    return upsample(e, self.g, *self.shape) * mc[..., None]
+
+
+def _dot(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+  """Per-channel inner products of [h, w, 3] fields (float64)."""
+  return np.einsum("ijc,ijc->c", a, b, dtype=np.float64)
+
+
+def mean_residual(solver: GridSolver) -> float:
+  """Mean absolute residual err / (3 * solved pixels) of the state."""
+  return solver.solve_to_residual(float("inf"), 1, 0)[1]
+
+
+# ------------------------------------------------------------ warm start ---
+
+
+def load_params(path: str | None = None) -> np.ndarray:
+  with np.load(path or WEIGHTS, allow_pickle=False) as f:
+    return f["params"].astype(np.float32)
+
+
+class WarmStart:
+  """The learned warm start of one trained network (see module docstring).
+
+  `sweeps` plain Jacobi sweeps per smoothing (default 6 f^2); networks are
+  kept per coarse shape, so repeated problems of one size reuse them.
+  precond "net" uses the network, "none" plain conjugate gradients (the
+  same method without the network, for comparison)."""
+
+  def __init__(self, params: np.ndarray, coarse: int = 256, cycles: int = 16,
+               sweeps: int | None = None, omega: float = 0.5,
+               stop: float = 0.005, coarse_tol: float = 1e-3,
+               coarse_iters: int = 2, precond: str = "net",
+               lowpass_sweeps: int = 4, coarsest: int = 64):
+    if precond not in ("net", "none"):
+      raise ValueError("precond must be 'net' or 'none'")
+    self.params = np.asarray(params, np.float32)
+    self.coarse, self.cycles, self.sweeps = coarse, cycles, sweeps
+    self.omega, self.stop = omega, stop
+    self.coarse_tol, self.coarse_iters = coarse_tol, coarse_iters
+    self.precond, self.lowpass_sweeps = precond, lowpass_sweeps
+    self.coarsest = coarsest  # side of the exact-solve grid (0: none)
+    self._nets: dict = {}
+
+  def factor(self, h: int, w: int) -> int:
+    f = 1
+    while max(h, w) // (2 * f) >= self.coarse:
+      f *= 2
+    return f
+
+  def _net(self, hc: int, wc: int) -> unet.UNet:
+    key = (hc + (-hc) % 8, wc + (-wc) % 8)
+    if key not in self._nets:
+      self._nets[key] = unet.UNet(1, key[0], key[1], params=self.params)
+    return self._nets[key]
+
+  def predict(self, rc: np.ndarray, mc: np.ndarray) -> np.ndarray:
+    """Network output [hc, wc, 3] for the coarse residual rc (each channel
+    scaled to unit rms over the solved cells; output scale arbitrary)."""
+    hc, wc = mc.shape
+    sig = np.sqrt(np.mean(rc[mc] ** 2, axis=0)) if mc.any() else np.zeros(3)
+    if not (sig > 0).any():
+      return np.zeros((hc, wc, 3), np.float32)
+    net = self._net(hc, wc)
+    x = np.zeros(net.shape[1:] + (8,), np.float32)
+    x[:hc, :wc, 0:3] = rc / np.where(sig > 0, sig, 1.0)
+    x[:hc, :wc, 6] = mc
+    return net.forward(x[None])[0, :hc, :wc] * mc[..., None]
+
+  def coarse_solve(self, rc: np.ndarray, mc: np.ndarray,
+                   coarsest: CoarsestSolve | None = None):
+    """Flexible conjugate gradients on A_c c = rc with the network (or, for
+    precond "none", the residual itself) and the coarsest exact solve as
+    search directions; returns (c float32, steps, final relative residual
+    per channel)."""
+    m = mc[..., None]
+    r = np.asarray(rc, np.float64) * m
+    c = np.zeros_like(r)
+    bn = np.sqrt(_dot(r, r))
+    rel = np.ones(3)
+    if not (bn > 0).any():
+      return c.astype(np.float32), 0, np.zeros(3)
+    dirs = []  # (p, A p, <p, A p>)
+    k = 0
+    for k in range(1, self.coarse_iters + 1):
+      cands = [lowpass(mc, self.predict(r, mc), self.lowpass_sweeps)
+               if self.precond == "net" else r.copy()]
+      if coarsest is not None:
+        cands.append(coarsest.direction(r, mc))
+      for z in cands:
+        z = z * m
+        for p, ap, pap in dirs:
+          z -= (_dot(z, ap) / pap) * p
+        az = coarse_apply(mc, z)
+        pap = _dot(z, az)
+        ok = pap > 1e-30 * np.maximum(_dot(z, z), 1e-300)
+        a = np.where(ok, _dot(r, z) / np.where(ok, pap, 1.0), 0.0)
+        c += a * z
+        r -= a * az
+        dirs.append((z, az, np.where(ok, pap, 1.0)))
+      rel = np.sqrt(_dot(r, r)) / np.where(bn > 0, bn, 1.0)
+      if (rel < self.coarse_tol).all():
         break
-    return n, r
+    return c.astype(np.float32), k, rel
 
-  # cold start
-  s = solver()
-  _, e0 = s.step(0)
+  def run(self, solver: GridSolver, mask: np.ndarray, tgt: np.ndarray,
+          log=None) -> dict:
+    """Warm-start the loaded problem (state = target) in place; returns
+    what was done.  Fallback to the target state as described above."""
+    t0 = time.perf_counter()
+    h, w = mask.shape
+    f = self.factor(h, w)
+    mc = coarse_mask(mask, f)
+    nu = self.sweeps if self.sweeps is not None else 6 * f * f
+    g = 1
+    while self.coarsest and max(mc.shape) // (2 * g) >= self.coarsest:
+      g *= 2
+    low = CoarsestSolve(mc, g) if self.coarsest and g > 1 else None
+    r_cold = mean_residual(solver)
+
+    def smooth():
+      solver.iterate(nu)
+      solver.relax(1, self.omega)
+
+    smooth()
+    steps = []
+    r_prev = mean_residual(solver)
+    for k in range(self.cycles):
+      rc = solver.restrict(f)
+      rc[~mc] = 0.0
+      cc, calls, rel = self.coarse_solve(rc, mc, low)
+      solver.prolong(f, cc)
+      smooth()
+      r_now = mean_residual(solver)
+      size = float(np.abs(cc).max()) if cc.size else 0.0
+      steps.append({"max_corr": size, "coarse_iters": calls,
+                    "coarse_rel_residual": float(rel.max()),
+                    "mean_residual": r_now})
+      if log:
+        log(f"#   cycle {k + 1}: max correction {size:.4g}, {calls} coarse "
+            f"iterations (relative residual {rel.max():.2g}), mean "
+            f"residual {r_now:.4g}")
+      if size < self.stop or not r_now < r_prev:
+        break
+      r_prev = r_now
+    r_warm = mean_residual(solver)
+    used = bool(r_warm < r_cold)
+    if not used:
+      solver.set_state(tgt)
+    return {"factor": f, "coarse": list(mc.shape), "sweeps_per_smooth": nu,
+            "coarsest": list(low.mcc.shape) if low is not None else None,
+            "precond": self.precond, "cycles": len(steps), "steps": steps,
+            "used": used, "start_residual_cold": r_cold,
+            "start_residual_warm": r_warm,
+            "seconds": time.perf_counter() - t0,
+            "sweeps": (len(steps) + 1) * (nu + 1)}
+
+
+# --------------------------------------------------------- training data ---
+
+
+def _composite_job(args):
+  render, seed = args
+  mask, tgt, grad, _ = synth.composite((render, render), seed)
+  return mask.astype(np.uint8), tgt, grad
+
+
+def _random_job(args):
+  size, seed = args
+  from scipy import ndimage
+  rng = np.random.default_rng([13, seed])
+  mc = synth.star_mask(size, size, rng) > 0
+  mc[[0, -1], :] = False
+  mc[:, [0, -1]] = False
+  rc = rng.standard_normal((size, size, 3))
+  sigma = rng.uniform(0.0, 8.0)
+  if sigma > 0.5:
+    rc = ndimage.gaussian_filter(rc, (sigma, sigma, 0))
+  if rng.random() < 0.5:  # a band along the mask edge
+    d = ndimage.distance_transform_edt(mc)
+    rc = rc * (d <= rng.uniform(1.5, 10.0))[..., None]
+  return mc.astype(np.uint8), (rc * mc[..., None]).astype(np.float32)
+
+
+def _normalise(a: np.ndarray, m: np.ndarray) -> np.ndarray:
+  sig = np.sqrt(np.mean(a[m] ** 2, axis=0)) if m.any() else np.zeros(3)
+  return (a / np.where(sig > 0, sig, 1.0) * m[..., None]).astype(np.float16)
+
+
+def training_set(n: int, size: int = 256, render: int = 1024,
+                 sweeps: int = 20000, random_frac: float = 0.25,
+                 seed0: int = 0, workers: int = 16, chunk: int = 512,
+                 log=print):
+  """n samples (M uint8 masks, X float16 normalised coarse residuals,
+  Y float16 normalised coarse corrections), each [n, size, size(, 3)]."""
+  from collections import deque
+  from concurrent.futures import ProcessPoolExecutor
+  t0 = time.perf_counter()
+  f = render // size
+  nu = 4 * f * f
+  n_rand = int(round(n * random_frac))
+  M = np.zeros((n, size, size), np.uint8)
+  R = np.zeros((n, size, size, 3), np.float32)
+  with ProcessPoolExecutor(max_workers=workers) as ex:
+    for i, (mc, rc) in enumerate(ex.map(
+        _random_job, [(size, seed0 + i) for i in range(n_rand)],
+        chunksize=16)):
+      M[i], R[i] = mc, rc
+    # composites: drawn by the workers (a bounded window ahead), smoothed
+    # and restricted on the GPU
+    seeds = iter(range(seed0 + n_rand, seed0 + n))
+    window = deque()
+    for sd in seeds:
+      window.append(ex.submit(_composite_job, (render, sd)))
+      if len(window) >= 2 * workers:
+        break
+    i = n_rand
+    while window:
+      mask, tgt, grad = window.popleft().result()
+      sd = next(seeds, None)
+      if sd is not None:
+        window.append(ex.submit(_composite_job, (render, sd)))
+      s = GridSolver(schedule="jacobi")
+      s.reset(0, mask, tgt, grad)
+      s.iterate(nu)
+      s.relax(1, 0.5)
+      mc = coarse_mask(mask, f)
+      rc = s.restrict(f)
+      rc[~mc] = 0.0
+      del s
+      M[i], R[i] = mc, rc
+      i += 1
+  log(f"inputs: {n} x {size}^2 ({n - n_rand} smoothed composites drawn at "
+      f"{render}^2, {n_rand} random) in {time.perf_counter() - t0:.1f} s")
+  X = np.zeros(R.shape, np.float16)
+  Y = np.zeros(R.shape, np.float16)
+  zeros = np.zeros((size, size, 3), np.float32)
+  t1 = time.perf_counter()
+  for c0 in range(0, n, chunk):
+    b = BatchSolver(sweeps_per_launch=8, rows_per_task=64)
+    idx = range(c0, min(n, c0 + chunk))
+    for i in idx:
+      b.add(M[i], zeros, R[i], max_sweeps=sweeps, tol=0.0)
+    b.solve(32)
+    for k, i in enumerate(idx):
+      _, _, c = b.read(k, state=True)
+      m = M[i] != 0
+      X[i] = _normalise(R[i], m)
+      Y[i] = _normalise(c, m)
+    del b
+    log(f"labels: {min(n, c0 + chunk)}/{n} ({sweeps} Jacobi sweeps each, "
+        f"{time.perf_counter() - t1:.1f} s)")
+  return M, X, Y
+
+
+# ------------------------------------------------------------ evaluation ---
+
+
+def _solver(mask, tgt, grad):
+  s = GridSolver(schedule="jacobi")
+  s.reset(0, mask, tgt, grad)
+  s.set_tile(8, 1, 128 if max(mask.shape) >= 2048 else 64)
+  return s
+
+
+def _solve(s, tol, check_every, max_sweeps, log, tag, chunk=200_000):
+  n, r = 0, mean_residual(s)
+  while r > tol and n < max_sweeps:
+    k, r = s.solve_to_residual(tol, check_every, min(chunk, max_sweeps - n))
+    n += k
+    log(f"#   {tag}: {n} sweeps, mean residual {r:.4g}")
+    if k == 0:
+      break
+  return n, r
+
+
+def converged(mask, tgt, grad, log, chunk=50_000, max_sweeps=8_000_000):
+  """Plain Jacobi from the target until the mean residual stops falling
+  (less than 1 % over `chunk` sweeps: the float32 floor) or is < 1e-7.
+  Returns (state, final mean residual, sweeps)."""
+  s = _solver(mask, tgt, grad)
+  n, prev = 0, mean_residual(s)
+  r = prev
+  while n < max_sweeps:
+    k, r = s.solve_to_residual(0.0, chunk, chunk)
+    n += k
+    log(f"#   converged plain Jacobi: {n} sweeps, mean residual {r:.4g}")
+    if r < 1e-7 or r > 0.99 * prev:
+      break
+    prev = r
+  return s.state(), r, n
+
+
+def evaluate(ws: WarmStart, size: int, seed: int, tol: float = 1e-3,
+             check_every: int = 512, max_sweeps: int = 8_000_000,
+             log=print) -> dict:
+  """Time to the plain method's final residual, cold vs warm, on one unseen
+  composite; accuracy against converged plain Jacobi."""
+  mask, tgt, grad, mode = synth.composite((size, size), seed)
+  m = mask != 0
+  x_ref, r_floor, n_ref = converged(mask, tgt, grad, log)
+  tol_eff = max(tol, 1.05 * r_floor)
+  # untimed: kernel loading and network creation for this coarse shape
+  s = _solver(mask, tgt, grad)
+  ws.run(s, mask, tgt)
+  del s
+
+  s = _solver(mask, tgt, grad)
+  s.sync()
   t = time.perf_counter()
-  n_cold, r_cold = solve(s, tol)
+  n_cold, r_cold = _solve(s, tol_eff, check_every, max_sweeps, log, "cold")
   t_cold = time.perf_counter() - t
   x_cold = s.state()
-  # reference: the same run continued for as many sweeps again
-  s.iterate(max(n_cold, 1))
-  _, e_ref = s.step(0)
-  x_ref = s.state()
-  r_ref = float(e_ref.sum()) / (3 * solved)
   del s
-  # warm start: prediction, fallback test and upload inside the timing
-  s = solver()
-  t = time.perf_counter()
-  x_pred = unet.warm_start(params, mask, tgt, grad)
-  s.set_state(x_pred)
-  _, e_pred = s.step(0)
-  used = bool(e_pred.sum() < e0.sum())
-  if not used:  # fallback: the prediction is worse than the target state
-    s.set_state(tgt)
-  t_pred = time.perf_counter() - t
-  n_warm, r_warm = solve(s, tol)
-  t_warm = time.perf_counter() - t
-  x_warm = s.state()
-  m = mask != 0
+
+  def warm(w):
+    s = _solver(mask, tgt, grad)
+    s.sync()
+    t = time.perf_counter()
+    info = w.run(s, mask, tgt, log=log)
+    n, r = _solve(s, tol_eff, check_every, max_sweeps, log, "warm")
+    return time.perf_counter() - t, n, r, info, s.state()
+
+  t_warm, n_warm, r_warm, info, x_warm = warm(ws)
+  # the same method without the network (plain conjugate gradients on the
+  # coarse grid, same iteration budget)
+  cg = WarmStart(ws.params, ws.coarse, ws.cycles, ws.sweeps, ws.omega,
+                 ws.stop, ws.coarse_tol, ws.coarse_iters, "none",
+                 ws.lowpass_sweeps, ws.coarsest)
+  t_cg, n_cg, r_cg, info_cg, x_cg = warm(cg)
+
+  def err(x):
+    d = np.abs(x.astype(np.float64) - x_ref)[m]
+    return {"max_abs_diff_vs_converged": float(d.max()),
+            "mean_abs_diff_vs_converged": float(d.mean())}
+
   rec = {
-      "size": size, "seed": seed, "gradient": mode, "solved": solved,
-      "tol_mean_residual": tol,
+      "size": size, "seed": seed, "gradient": mode, "solved": int(m.sum()),
+      "tol_mean_residual": tol_eff,
+      "converged": {"sweeps": n_ref, "mean_residual": r_floor},
       "cold": {"seconds": t_cold, "sweeps": n_cold, "residual": r_cold,
-               "start_residual": float(e0.sum()) / (3 * solved),
-               "start_max_abs_diff_vs_reference": float(np.abs(
-                   tgt[m] - x_ref[m]).max()),
-               "max_abs_diff_vs_reference": float(np.abs(
-                   x_cold[m] - x_ref[m]).max())},
-      "warm": {"seconds": t_warm, "prediction_seconds": t_pred,
-               "sweeps": n_warm, "residual": r_warm,
-               "start_residual": float(e_pred.sum()) / (3 * solved),
-               "start_max_abs_diff_vs_reference": float(np.abs(
-                   x_pred[m] - x_ref[m]).max()),
-               "start_mean_abs_diff_vs_reference": float(np.abs(
-                   x_pred[m] - x_ref[m]).mean()),
-               "prediction_used": used,
-               "max_abs_diff_vs_reference": float(np.abs(
-                   x_warm[m] - x_ref[m]).max())},
-      "reference": {"sweeps": 2 * n_cold, "residual": r_ref},
+               **err(x_cold), "start": err(tgt)},
+      "warm": {"seconds": t_warm, "warm_start_seconds": info["seconds"],
+               "jacobi_sweeps": n_warm, "residual": r_warm, **err(x_warm),
+               "warm_start": info},
+      "without_network": {"seconds": t_cg,
+                          "warm_start_seconds": info_cg["seconds"],
+                          "jacobi_sweeps": n_cg, "residual": r_cg,
+                          **err(x_cg), "warm_start": info_cg},
       "speedup": t_cold / max(t_warm, 1e-9),
+      "speedup_without_network": t_cold / max(t_cg, 1e-9),
   }
+  rec["same_accuracy"] = bool(
+      rec["warm"]["max_abs_diff_vs_converged"] <=
+      rec["cold"]["max_abs_diff_vs_converged"])
   log(json.dumps(rec))
   return rec
 
 
 def main(argv=None) -> None:
   ap = argparse.ArgumentParser()
-  ap.add_argument("--train", type=int, default=4096)
+  ap.add_argument("--train", type=int, default=20000)
   ap.add_argument("--size", type=int, default=256)
   ap.add_argument("--render", type=int, default=1024,
-                  help="draw training composites at this size, then coarsen")
+                  help="draw training composites at this size")
+  ap.add_argument("--random-frac", type=float, default=0.25)
   ap.add_argument("--label-sweeps", type=int, default=20000)
   ap.add_argument("--epochs", type=int, default=8)
   ap.add_argument("--batch", type=int, default=64)
   ap.add_argument("--lr", type=float, default=1e-3)
   ap.add_argument("--sizes", default="1024,2048")
-  ap.add_argument("--seeds", default="900001,900002")
+  ap.add_argument("--seeds", default="900001")
   ap.add_argument("--tol", type=float, default=1e-3)
+  ap.add_argument("--coarse", type=int, default=256)
+  ap.add_argument("--cycles", type=int, default=16)
+  ap.add_argument("--stop", type=float, default=0.005)
+  ap.add_argument("--coarsest", type=int, default=64)
+  ap.add_argument("--coarse-tol", type=float, default=1e-3)
+  ap.add_argument("--coarse-iters", type=int, default=2)
   ap.add_argument("--save", default="")
   ap.add_argument("--load", default="")
   a = ap.parse_args(argv)
@@ -132,20 +564,22 @@ def main(argv=None) -> None:
   def log(msg):
     print(msg, flush=True)
 
+  tr = {}
   if a.load:
-    with np.load(a.load) as f:
-      params = f["params"].astype(np.float32)
-    tr = {}
+    params = load_params(a.load)
   else:
-    M, T, G2, Y = unet.training_set(a.train, a.size, sweeps=a.label_sweeps,
-                                    render=a.render, log=log)
-    net, tr = unet.train(M, T, G2, Y, epochs=a.epochs, batch=a.batch,
-                         lr=a.lr, log=log)
+    M, X, Y = training_set(a.train, a.size, a.render, a.label_sweeps,
+                           a.random_frac, log=log)
+    net, tr = unet.train(M, X, Y, epochs=a.epochs, batch=a.batch, lr=a.lr,
+                         log=log)
     params = net.params()
-    del net, M, T, G2, Y
+    del net, M, X, Y
     if a.save:
       np.savez_compressed(a.save, params=params.astype(np.float16))
-  evals = [evaluate(params, int(sz), int(sd), a.tol, log=log)
+  ws = WarmStart(params, coarse=a.coarse, cycles=a.cycles, stop=a.stop,
+                 coarse_tol=a.coarse_tol, coarse_iters=a.coarse_iters,
+                 coarsest=a.coarsest)
+  evals = [evaluate(ws, int(sz), int(sd), a.tol, log=log)
            for sz in a.sizes.split(",") for sd in a.seeds.split(",")]
   summary = {
       "train_samples": a.train, "train_size": a.size,
@@ -158,6 +592,11 @@ def main(argv=None) -> None:
           sz: float(np.mean([e["speedup"] for e in evals
                              if e["size"] == int(sz)]))
           for sz in a.sizes.split(",")},
+      "speedup_without_network_by_size": {
+          sz: float(np.mean([e["speedup_without_network"] for e in evals
+                             if e["size"] == int(sz)]))
+          for sz in a.sizes.split(",")},
+      "same_accuracy": all(e["same_accuracy"] for e in evals),
   }
   log(json.dumps({"summary": summary}))
 

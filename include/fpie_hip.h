@@ -143,6 +143,20 @@ int pie_solve_to_residual(pie_solver* s, double tol_mean, int check_every,
                           long long max_sweeps, long long* sweeps,
                           double* mean_residual);
 
+/* ---- coarse-grid correction (GridSolver warm start): no counterpart in
+ * the reference.  The coarse grid of `factor` is (h // factor) x
+ * (w // factor); block (I, J) covers rows I*factor.., cols J*factor.. */
+/* out float[hc][wc][3]: per block, the sum over its solved pixels of the
+ * signed residual (g + four neighbours, float adds in the solver's order)
+ * - 4 x of the current state, accumulated in double. */
+int pie_restrict(pie_solver* s, int factor, float* out);
+/* state += cell-centred bilinear upsampling of corr float[hc][wc][3] at the
+ * solved pixels; hc, wc must be the coarse grid of `factor`. */
+int pie_prolong(pie_solver* s, int factor, int hc, int wc, const float* corr);
+/* `sweeps` weighted Jacobi sweeps x <- x + omega * ((g + sum) / 4 - x),
+ * 0 < omega <= 1 (omega 1/2: the mean of two consecutive Jacobi iterates). */
+int pie_relax(pie_solver* s, int sweeps, float omega);
+
 /* ---- batched GridSolver: no counterpart in the reference ---------------
  * Many composites of uneven size and mask, each with its own stop rule,
  * solved together by one persistent Jacobi sweep kernel over a shared work

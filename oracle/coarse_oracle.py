@@ -1,0 +1,119 @@
+"""CPU oracle for the coarse-grid correction of the learned warm start.
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this module, and only as the
+checker.  The product package never imports it.
+
+The warm start has no counterpart in the reference (fpie 0.3.2), whose
+GridSolver only iterates (fpie/core/gcc/grid.cc:21-80,
+fpie/core/openmp/grid.cc:29-110).  What the device calls pie_restrict,
+pie_prolong and pie_relax compute is restated here with numpy in the same
+float32 operation order, plus an exact sparse solve (scipy, float64) of the
+GridSolver system that the tests use as the converged solution:
+
+  4 x_p - sum_{q in N(p), q solved} x_q = g_p + sum_{q in N(p), q unsolved} t_q
+
+for every solved pixel p (the fixed point of the update in
+fpie/core/gcc/grid.cc:21-36).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+_OFFS = {"ulrd": ((-1, 0), (0, -1), (0, 1), (1, 0)),
+         "udlr": ((-1, 0), (1, 0), (0, -1), (0, 1))}
+
+
+def neighbour_sum(x: np.ndarray, grad: np.ndarray, order: str = "ulrd"):
+  """float32 g + four neighbours, added left to right in the solver's order,
+  at every interior pixel ([h, w, 3]; the outermost ring is left 0)."""
+  x = np.asarray(x, np.float32)
+  s = np.zeros_like(x)
+  h, w = x.shape[:2]
+  acc = np.asarray(grad, np.float32)[1:-1, 1:-1].copy()
+  for di, dj in _OFFS[order]:
+    acc = acc + x[1 + di:h - 1 + di, 1 + dj:w - 1 + dj]
+  s[1:-1, 1:-1] = acc
+  return s
+
+
+def restrict(mask, x, grad, f: int, order: str = "ulrd") -> np.ndarray:
+  """Block sums over f x f blocks of the signed residual s - 4x at solved
+  pixels; s is the float32 neighbour sum, the difference and the sums are
+  float64.  Returns float32 [h // f, w // f, 3]."""
+  m = np.asarray(mask) != 0
+  x = np.asarray(x, np.float32)
+  s = neighbour_sum(x, grad, order)
+  r = (s.astype(np.float64) - x.astype(np.float64) * 4.0) * m[..., None]
+  h, w = (m.shape[0] // f) * f, (m.shape[1] // f) * f
+  blocks = r[:h, :w].reshape(h // f, f, w // f, f, 3)
+  return blocks.sum((1, 3)).astype(np.float32)
+
+
+def prolong(mask, x, corr, f: int) -> np.ndarray:
+  """x + cell-centred bilinear upsampling of corr at solved pixels, in the
+  kernel's float32 operation order."""
+  m = np.asarray(mask) != 0
+  x = np.asarray(x, np.float32)
+  corr = np.asarray(corr, np.float32)
+  hc, wc = corr.shape[:2]
+  h, w = m.shape
+  f32 = np.float32
+  inv = f32(1.0) / f32(f)
+  ys = (np.arange(h, dtype=f32) + f32(0.5)) * inv - f32(0.5)
+  xs = (np.arange(w, dtype=f32) + f32(0.5)) * inv - f32(0.5)
+  y0 = np.clip(np.floor(ys).astype(np.int64), 0, hc - 1)
+  x0 = np.clip(np.floor(xs).astype(np.int64), 0, wc - 1)
+  y1, x1 = np.minimum(y0 + 1, hc - 1), np.minimum(x0 + 1, wc - 1)
+  fy = np.clip(ys - y0.astype(f32), f32(0), f32(1))[:, None, None]
+  fx = np.clip(xs - x0.astype(f32), f32(0), f32(1))[None, :, None]
+  one = f32(1.0)
+  top = corr[y0][:, x0] * (one - fx) + corr[y0][:, x1] * fx
+  bot = corr[y1][:, x0] * (one - fx) + corr[y1][:, x1] * fx
+  v = top * (one - fy) + bot * fy
+  return np.where(m[..., None], x + v, x).astype(np.float32)
+
+
+def relax(mask, x, grad, sweeps: int, omega: float,
+          order: str = "ulrd") -> np.ndarray:
+  """Weighted Jacobi in the kernel's float32 order: j = s * 0.25;
+  x + omega * (j - x) (exactly j when omega is 1)."""
+  m = (np.asarray(mask) != 0)[..., None]
+  x = np.asarray(x, np.float32).copy()
+  om = np.float32(omega)
+  for _ in range(sweeps):
+    j = neighbour_sum(x, grad, order) * np.float32(0.25)
+    new = j if omega == 1.0 else x + om * (j - x)
+    x = np.where(m, new, x).astype(np.float32)
+  return x
+
+
+def exact_solution(mask, tgt, grad) -> np.ndarray:
+  """float64 [h, w, 3]: the solved pixels from a sparse direct solve of the
+  GridSolver system, the other pixels from tgt."""
+  import scipy.sparse as sp
+  import scipy.sparse.linalg as spl
+  m = np.asarray(mask) != 0
+  ii, jj = np.nonzero(m)
+  n = ii.size
+  ids = -np.ones(m.shape, np.int64)
+  ids[ii, jj] = np.arange(n)
+  rows, cols, vals = [np.arange(n)], [np.arange(n)], [np.full(n, 4.0)]
+  t = np.asarray(tgt, np.float64)
+  b = np.asarray(grad, np.float64)[ii, jj].copy()
+  for di, dj in _OFFS["ulrd"]:
+    nid = ids[ii + di, jj + dj]
+    inside = nid >= 0
+    rows.append(np.arange(n)[inside])
+    cols.append(nid[inside])
+    vals.append(-np.ones(int(inside.sum())))
+    b += np.where(inside[:, None], 0.0, t[ii + di, jj + dj])
+  A = sp.csc_matrix((np.concatenate(vals),
+                     (np.concatenate(rows), np.concatenate(cols))),
+                    shape=(n, n))
+  x = t.copy()
+  lu = spl.splu(A)
+  for c in range(3):
+    x[ii, jj, c] = lu.solve(b[:, c])
+  return x

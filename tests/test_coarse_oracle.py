@@ -1,0 +1,155 @@
+"""Pins of the coarse-grid correction oracle (CPU).
+
+Each check is chosen so that a plausible slip (wrong sign, a missing
+neighbour, an off-by-half in the cell-centred upsampling, a transposed
+block index) fails it:
+
+* restriction at factor 1 equals b - A x from a Laplacian assembled
+  independently (Kronecker sums on the full grid);
+* restriction at factor f equals the f x f block sums of factor 1, also on
+  sizes that f does not divide;
+* an exact fixed point (a linear ramp, discrete Laplacian 0) restricts to 0
+  exactly; the sparse direct solution restricts to float32 noise only and
+  is where the reference's Jacobi loop (C restatement) converges;
+* upsampling a dyadic constant adds it exactly; a dyadic linear coarse field
+  is reproduced exactly at the fine pixel centres between coarse centres;
+* weighted Jacobi with omega 1 is the reference's Jacobi sweep bit for bit,
+  omega 1/2 the mean of two consecutive sweeps;
+* the two-grid cycle built from these pieces (exact coarse solve) cuts the
+  error of a composite by a factor of 2.5 or more per cycle after the first.
+"""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import coarse_oracle as co
+
+
+def _problem(h, w, seed):
+  from fpie_amd import synth
+  return synth.composite((h, w), seed)
+
+
+def _laplacian_residual(mask, x, grad):
+  """g + sum(neighbours) - 4 x at solved pixels via a full-grid 5-point
+  operator assembled as Kronecker sums (float64)."""
+  h, w = mask.shape
+  def lap1(n):
+    return sp.diags([np.ones(n - 1), -2 * np.ones(n), np.ones(n - 1)],
+                    [-1, 0, 1])
+  L = sp.kron(lap1(h), sp.eye(w)) + sp.kron(sp.eye(h), lap1(w))
+  out = np.zeros((h, w, 3))
+  for c in range(3):
+    out[..., c] = (L @ x[..., c].astype(np.float64).reshape(-1)).reshape(h, w)
+  out += grad
+  return out * (mask[..., None] != 0)
+
+
+def test_restrict_factor1_is_the_residual():
+  mask, tgt, grad, _ = _problem(40, 52, 3)
+  rng = np.random.default_rng(0)
+  x = (tgt + rng.normal(0, 20, tgt.shape)).astype(np.float32)
+  got = co.restrict(mask, x, grad, 1)
+  want = _laplacian_residual(mask, x, grad)
+  np.testing.assert_allclose(got, want, rtol=0, atol=2e-3)
+
+
+@pytest.mark.parametrize("f", [2, 3, 4, 7])
+def test_restrict_is_block_sum(f):
+  mask, tgt, grad, _ = _problem(45, 61, 4)
+  x = np.random.default_rng(1).uniform(0, 255, tgt.shape).astype(np.float32)
+  fine = co.restrict(mask, x, grad, 1).astype(np.float64)
+  got = co.restrict(mask, x, grad, f)
+  hc, wc = 45 // f, 61 // f
+  assert got.shape == (hc, wc, 3)
+  for I in (0, hc // 2, hc - 1):
+    for J in (0, wc // 3, wc - 1):
+      want = fine[I * f:(I + 1) * f, J * f:(J + 1) * f].sum((0, 1))
+      np.testing.assert_allclose(got[I, J], want, rtol=1e-5, atol=1e-3)
+
+
+def test_restrict_of_exact_fixed_points():
+  from fpie_amd import process
+  h, w = 30, 34
+  ii, jj = np.mgrid[0:h, 0:w]
+  ramp = np.stack([2 * ii + jj, 3 * jj, 100 + ii - jj], -1).astype(np.float32)
+  mask = np.zeros((h, w), np.int32)
+  mask[3:-3, 4:-4] = 1
+  grad = process.divergence(ramp, ramp, mask,
+                            lambda a, b: process.mixgrad(a, b, "src"))
+  assert not co.restrict(mask, ramp, grad, 1).any()
+  assert not co.restrict(mask, ramp, grad, 5).any()
+  m2, t2, g2, _ = _problem(36, 40, 9)
+  xs = co.exact_solution(m2, t2, g2).astype(np.float32)
+  assert np.abs(co.restrict(m2, xs, g2, 1)).max() < 2e-3
+
+
+def test_exact_solution_is_where_jacobi_converges(oracle):
+  mask, tgt, grad, _ = _problem(20, 24, 11)
+  xs = co.exact_solution(mask, tgt, grad)
+  xj = oracle.grid_jacobi(mask, tgt, grad, 4000, "ulrd")
+  m = mask != 0
+  assert np.abs(xj - xs)[m].max() < 1e-3
+  assert np.array_equal(xs[~m], tgt[~m])
+
+
+def test_prolong_constant_and_linear():
+  h, w, f = 32, 40, 4
+  mask = np.zeros((h, w), np.int32)
+  mask[2:-2, 3:-3] = 1
+  x = np.random.default_rng(2).uniform(0, 255, (h, w, 3)).astype(np.float32)
+  x = np.round(x * 8) / 8  # dyadic, so every sum below is exact
+  hc, wc = h // f, w // f
+  const = np.full((hc, wc, 3), 2.5, np.float32)
+  got = co.prolong(mask, x, const, f)
+  m = mask != 0
+  assert np.array_equal(got[m], x[m] + np.float32(2.5))
+  assert np.array_equal(got[~m], x[~m])
+  I, J = np.mgrid[0:hc, 0:wc]
+  lin = np.stack([I * 2.0 - J, J * 0.5, I + J], -1).astype(np.float32)
+  got = co.prolong(np.ones((h, w), np.int32), np.zeros_like(x), lin, f)
+  i, j = np.mgrid[0:h, 0:w]
+  ys, xs = (i + 0.5) / f - 0.5, (j + 0.5) / f - 0.5
+  want = np.stack([ys * 2.0 - xs, xs * 0.5, ys + xs], -1)
+  inner = (ys >= 0) & (ys <= hc - 1) & (xs >= 0) & (xs <= wc - 1)
+  np.testing.assert_array_equal(got[inner], want[inner].astype(np.float32))
+
+
+def test_relax_weights(oracle):
+  mask, tgt, grad, _ = _problem(33, 47, 5)
+  x = np.random.default_rng(3).uniform(0, 255, tgt.shape).astype(np.float32)
+  one = co.relax(mask, x, grad, 1, 1.0)
+  ref = oracle.grid_jacobi(mask, x, grad, 1, "ulrd")
+  assert np.array_equal(one.view(np.uint32), ref.view(np.uint32))
+  half = co.relax(mask, x, grad, 1, 0.5)
+  np.testing.assert_allclose(half, 0.5 * (x.astype(np.float64) + ref),
+                             rtol=2e-7, atol=1e-5)
+
+
+def _two_grid_errors(mask, tgt, grad, f, cycles, sweeps):
+  xs = co.exact_solution(mask, tgt, grad)
+  m = mask != 0
+  hc, wc = mask.shape[0] // f, mask.shape[1] // f
+  mc = (m[:hc * f, :wc * f].reshape(hc, f, wc, f).mean((1, 3)) > 0.5)
+  mc[[0, -1], :] = False
+  mc[:, [0, -1]] = False
+  x = tgt.astype(np.float32).copy()
+  errs = [np.abs(x - xs)[m].max()]
+  for _ in range(cycles):
+    rc = co.restrict(mask, x, grad, f)
+    cc = co.exact_solution(mc, np.zeros((hc, wc, 3)), rc * mc[..., None])
+    x = co.prolong(mask, x, cc.astype(np.float32), f)
+    x = co.relax(mask, x, grad, sweeps, 1.0)
+    x = co.relax(mask, x, grad, 1, 0.5)
+    errs.append(np.abs(x - xs)[m].max())
+  return errs
+
+
+def test_two_grid_cycle_converges():
+  mask, tgt, grad, _ = _problem(96, 96, 21)
+  errs = _two_grid_errors(mask, tgt, grad, 4, 3, 40)
+  assert errs[1] < errs[0] / 1.5, errs
+  assert errs[2] < errs[1] / 2.5, errs
+  assert errs[3] < errs[2] / 2.5, errs
+  assert errs[3] < errs[0] / 15, errs

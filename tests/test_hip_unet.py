@@ -83,13 +83,3 @@ def test_adam_steps_reduce_the_loss(unet):
   assert last < 0.5 * first, (first, last)
   st = net.stats()
   assert st["conv_flops"] > 0 and st["ms"] > 0
-
-
-def test_bad_prediction_falls_back_to_cold_start(unet):
-  """A start with a larger residual than the target state is rejected."""
-  mask, tgt, grad, _ = unet.composite((64, 64), 3)
-  cold = unet.mean_residual(mask, tgt, grad, tgt)
-  bad = tgt + 50.0 * (mask[..., None] != 0)
-  assert unet.mean_residual(mask, tgt, grad, bad) > cold
-  x0 = unet.choose_start(mask, tgt, grad, bad)
-  np.testing.assert_array_equal(x0, tgt)
