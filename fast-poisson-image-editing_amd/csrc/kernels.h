@@ -143,10 +143,10 @@ hipError_t launch_reduce(const double* partial, int n, float* err,
 
 // ---- coarse-grid correction (warm start) -------------------------------
 // out[hc][wc][3] = per f x f block, the sum over solved pixels of the
-// signed residual (g + neighbours) - 4 x.
-hipError_t launch_restrict(int order, const float* x, const float* g,
-                           const uint8_t* m, long long plane, int P, int f,
-                           int hc, int wc, float* out, hipStream_t st);
+// signed residual (g + neighbours) - 4 x, in double.
+hipError_t launch_restrict(const float* x, const float* g, const uint8_t* m,
+                           long long plane, int P, int f, int hc, int wc,
+                           float* out, hipStream_t st);
 // x += cell-centred bilinear upsampling of corr[hc][wc][3] at solved pixels
 hipError_t launch_prolong(float* x, const uint8_t* m, long long plane, int H,
                           int W, int P, int f, int hc, int wc,

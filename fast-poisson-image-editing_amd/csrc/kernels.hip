@@ -332,9 +332,10 @@ __device__ __forceinline__ float neighbour_sum(const float* __restrict__ xc,
 }
 
 // out[I][J][c] = sum over the solved pixels of the f x f block (I, J) of the
-// signed residual (float neighbour sum) - 4 x, accumulated in double in
+// signed residual g + four neighbours - 4 x, each term and the sums in double
+// (exact for float inputs: a float32 sum would round to the state's own
+// fixed point and hide smooth errors below half an ulp per pixel), in
 // row-major order inside the block.  One thread per coarse cell.
-template <int ORDER>
 __global__ void __launch_bounds__(kBlock)
     restrict_kernel(const float* __restrict__ x, const float* __restrict__ g,
                     const uint8_t* __restrict__ m, long long plane, int P,
@@ -351,8 +352,10 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const float* xc = x + c * plane;
-        const float s = neighbour_sum<ORDER>(xc, g[c * plane + o], o, P);
-        acc[c] += (double)s - (double)xc[o] * 4.0;
+        const double s = (double)g[c * plane + o] + (double)xc[o - P] +
+                         (double)xc[o - 1] + (double)xc[o + 1] +
+                         (double)xc[o + P];
+        acc[c] += s - (double)xc[o] * 4.0;
       }
     }
   }
@@ -563,16 +566,12 @@ hipError_t launch_finalize(int order, const float* x, const float* g,
   return hipGetLastError();
 }
 
-hipError_t launch_restrict(int order, const float* x, const float* g,
-                           const uint8_t* m, long long plane, int P, int f,
-                           int hc, int wc, float* out, hipStream_t st) {
+hipError_t launch_restrict(const float* x, const float* g, const uint8_t* m,
+                           long long plane, int P, int f, int hc, int wc,
+                           float* out, hipStream_t st) {
   const int blocks = (int)(((long long)hc * wc + kBlock - 1) / kBlock);
-  if (order == ORDER_UDLR)
-    hipLaunchKernelGGL(restrict_kernel<ORDER_UDLR>, dim3(blocks), dim3(kBlock),
-                       0, st, x, g, m, plane, P, f, hc, wc, out);
-  else
-    hipLaunchKernelGGL(restrict_kernel<ORDER_ULRD>, dim3(blocks), dim3(kBlock),
-                       0, st, x, g, m, plane, P, f, hc, wc, out);
+  hipLaunchKernelGGL(restrict_kernel, dim3(blocks), dim3(kBlock), 0, st, x, g,
+                     m, plane, P, f, hc, wc, out);
   return hipGetLastError();
 }
 

@@ -621,9 +621,8 @@ int pie_restrict(pie_solver* s, int factor, float* out) {
   int hc = 0, wc = 0;
   if (!out) return fail(PIE_EINVAL, "NULL argument");
   PIE_CHECK(coarse_grid(s, factor, &hc, &wc));
-  PIE_HIP(pie::launch_restrict(s->eff_order(), s->state(), s->g0(), s->m0(),
-                               s->plane, s->P, factor, hc, wc, s->coarse.p,
-                               s->stream));
+  PIE_HIP(pie::launch_restrict(s->state(), s->g0(), s->m0(), s->plane, s->P,
+                               factor, hc, wc, s->coarse.p, s->stream));
   PIE_HIP(hipMemcpyAsync(out, s->coarse.p, (size_t)hc * wc * 3 * sizeof(float),
                          hipMemcpyDeviceToHost, s->stream));
   PIE_HIP(hipStreamSynchronize(s->stream));

@@ -147,8 +147,8 @@ int pie_solve_to_residual(pie_solver* s, double tol_mean, int check_every,
  * the reference.  The coarse grid of `factor` is (h // factor) x
  * (w // factor); block (I, J) covers rows I*factor.., cols J*factor.. */
 /* out float[hc][wc][3]: per block, the sum over its solved pixels of the
- * signed residual (g + four neighbours, float adds in the solver's order)
- * - 4 x of the current state, accumulated in double. */
+ * signed residual g + four neighbours - 4 x of the current state, in double
+ * (the exact residual of the float state; rounded to float once). */
 int pie_restrict(pie_solver* s, int factor, float* out);
 /* state += cell-centred bilinear upsampling of corr float[hc][wc][3] at the
  * solved pixels; hc, wc must be the coarse grid of `factor`. */

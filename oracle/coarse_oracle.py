@@ -8,7 +8,8 @@ The warm start has no counterpart in the reference (fpie 0.3.2), whose
 GridSolver only iterates (fpie/core/gcc/grid.cc:21-80,
 fpie/core/openmp/grid.cc:29-110).  What the device calls pie_restrict,
 pie_prolong and pie_relax compute is restated here with numpy in the same
-float32 operation order, plus an exact sparse solve (scipy, float64) of the
+float32 operation order (the restriction in float64, as the kernel), plus an
+exact sparse solve (scipy, float64) of the
 GridSolver system that the tests use as the converged solution:
 
   4 x_p - sum_{q in N(p), q solved} x_q = g_p + sum_{q in N(p), q unsolved} t_q
@@ -38,14 +39,17 @@ def neighbour_sum(x: np.ndarray, grad: np.ndarray, order: str = "ulrd"):
   return s
 
 
-def restrict(mask, x, grad, f: int, order: str = "ulrd") -> np.ndarray:
-  """Block sums over f x f blocks of the signed residual s - 4x at solved
-  pixels; s is the float32 neighbour sum, the difference and the sums are
-  float64.  Returns float32 [h // f, w // f, 3]."""
+def restrict(mask, x, grad, f: int) -> np.ndarray:
+  """Block sums over f x f blocks of the signed residual g + neighbours - 4x
+  at solved pixels, all in float64 (exact for float32 x and g).  Returns
+  float32 [h // f, w // f, 3]."""
   m = np.asarray(mask) != 0
-  x = np.asarray(x, np.float32)
-  s = neighbour_sum(x, grad, order)
-  r = (s.astype(np.float64) - x.astype(np.float64) * 4.0) * m[..., None]
+  x = np.asarray(x, np.float32).astype(np.float64)
+  g = np.asarray(grad, np.float32).astype(np.float64)
+  r = np.zeros_like(x)
+  r[1:-1, 1:-1] = (g[1:-1, 1:-1] + x[:-2, 1:-1] + x[1:-1, :-2] +
+                   x[1:-1, 2:] + x[2:, 1:-1] - 4.0 * x[1:-1, 1:-1])
+  r = r * m[..., None]
   h, w = (m.shape[0] // f) * f, (m.shape[1] // f) * f
   blocks = r[:h, :w].reshape(h // f, f, w // f, f, 3)
   return blocks.sum((1, 3)).astype(np.float32)

@@ -52,7 +52,7 @@ def test_restrict_factor1_is_the_residual():
   x = (tgt + rng.normal(0, 20, tgt.shape)).astype(np.float32)
   got = co.restrict(mask, x, grad, 1)
   want = _laplacian_residual(mask, x, grad)
-  np.testing.assert_allclose(got, want, rtol=0, atol=2e-3)
+  np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-6)
 
 
 @pytest.mark.parametrize("f", [2, 3, 4, 7])

@@ -154,6 +154,14 @@ def test_coarsest_direction_is_the_exact_coarsest_solve():
                              atol=1e-5 * np.abs(want).max())
 
 
+def test_exact_reference_matches_sparse_direct_solve():
+  from fpie_amd import exact_ref
+  mask, tgt, grad, _ = synth.composite((48, 56), 41)
+  want = co.exact_solution(mask, tgt, grad)
+  got = exact_ref.solve(mask, tgt, grad, tgt, log=lambda *_: None)
+  np.testing.assert_allclose(got, want, rtol=0, atol=1e-8)
+
+
 def test_bad_prediction_falls_back_to_cold_start(monkeypatch):
   mask, tgt, grad, _ = synth.composite((64, 64), 3)
   ws = warmstart.WarmStart(np.zeros(1, np.float32), coarse=16, cycles=2)
