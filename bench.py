@@ -10,7 +10,9 @@ timed region).
 
 Printed: one JSON line with the metric (solved-pixel updates per second),
 the roofline of the sweep kernel (algorithmic bytes per launch over its
-event-timed average launch duration, against the 8 TB/s HBM peak) and the
+event-timed average launch duration, against the 8 TB/s HBM peak), the
+time to a mean residual of 1e-3 on an unseen 1024 x 1024 composite with and
+without the learned warm start (``time_to_residual``) and the
 reference's CPU path timed on this host (its openmp GridSolver compiled from
 the reference sources, or the oracle's restatement when that is absent) on a
 bounded sample of the same workload.
@@ -168,6 +170,51 @@ def cpu_baseline(w, budget_s: float):
                     f"{cores} thread(s)"}
 
 
+def bench_warm_start(size: int = 1024, seed: int = 900_001,
+                     tol: float = 1e-3) -> dict:
+  """Time to a mean absolute residual of `tol` on an unseen size x size
+  composite: plain Jacobi from the target (cold) vs the learned warm start
+  (fpie_amd.warmstart with the packaged network) followed by plain Jacobi.
+  Accuracy against the exact solution is measured by
+  `python -m fpie_amd.warmstart` (it needs a float64 reference solve)."""
+  from fpie_amd import synth, warmstart
+  from fpie_amd.core_hip import GridSolver
+  if not os.path.exists(warmstart.WEIGHTS):
+    return {"skipped": "no packaged network weights"}
+  mask, tgt, grad, mode = synth.composite((size, size), seed)
+
+  def solver():
+    s = GridSolver(schedule="jacobi")
+    s.reset(0, mask, tgt, grad)
+    s.set_tile(8, 1, 64)
+    s.sync()
+    return s
+
+  ws = warmstart.WarmStart(warmstart.load_params())
+  ws.run(solver(), mask, tgt)  # untimed: kernel loading, network set-up
+  s = solver()
+  t = time.perf_counter()
+  n_cold, r_cold = s.solve_to_residual(tol, 512, 10**8)
+  t_cold = time.perf_counter() - t
+  del s
+  s = solver()
+  t = time.perf_counter()
+  info = ws.run(s, mask, tgt)
+  n_warm, r_warm = s.solve_to_residual(tol, 512, 10**8)
+  t_warm = time.perf_counter() - t
+  return {
+      "workload": f"composite {size}x{size} seed {seed} (blob source and "
+                  f"target, star mask, gradient {mode})",
+      "solved_pixels": int(np.count_nonzero(mask)),
+      "tol_mean_residual": tol,
+      "cold": {"seconds": t_cold, "sweeps": n_cold, "residual": r_cold},
+      "warm": {"seconds": t_warm, "cycles": info["cycles"],
+               "smoothing_sweeps": info["sweeps"], "jacobi_sweeps": n_warm,
+               "residual": r_warm, "prediction_used": info["used"]},
+      "speedup": t_cold / max(t_warm, 1e-9),
+  }
+
+
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument("--gpus", type=int, default=1)
@@ -183,6 +230,8 @@ def main():
                   help="S,C,rows to skip the autotuner")
   ap.add_argument("--cpu-budget", type=float, default=12.0)
   ap.add_argument("--no-cpu-baseline", action="store_true")
+  ap.add_argument("--no-warm-start", action="store_true",
+                  help="skip the time-to-residual comparison")
   a = ap.parse_args()
   if a.gpus != 1:
     print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -240,7 +289,8 @@ def main():
   if traffic and traffic.get("tile") != list(tile):
     traffic = None  # measured for another tile
   line = {
-      "metric": "Jacobi pixel-updates/s + % HBM peak",
+      "metric": "Jacobi pixel-updates/s + % HBM peak; time-to-residual "
+                "warm-start vs plain",
       "value": value,
       "unit": "pixel-updates/s",
       "n_gpus": 1,
@@ -269,6 +319,9 @@ def main():
                    "avg_launch_ms": launch_ms,
                    "launches_timed": tm["launches"]},
   }
+  if not a.no_warm_start and w.method == "grid":
+    del core
+    line["time_to_residual"] = bench_warm_start()
   if not a.no_cpu_baseline:
     line["cpu_baseline"] = cpu_baseline(w, a.cpu_budget)
   print(json.dumps(line), flush=True)

@@ -151,6 +151,15 @@ hipError_t launch_restrict(const float* x, const float* g, const uint8_t* m,
 hipError_t launch_prolong(float* x, const uint8_t* m, long long plane, int H,
                           int W, int P, int f, int hc, int wc,
                           const float* corr, hipStream_t st);
+// Line-search sums of the prolonged correction d of corr: partial[b][0..2]
+// = sum r . d (exact residual of x), partial[b][3..5] = sum d . A d, per
+// channel, for blocks b < nblocks (prolong_dots_blocks()).
+int prolong_dots_blocks(int H, int W);
+hipError_t launch_prolong_dots(const float* x, const float* g,
+                               const uint8_t* m, long long plane, int H,
+                               int W, int P, int f, int hc, int wc,
+                               const float* corr, double* partial,
+                               int nblocks, hipStream_t st);
 // one weighted Jacobi sweep xin -> xout (weight omega)
 hipError_t launch_weighted(int order, const float* xin, float* xout,
                            const float* g, const uint8_t* m, long long plane,

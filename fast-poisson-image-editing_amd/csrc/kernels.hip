@@ -363,9 +363,32 @@ __global__ void __launch_bounds__(kBlock)
   for (int c = 0; c < 3; ++c) out[id * 3 + c] = (float)acc[c];
 }
 
-// x[i][j] += bilinear sample of corr[hc][wc][3] (cell-centred: fine pixel
-// (i, j) sits at coarse coordinate ((i + 0.5) / f - 0.5, (j + 0.5) / f - 0.5),
-// clamped to the coarse grid) at the solved pixels.
+// Bilinear sample of corr[hc][wc][3] at fine pixel (i, j), cell-centred:
+// the pixel sits at coarse coordinate ((i + 0.5) / f - 0.5,
+// (j + 0.5) / f - 0.5), clamped to the coarse grid.
+__device__ __forceinline__ void bilinear3(const float* __restrict__ corr,
+                                          int i, int j, float inv, int hc,
+                                          int wc, float v[3]) {
+  const float ys = ((float)i + 0.5f) * inv - 0.5f;
+  const float xs = ((float)j + 0.5f) * inv - 0.5f;
+  const int y0 = min(max((int)floorf(ys), 0), hc - 1);
+  const int x0 = min(max((int)floorf(xs), 0), wc - 1);
+  const int y1 = min(y0 + 1, hc - 1), x1 = min(x0 + 1, wc - 1);
+  const float fy = fminf(fmaxf(ys - (float)y0, 0.0f), 1.0f);
+  const float fx = fminf(fmaxf(xs - (float)x0, 0.0f), 1.0f);
+  const float* c00 = corr + ((long long)y0 * wc + x0) * 3;
+  const float* c01 = corr + ((long long)y0 * wc + x1) * 3;
+  const float* c10 = corr + ((long long)y1 * wc + x0) * 3;
+  const float* c11 = corr + ((long long)y1 * wc + x1) * 3;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float top = c00[c] * (1.0f - fx) + c01[c] * fx;
+    const float bot = c10[c] * (1.0f - fx) + c11[c] * fx;
+    v[c] = top * (1.0f - fy) + bot * fy;
+  }
+}
+
+// x[i][j] += bilinear sample of corr at the solved pixels.
 __global__ void __launch_bounds__(kBlock)
     prolong_kernel(float* __restrict__ x, const uint8_t* __restrict__ m,
                    long long plane, int H, int W, int P, int f, int hc, int wc,
@@ -377,24 +400,66 @@ __global__ void __launch_bounds__(kBlock)
     const int i = (int)(id / W), j = (int)(id % W);
     const long long o = (long long)i * P + j;
     if (!m[o]) continue;
-    const float ys = ((float)i + 0.5f) * inv - 0.5f;
-    const float xs = ((float)j + 0.5f) * inv - 0.5f;
-    const int y0 = min(max((int)floorf(ys), 0), hc - 1);
-    const int x0 = min(max((int)floorf(xs), 0), wc - 1);
-    const int y1 = min(y0 + 1, hc - 1), x1 = min(x0 + 1, wc - 1);
-    const float fy = fminf(fmaxf(ys - (float)y0, 0.0f), 1.0f);
-    const float fx = fminf(fmaxf(xs - (float)x0, 0.0f), 1.0f);
-    const float* c00 = corr + ((long long)y0 * wc + x0) * 3;
-    const float* c01 = corr + ((long long)y0 * wc + x1) * 3;
-    const float* c10 = corr + ((long long)y1 * wc + x0) * 3;
-    const float* c11 = corr + ((long long)y1 * wc + x1) * 3;
+    float v[3];
+    bilinear3(corr, i, j, inv, hc, wc, v);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) x[c * plane + o] += v[c];
+  }
+}
+
+// Line-search sums of a prolonged correction d (d = bilinear sample of corr
+// at solved pixels, 0 elsewhere), per channel, in double:
+// partial[block][0..2] += r . d with r the exact residual g + neighbours
+// - 4 x of the state, partial[block][3..5] += d . (A d) with
+// (A d)_p = 4 d_p - sum of the neighbours' d.
+__global__ void __launch_bounds__(kBlock)
+    prolong_dots_kernel(const float* __restrict__ x,
+                        const float* __restrict__ g,
+                        const uint8_t* __restrict__ m, long long plane, int H,
+                        int W, int P, int f, int hc, int wc,
+                        const float* __restrict__ corr,
+                        double* __restrict__ partial) {
+  double rd[3] = {0.0, 0.0, 0.0}, dad[3] = {0.0, 0.0, 0.0};
+  const long long n = (long long)H * W;
+  const float inv = 1.0f / (float)f;
+  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < n;
+       id += (long long)gridDim.x * kBlock) {
+    const int i = (int)(id / W), j = (int)(id % W);
+    const long long o = (long long)i * P + j;
+    if (!m[o]) continue;  // solved pixels are never on the crop border
+    float d[3], dn[3];
+    double ad[3];
+    bilinear3(corr, i, j, inv, hc, wc, d);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ad[c] = 4.0 * (double)d[c];
+    const int ni[4] = {i - 1, i, i, i + 1}, nj[4] = {j, j - 1, j + 1, j};
+    const long long no[4] = {o - P, o - 1, o + 1, o + P};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (!m[no[k]]) continue;
+      bilinear3(corr, ni[k], nj[k], inv, hc, wc, dn);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ad[c] -= (double)dn[c];
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const float top = c00[c] * (1.0f - fx) + c01[c] * fx;
-      const float bot = c10[c] * (1.0f - fx) + c11[c] * fx;
-      x[c * plane + o] += top * (1.0f - fy) + bot * fy;
+      const float* xc = x + c * plane;
+      const double r = (double)g[c * plane + o] + (double)xc[o - P] +
+                       (double)xc[o - 1] + (double)xc[o + 1] +
+                       (double)xc[o + P] - 4.0 * (double)xc[o];
+      rd[c] += r * (double)d[c];
+      dad[c] += (double)d[c] * ad[c];
     }
   }
+  block_sum3(rd);
+  __syncthreads();  // the shared buffer is reused by the second sum
+  block_sum3(dad);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      partial[blockIdx.x * 6 + c] = rd[c];
+      partial[blockIdx.x * 6 + 3 + c] = dad[c];
+    }
 }
 
 // One weighted Jacobi sweep: xout = xin + omega * ((sum) * 0.25 - xin) at
@@ -581,6 +646,20 @@ hipError_t launch_prolong(float* x, const uint8_t* m, long long plane, int H,
   hipLaunchKernelGGL(prolong_kernel,
                      dim3(grid_blocks((long long)H * W, 8192)), dim3(kBlock),
                      0, st, x, m, plane, H, W, P, f, hc, wc, corr);
+  return hipGetLastError();
+}
+
+int prolong_dots_blocks(int H, int W) {
+  return grid_blocks((long long)H * W, 2048);
+}
+
+hipError_t launch_prolong_dots(const float* x, const float* g,
+                               const uint8_t* m, long long plane, int H,
+                               int W, int P, int f, int hc, int wc,
+                               const float* corr, double* partial,
+                               int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(prolong_dots_kernel, dim3(nblocks), dim3(kBlock), 0, st,
+                     x, g, m, plane, H, W, P, f, hc, wc, corr, partial);
   return hipGetLastError();
 }
 

@@ -644,6 +644,31 @@ int pie_prolong(pie_solver* s, int factor, int hc, int wc, const float* corr) {
   return PIE_OK;
 }
 
+int pie_prolong_dots(pie_solver* s, int factor, int hc, int wc,
+                     const float* corr, double* out) {
+  int ec = 0, ew = 0;
+  if (!corr || !out) return fail(PIE_EINVAL, "NULL argument");
+  PIE_CHECK(coarse_grid(s, factor, &ec, &ew));
+  if (hc != ec || wc != ew)
+    return fail(PIE_EINVAL, "coarse grid must be (h // factor, w // factor)");
+  const int nb = pie::prolong_dots_blocks(s->H, s->W);
+  DevBuf<double> part;
+  PIE_HIP(part.alloc((size_t)nb * 6));
+  PIE_CHECK(upload(corr, (size_t)hc * wc * 3 * sizeof(float), s->coarse.p,
+                   s->stream));
+  PIE_HIP(pie::launch_prolong_dots(s->state(), s->g0(), s->m0(), s->plane,
+                                   s->H, s->W, s->P, factor, hc, wc,
+                                   s->coarse.p, part.p, nb, s->stream));
+  std::vector<double> h((size_t)nb * 6);
+  PIE_HIP(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, s->stream));
+  PIE_HIP(hipStreamSynchronize(s->stream));
+  for (int k = 0; k < 6; ++k) out[k] = 0.0;
+  for (int b = 0; b < nb; ++b)  // fixed order: deterministic
+    for (int k = 0; k < 6; ++k) out[k] += h[(size_t)b * 6 + k];
+  return PIE_OK;
+}
+
 int pie_relax(pie_solver* s, int sweeps, float omega) {
   if (!s) return fail(PIE_EINVAL, "NULL solver");
   if (!s->loaded) return fail(PIE_ESTATE, "relax before reset");

@@ -228,6 +228,19 @@ class GridSolver(_Core):
     native.check(self._lib.pie_prolong(self._h, int(factor), hc, wc,
                                        _ptr(corr)))
 
+  def prolong_dots(self, factor: int, corr: np.ndarray) -> np.ndarray:
+    """float64 [2, 3]: per channel, sum r . d and sum d . A d for the
+    correction d that prolong(factor, corr) would add (r the exact residual
+    of the state); the energy-optimal scale of corr is row 0 / row 1."""
+    corr = np.ascontiguousarray(corr, dtype=np.float32)
+    hc, wc = self.coarse_shape(factor)
+    if corr.shape != (hc, wc, 3):
+      raise ValueError(f"correction must be {(hc, wc, 3)}, not {corr.shape}")
+    out = np.zeros(6, np.float64)
+    native.check(self._lib.pie_prolong_dots(self._h, int(factor), hc, wc,
+                                            _ptr(corr), _ptr(out)))
+    return out.reshape(2, 3)
+
   def relax(self, sweeps: int = 1, omega: float = 0.5) -> None:
     """`sweeps` weighted Jacobi sweeps x <- x + omega ((g + sum) / 4 - x)."""
     native.check(self._lib.pie_relax(self._h, int(sweeps), float(omega)))

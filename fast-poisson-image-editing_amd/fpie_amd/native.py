@@ -56,6 +56,7 @@ _SIGNATURES = {
     "pie_restrict": ([_vp, _ip, _vp], _ip),
     "pie_prolong": ([_vp, _ip, _ip, _ip, _vp], _ip),
     "pie_relax": ([_vp, _ip, ctypes.c_float], _ip),
+    "pie_prolong_dots": ([_vp, _ip, _ip, _ip, _vp, _vp], _ip),
     "pie_unet_create": ([_ip, _ip, _ip, ctypes.POINTER(_vp)], _ip),
     "pie_unet_destroy": ([_vp], _ip),
     "pie_unet_param_count": ([_vp, ctypes.POINTER(ctypes.c_longlong)], _ip),

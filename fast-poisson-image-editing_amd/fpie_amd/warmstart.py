@@ -12,37 +12,39 @@ smooth-accurate (an upsampled prediction) therefore gains nothing; it has to
 be free of both.
 
 Algorithm (device work through the GridSolver calls restrict / prolong /
-relax / iterate; the coarse problem on the host with the network on the
-GPU):
+relax / iterate; the coarse problem is itself a GridSolver problem):
 
-  smooth: 6 f^2 plain Jacobi sweeps, then one weighted sweep with omega 1/2
-          (the mean of two consecutive Jacobi iterates), which removes the
-          fine error the coarse grid cannot carry and the odd-even modes
+  smooth: 24 f^2 plain Jacobi sweeps, then one weighted sweep with
+          omega 1/2 (the mean of two consecutive Jacobi iterates), which
+          removes the fine error the coarse grid cannot carry and the
+          odd-even modes
   smooth once (the first coarse residual is then free of pixel noise)
   for each cycle:
-    r_c = block sums (factor f) of the residual b - A x of the state
-    solve A_c c = r_c (zero on the coarse boundary) approximately by
-      `coarse_iters` steps of flexible conjugate gradients with two search
-      directions per step:
-        - the U-Net's answer to the current coarse residual (per-channel
-          rms-normalised), low-passed by 4 damped smoothing sweeps of the
-          coarse Laplacian (bf16 noise in the output would otherwise
-          dominate <p, A_c p> and shrink every step);
-        - the exact solution of the residual's restriction to a grid 4x
-          coarser again (sparse LU, factorised once per problem; at most
-          64 x 64 cells), upsampled: the few smoothest modes, which carry
-          the largest error and the smallest residual;
-      each direction is A_c-orthogonalised against the cycle's earlier ones
-      and taken with the energy-optimal step, so the coarse error's energy
-      norm never grows whatever the network returns
-    x += bilinear upsampling of c (solved pixels); smooth
-    stop once max |c| < `stop` pixel units, or when the mean residual of the
-    state did not fall over the cycle
+    r_c = block sums (factor f) of the residual b - A x of the state, in
+          double (a float32 residual rounds to zero at the float fixed
+          points of Jacobi and hides smooth errors)
+    coarse correction c for A_c c = r_c (c = 0 on the coarse boundary):
+      `net_steps` (2) flexible conjugate-gradient steps whose search
+      directions are the U-Net's answers to the current coarse residual
+      (per-channel rms-normalised), low-passed by 4 damped smoothing sweeps
+      of the coarse Laplacian (bf16 noise would otherwise dominate
+      <p, A_c p>), A_c-orthogonalised and taken with the energy-optimal
+      step.  Optionally (`coarse_rtol` set) the coarse GridSolver then
+      polishes c: one omega-1/2 sweep and plain Jacobi to `coarse_rtol`
+      times the zero start's mean residual (slower, more accurate); with
+      precond "none" that is a classical two-grid cycle with no network
+    d = bilinear upsampling of c (solved pixels); x += alpha d with the
+      per-channel alpha = <r, d> / <d, A d> on the fine grid (exact
+      residual r): the energy-optimal multiple, so no cycle can raise the
+      fine error's energy norm even where the 5-point coarse problem is a
+      poor model of the fine one (it is not the Galerkin operator); smooth
+    stop once max |alpha c| < `stop` pixel units, or when the mean residual
+    of the state did not fall over the cycle
   fallback: if the result's mean residual is not below the target state's
   (the cold start's), the state is reset to the target.
 
-Plain Jacobi then finishes to the residual tolerance.  f is the power of two
-that brings the larger side to about `coarse` (256) pixels.
+Plain Jacobi then finishes to the residual tolerance.  f is the power of two,
+at least 4, that brings the larger side to about `coarse` (256) cells.
 
 Training data (correction form; labels from the GPU Jacobi solver itself):
 composites drawn at `render` x `render` (blob source and target, star mask,
@@ -126,61 +128,6 @@ def lowpass(mc: np.ndarray, p: np.ndarray, sweeps: int = 4) -> np.ndarray:
   return q
 
 
-def upsample(a: np.ndarray, f: int, h: int, w: int) -> np.ndarray:
-  """Cell-centred bilinear upsampling of a [hc, wc, 3] by f to [h, w, 3]
-  (the pie_prolong geometry, in float64)."""
-  hc, wc = a.shape[:2]
-  ys = (np.arange(h) + 0.5) / f - 0.5
-  xs = (np.arange(w) + 0.5) / f - 0.5
-  y0 = np.clip(np.floor(ys).astype(np.int64), 0, hc - 1)
-  x0 = np.clip(np.floor(xs).astype(np.int64), 0, wc - 1)
-  y1, x1 = np.minimum(y0 + 1, hc - 1), np.minimum(x0 + 1, wc - 1)
-  fy = np.clip(ys - y0, 0.0, 1.0)[:, None, None]
-  fx = np.clip(xs - x0, 0.0, 1.0)[None, :, None]
-  top = a[y0][:, x0] * (1 - fx) + a[y0][:, x1] * fx
-  bot = a[y1][:, x0] * (1 - fx) + a[y1][:, x1] * fx
-  return top * (1 - fy) + bot * fy
-
-
-class CoarsestSolve:
-  """Exact solve of the coarse problem restricted by `g` (block sums,
-  majority mask): sparse LU of the 5-point system, factorised once."""
-
-  def __init__(self, mc: np.ndarray, g: int):
-    import scipy.sparse as sp
-    import scipy.sparse.linalg as spl
-    self.g, self.shape = g, mc.shape
-    self.mcc = coarse_mask(mc, g)
-    ii, jj = np.nonzero(self.mcc)
-    self.ii, self.jj = ii, jj
-    n = ii.size
-    self.lu = None
-    if n == 0:
-      return
-    ids = -np.ones(self.mcc.shape, np.int64)
-    ids[ii, jj] = np.arange(n)
-    rows, cols = [np.arange(n)], [np.arange(n)]
-    vals = [np.full(n, 4.0)]
-    for di, dj in ((-1, 0), (1, 0), (0, -1), (0, 1)):
-      nid = ids[ii + di, jj + dj]
-      ok = nid >= 0
-      rows.append(np.arange(n)[ok])
-      cols.append(nid[ok])
-      vals.append(-np.ones(int(ok.sum())))
-    A = sp.csc_matrix((np.concatenate(vals),
-                       (np.concatenate(rows), np.concatenate(cols))),
-                      shape=(n, n))
-    self.lu = spl.splu(A)
-
-  def direction(self, r: np.ndarray, mc: np.ndarray) -> np.ndarray:
-    if self.lu is None:
-      return np.zeros(r.shape)
-    rcc = block_sum(r, self.g)[self.ii, self.jj]
-    e = np.zeros(self.mcc.shape + (3,))
-    e[self.ii, self.jj] = self.lu.solve(rcc)
-    return upsample(e, self.g, *self.shape) * mc[..., None]
-
-
 def _dot(a: np.ndarray, b: np.ndarray) -> np.ndarray:
   """Per-channel inner products of [h, w, 3] fields (float64)."""
   return np.einsum("ijc,ijc->c", a, b, dtype=np.float64)
@@ -202,28 +149,31 @@ def load_params(path: str | None = None) -> np.ndarray:
 class WarmStart:
   """The learned warm start of one trained network (see module docstring).
 
-  `sweeps` plain Jacobi sweeps per smoothing (default 6 f^2); networks are
-  kept per coarse shape, so repeated problems of one size reuse them.
-  precond "net" uses the network, "none" plain conjugate gradients (the
-  same method without the network, for comparison)."""
+  `sweeps` plain Jacobi sweeps per smoothing (default 24 f^2); networks
+  are kept per coarse shape, so repeated problems of one size reuse them.
+  precond "none" drops the network, for comparison.  `coarse_solver` makes
+  the coarse GridSolver of the optional polish."""
 
-  def __init__(self, params: np.ndarray, coarse: int = 256, cycles: int = 16,
+  def __init__(self, params: np.ndarray, coarse: int = 256, cycles: int = 12,
                sweeps: int | None = None, omega: float = 0.5,
-               stop: float = 0.005, coarse_tol: float = 1e-3,
-               coarse_iters: int = 2, precond: str = "net",
-               lowpass_sweeps: int = 4, coarsest: int = 64):
+               stop: float = 0.005, net_steps: int = 2, precond: str = "net",
+               lowpass_sweeps: int = 4, coarse_rtol: float | None = None,
+               coarse_max_sweeps: int = 1_000_000, coarse_solver=None):
     if precond not in ("net", "none"):
       raise ValueError("precond must be 'net' or 'none'")
     self.params = np.asarray(params, np.float32)
     self.coarse, self.cycles, self.sweeps = coarse, cycles, sweeps
-    self.omega, self.stop = omega, stop
-    self.coarse_tol, self.coarse_iters = coarse_tol, coarse_iters
+    self.omega, self.stop, self.net_steps = omega, stop, net_steps
     self.precond, self.lowpass_sweeps = precond, lowpass_sweeps
-    self.coarsest = coarsest  # side of the exact-solve grid (0: none)
+    self.coarse_rtol, self.coarse_max_sweeps = coarse_rtol, coarse_max_sweeps
+    self.coarse_solver = coarse_solver or (
+        lambda: GridSolver(schedule="jacobi"))
     self._nets: dict = {}
 
   def factor(self, h: int, w: int) -> int:
-    f = 1
+    """Power of two >= 4 (the restriction the network is trained on) that
+    brings the larger side to at most about `coarse` cells."""
+    f = 4
     while max(h, w) // (2 * f) >= self.coarse:
       f *= 2
     return f
@@ -247,41 +197,49 @@ class WarmStart:
     x[:hc, :wc, 6] = mc
     return net.forward(x[None])[0, :hc, :wc] * mc[..., None]
 
-  def coarse_solve(self, rc: np.ndarray, mc: np.ndarray,
-                   coarsest: CoarsestSolve | None = None):
-    """Flexible conjugate gradients on A_c c = rc with the network (or, for
-    precond "none", the residual itself) and the coarsest exact solve as
-    search directions; returns (c float32, steps, final relative residual
-    per channel)."""
+  def net_start(self, rc: np.ndarray, mc: np.ndarray) -> np.ndarray:
+    """Learned start of the coarse solve: `net_steps` flexible
+    conjugate-gradient steps over low-passed network directions (the coarse
+    error's energy norm never grows, whatever the network returns)."""
     m = mc[..., None]
     r = np.asarray(rc, np.float64) * m
     c = np.zeros_like(r)
-    bn = np.sqrt(_dot(r, r))
-    rel = np.ones(3)
-    if not (bn > 0).any():
-      return c.astype(np.float32), 0, np.zeros(3)
     dirs = []  # (p, A p, <p, A p>)
-    k = 0
-    for k in range(1, self.coarse_iters + 1):
-      cands = [lowpass(mc, self.predict(r, mc), self.lowpass_sweeps)
-               if self.precond == "net" else r.copy()]
-      if coarsest is not None:
-        cands.append(coarsest.direction(r, mc))
-      for z in cands:
-        z = z * m
-        for p, ap, pap in dirs:
-          z -= (_dot(z, ap) / pap) * p
-        az = coarse_apply(mc, z)
-        pap = _dot(z, az)
-        ok = pap > 1e-30 * np.maximum(_dot(z, z), 1e-300)
-        a = np.where(ok, _dot(r, z) / np.where(ok, pap, 1.0), 0.0)
-        c += a * z
-        r -= a * az
-        dirs.append((z, az, np.where(ok, pap, 1.0)))
-      rel = np.sqrt(_dot(r, r)) / np.where(bn > 0, bn, 1.0)
-      if (rel < self.coarse_tol).all():
-        break
-    return c.astype(np.float32), k, rel
+    for _ in range(self.net_steps):
+      z = lowpass(mc, self.predict(r, mc), self.lowpass_sweeps) * m
+      for p, ap, pap in dirs:
+        z -= (_dot(z, ap) / pap) * p
+      az = coarse_apply(mc, z)
+      pap = _dot(z, az)
+      ok = pap > 1e-30 * np.maximum(_dot(z, z), 1e-300)
+      a = np.where(ok, _dot(r, z) / np.where(ok, pap, 1.0), 0.0)
+      c += a * z
+      r -= a * az
+      dirs.append((z, az, np.where(ok, pap, 1.0)))
+    return c.astype(np.float32)
+
+  def coarse_solve(self, rc: np.ndarray, mc: np.ndarray, cs=None):
+    """Coarse correction: the learned start, then (if `coarse_rtol` is set)
+    the coarse GridSolver `cs` (omega-1/2 sweep and plain Jacobi to
+    `coarse_rtol`).  Returns (c float32 [hc, wc, 3], info)."""
+    hc, wc = mc.shape
+    use_net = self.precond == "net" and self.net_steps > 0
+    c0 = (self.net_start(rc, mc) if use_net
+          else np.zeros((hc, wc, 3), np.float32))
+    if self.coarse_rtol is None:
+      return c0, {"coarse_sweeps": 0}
+    if cs is None:
+      cs = self.coarse_solver()
+    cs.reset(0, mc.astype(np.int32), np.zeros((hc, wc, 3), np.float32),
+             np.ascontiguousarray(rc, np.float32))
+    if use_net:
+      cs.set_state(c0)
+      cs.relax(1, self.omega)
+    r0 = float(np.abs(rc[mc]).mean()) if mc.any() else 0.0
+    n, r = cs.solve_to_residual(self.coarse_rtol * r0, 256,
+                                self.coarse_max_sweeps)
+    return cs.state().astype(np.float32), {
+        "coarse_sweeps": int(n), "coarse_rel_residual": r / max(r0, 1e-300)}
 
   def run(self, solver: GridSolver, mask: np.ndarray, tgt: np.ndarray,
           log=None) -> dict:
@@ -291,11 +249,8 @@ class WarmStart:
     h, w = mask.shape
     f = self.factor(h, w)
     mc = coarse_mask(mask, f)
-    nu = self.sweeps if self.sweeps is not None else 6 * f * f
-    g = 1
-    while self.coarsest and max(mc.shape) // (2 * g) >= self.coarsest:
-      g *= 2
-    low = CoarsestSolve(mc, g) if self.coarsest and g > 1 else None
+    nu = self.sweeps if self.sweeps is not None else 24 * f * f
+    cs = self.coarse_solver() if self.coarse_rtol is not None else None
     r_cold = mean_residual(solver)
 
     def smooth():
@@ -308,28 +263,32 @@ class WarmStart:
     for k in range(self.cycles):
       rc = solver.restrict(f)
       rc[~mc] = 0.0
-      cc, calls, rel = self.coarse_solve(rc, mc, low)
+      cc, info = self.coarse_solve(rc, mc, cs)
+      num, den = solver.prolong_dots(f, cc)
+      alpha = np.where(den > 0, num / np.where(den > 0, den, 1.0), 0.0)
+      alpha = np.maximum(alpha, 0.0)
+      cc = (cc * alpha).astype(np.float32)
       solver.prolong(f, cc)
       smooth()
       r_now = mean_residual(solver)
       size = float(np.abs(cc).max()) if cc.size else 0.0
-      steps.append({"max_corr": size, "coarse_iters": calls,
-                    "coarse_rel_residual": float(rel.max()),
-                    "mean_residual": r_now})
+      steps.append({"max_corr": size, "alpha": [float(a) for a in alpha],
+                    **info, "mean_residual": r_now})
       if log:
-        log(f"#   cycle {k + 1}: max correction {size:.4g}, {calls} coarse "
-            f"iterations (relative residual {rel.max():.2g}), mean "
-            f"residual {r_now:.4g}")
+        log(f"#   cycle {k + 1}: max correction {size:.4g}, "
+            f"{info['coarse_sweeps']} coarse sweeps, mean residual "
+            f"{r_now:.4g}")
       if size < self.stop or not r_now < r_prev:
         break
       r_prev = r_now
+    del cs
     r_warm = mean_residual(solver)
     used = bool(r_warm < r_cold)
     if not used:
       solver.set_state(tgt)
     return {"factor": f, "coarse": list(mc.shape), "sweeps_per_smooth": nu,
-            "coarsest": list(low.mcc.shape) if low is not None else None,
             "precond": self.precond, "cycles": len(steps), "steps": steps,
+            "coarse_sweeps": int(sum(st["coarse_sweeps"] for st in steps)),
             "used": used, "start_residual_cold": r_cold,
             "start_residual_warm": r_warm,
             "seconds": time.perf_counter() - t0,
@@ -410,6 +369,9 @@ def training_set(n: int, size: int = 256, render: int = 1024,
       del s
       M[i], R[i] = mc, rc
       i += 1
+      if (i - n_rand) % 2048 == 0:
+        log(f"inputs: {i - n_rand}/{n - n_rand} composites "
+            f"({time.perf_counter() - t0:.1f} s)")
   log(f"inputs: {n} x {size}^2 ({n - n_rand} smoothed composites drawn at "
       f"{render}^2, {n_rand} random) in {time.perf_counter() - t0:.1f} s")
   X = np.zeros(R.shape, np.float16)
@@ -494,8 +456,9 @@ def exact_solution(mask, tgt, grad, x0, log=print) -> np.ndarray:
 def evaluate(ws: WarmStart, size: int, seed: int, tol: float = 1e-3,
              check_every: int = 512, max_sweeps: int = 8_000_000,
              log=print) -> dict:
-  """Time to the plain method's final residual, cold vs warm (and warm
-  without the network), on one unseen composite; accuracy against converged
+  """Time to the plain method's final residual, cold vs warm (and, for
+  comparison, the same cycles without the network and a classical two-grid
+  cycle), on one unseen composite; accuracy against converged
   plain Jacobi and against the exact float64 solution.  "Same accuracy"
   means a warm max error vs the exact solution no larger than the cold
   one's: float32 Jacobi stops at one of a family of fixed points whose
@@ -528,12 +491,16 @@ def evaluate(ws: WarmStart, size: int, seed: int, tol: float = 1e-3,
     return time.perf_counter() - t, n, r, info, s.state()
 
   t_warm, n_warm, r_warm, info, x_warm = warm(ws)
-  # the same method without the network (plain conjugate gradients on the
-  # coarse grid, same iteration budget)
-  cg = WarmStart(ws.params, ws.coarse, ws.cycles, ws.sweeps, ws.omega,
-                 ws.stop, ws.coarse_tol, ws.coarse_iters, "none",
-                 ws.lowpass_sweeps, ws.coarsest)
-  t_cg, n_cg, r_cg, info_cg, x_cg = warm(cg)
+  # the same cycles without the network
+  nn = WarmStart(ws.params, ws.coarse, ws.cycles, ws.sweeps, ws.omega,
+                 ws.stop, ws.net_steps, "none", ws.lowpass_sweeps,
+                 ws.coarse_rtol, ws.coarse_max_sweeps)
+  t_nn, n_nn, r_nn, info_nn, x_nn = warm(nn)
+  # a classical two-grid cycle, no network: coarse Jacobi from zero to 1 %
+  cl = WarmStart(ws.params, ws.coarse, ws.cycles, ws.sweeps, ws.omega,
+                 ws.stop, ws.net_steps, "none", ws.lowpass_sweeps, 1e-2,
+                 ws.coarse_max_sweeps)
+  t_cl, n_cl, r_cl, info_cl, x_cl = warm(cl)
 
   x_star = exact_solution(mask, tgt, grad, x_ref, log=log)
 
@@ -556,12 +523,17 @@ def evaluate(ws: WarmStart, size: int, seed: int, tol: float = 1e-3,
       "warm": {"seconds": t_warm, "warm_start_seconds": info["seconds"],
                "jacobi_sweeps": n_warm, "residual": r_warm, **err(x_warm),
                "warm_start": info},
-      "without_network": {"seconds": t_cg,
-                          "warm_start_seconds": info_cg["seconds"],
-                          "jacobi_sweeps": n_cg, "residual": r_cg,
-                          **err(x_cg), "warm_start": info_cg},
+      "without_network": {"seconds": t_nn,
+                          "warm_start_seconds": info_nn["seconds"],
+                          "jacobi_sweeps": n_nn, "residual": r_nn,
+                          **err(x_nn), "warm_start": info_nn},
+      "classical_two_grid": {"seconds": t_cl,
+                             "warm_start_seconds": info_cl["seconds"],
+                             "jacobi_sweeps": n_cl, "residual": r_cl,
+                             **err(x_cl), "warm_start": info_cl},
       "speedup": t_cold / max(t_warm, 1e-9),
-      "speedup_without_network": t_cold / max(t_cg, 1e-9),
+      "speedup_without_network": t_cold / max(t_nn, 1e-9),
+      "speedup_classical_two_grid": t_cold / max(t_cl, 1e-9),
   }
   rec["same_accuracy"] = bool(
       rec["warm"]["max_abs_err_vs_exact"] <=
@@ -585,11 +557,14 @@ def main(argv=None) -> None:
   ap.add_argument("--seeds", default="900001")
   ap.add_argument("--tol", type=float, default=1e-3)
   ap.add_argument("--coarse", type=int, default=256)
-  ap.add_argument("--cycles", type=int, default=16)
+  ap.add_argument("--cycles", type=int, default=12)
   ap.add_argument("--stop", type=float, default=0.005)
-  ap.add_argument("--coarsest", type=int, default=64)
-  ap.add_argument("--coarse-tol", type=float, default=1e-3)
-  ap.add_argument("--coarse-iters", type=int, default=2)
+  ap.add_argument("--net-steps", type=int, default=2)
+  ap.add_argument("--smooth", type=int, default=0,
+                  help="fine sweeps per smoothing (default 24 f^2)")
+  ap.add_argument("--coarse-rtol", type=float, default=0.0,
+                  help="polish each coarse correction by Jacobi to this "
+                       "relative residual (0: no polish)")
   ap.add_argument("--save", default="")
   ap.add_argument("--load", default="")
   a = ap.parse_args(argv)
@@ -610,8 +585,8 @@ def main(argv=None) -> None:
     if a.save:
       np.savez_compressed(a.save, params=params.astype(np.float16))
   ws = WarmStart(params, coarse=a.coarse, cycles=a.cycles, stop=a.stop,
-                 coarse_tol=a.coarse_tol, coarse_iters=a.coarse_iters,
-                 coarsest=a.coarsest)
+                 net_steps=a.net_steps, coarse_rtol=a.coarse_rtol or None,
+                 sweeps=a.smooth or None)
   evals = [evaluate(ws, int(sz), int(sd), a.tol, log=log)
            for sz in a.sizes.split(",") for sd in a.seeds.split(",")]
   summary = {
@@ -627,6 +602,10 @@ def main(argv=None) -> None:
           for sz in a.sizes.split(",")},
       "speedup_without_network_by_size": {
           sz: float(np.mean([e["speedup_without_network"] for e in evals
+                             if e["size"] == int(sz)]))
+          for sz in a.sizes.split(",")},
+      "speedup_classical_two_grid_by_size": {
+          sz: float(np.mean([e["speedup_classical_two_grid"] for e in evals
                              if e["size"] == int(sz)]))
           for sz in a.sizes.split(",")},
       "same_accuracy": all(e["same_accuracy"] for e in evals),

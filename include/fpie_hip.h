@@ -153,6 +153,13 @@ int pie_restrict(pie_solver* s, int factor, float* out);
 /* state += cell-centred bilinear upsampling of corr float[hc][wc][3] at the
  * solved pixels; hc, wc must be the coarse grid of `factor`. */
 int pie_prolong(pie_solver* s, int factor, int hc, int wc, const float* corr);
+/* Line search for pie_prolong: with d the correction pie_prolong would add
+ * (0 off the solved pixels), out[c] = sum r . d and out[3 + c] = sum d . A d
+ * over the solved pixels, per channel c, in double (r the exact residual of
+ * the state, (A d)_p = 4 d_p - sum of the neighbours' d); scaling corr by
+ * out[c] / out[3 + c] minimises the energy norm of the new error. */
+int pie_prolong_dots(pie_solver* s, int factor, int hc, int wc,
+                     const float* corr, double* out);
 /* `sweeps` weighted Jacobi sweeps x <- x + omega * ((g + sum) / 4 - x),
  * 0 < omega <= 1 (omega 1/2: the mean of two consecutive Jacobi iterates). */
 int pie_relax(pie_solver* s, int sweeps, float omega);

@@ -79,6 +79,25 @@ def prolong(mask, x, corr, f: int) -> np.ndarray:
   return np.where(m[..., None], x + v, x).astype(np.float32)
 
 
+def prolong_dots(mask, x, grad, corr, f: int) -> np.ndarray:
+  """[2, 3] float64: per channel sum r . d and sum d . A d over the solved
+  pixels, d the float32 correction prolong() adds (0 off the mask), r the
+  exact residual of x, (A d)_p = 4 d_p - sum of the neighbours' d."""
+  m = np.asarray(mask) != 0
+  d = prolong(mask, np.zeros(m.shape + (3,), np.float32), corr, f)
+  d = d.astype(np.float64) * m[..., None]
+  z = np.pad(d, ((1, 1), (1, 1), (0, 0)))
+  ad = 4.0 * d - (z[:-2, 1:-1] + z[2:, 1:-1] + z[1:-1, :-2] + z[1:-1, 2:])
+  h, w = m.shape
+  x64 = np.asarray(x, np.float32).astype(np.float64)
+  g64 = np.asarray(grad, np.float32).astype(np.float64)
+  r = np.zeros_like(x64)
+  r[1:-1, 1:-1] = (g64[1:-1, 1:-1] + x64[:-2, 1:-1] + x64[1:-1, :-2] +
+                   x64[1:-1, 2:] + x64[2:, 1:-1] - 4.0 * x64[1:-1, 1:-1])
+  mm = m[..., None]
+  return np.stack([(r * d * mm).sum((0, 1)), (d * ad * mm).sum((0, 1))])
+
+
 def relax(mask, x, grad, sweeps: int, omega: float,
           order: str = "ulrd") -> np.ndarray:
   """Weighted Jacobi in the kernel's float32 order: j = s * 0.25;

@@ -15,6 +15,9 @@ block index) fails it:
   is reproduced exactly at the fine pixel centres between coarse centres;
 * weighted Jacobi with omega 1 is the reference's Jacobi sweep bit for bit,
   omega 1/2 the mean of two consecutive sweeps;
+* the line-search sums: for the exact error as the correction (factor 1),
+  r . d equals d . A d (r = A e), and the energy-optimal multiple of a
+  random correction never raises the error's energy norm;
 * the two-grid cycle built from these pieces (exact coarse solve) cuts the
   error of a composite by a factor of 2.5 or more per cycle after the first.
 """
@@ -125,6 +128,29 @@ def test_relax_weights(oracle):
   half = co.relax(mask, x, grad, 1, 0.5)
   np.testing.assert_allclose(half, 0.5 * (x.astype(np.float64) + ref),
                              rtol=2e-7, atol=1e-5)
+
+
+def test_prolong_dots_line_search():
+  mask, tgt, grad, _ = _problem(40, 44, 13)
+  m = mask != 0
+  xs = co.exact_solution(mask, tgt, grad)
+  x = tgt.astype(np.float32)
+  e = (xs - x).astype(np.float32) * m[..., None]
+  rd, dad = co.prolong_dots(mask, x, grad, e, 1)
+  np.testing.assert_allclose(rd, dad, rtol=1e-5)
+
+  def energy(v):  # e^T A e from the exact solution, per channel
+    err = (xs - v) * m[..., None]
+    return co.prolong_dots(mask, np.zeros_like(x), np.zeros_like(grad),
+                           err.astype(np.float32), 1)[1]
+
+  rng = np.random.default_rng(4)
+  for f in (1, 2, 4):
+    corr = rng.normal(0, 20, (40 // f, 44 // f, 3)).astype(np.float32)
+    rd, dad = co.prolong_dots(mask, x, grad, corr, f)
+    a = rd / dad
+    new = co.prolong(mask, x, (corr * a).astype(np.float32), f)
+    assert (energy(new) <= energy(x) + 1e-6).all(), f
 
 
 def _two_grid_errors(mask, tgt, grad, f, cycles, sweeps):

@@ -56,6 +56,19 @@ def test_prolong_matches_oracle(hip, h, w, f):
   assert np.array_equal(_bits(s.state()), _bits(co.prolong(mask, x, corr, f)))
 
 
+@pytest.mark.parametrize("h,w,f", [(64, 80, 4), (67, 91, 3), (300, 257, 16),
+                                   (64, 80, 1)])
+def test_prolong_dots_match_oracle(hip, h, w, f):
+  print(f"prolong_dots {h}x{w} factor {f}", flush=True)
+  s, mask, _, grad, x = _loaded(hip, h, w, 300 + f)
+  corr = np.random.default_rng(f).normal(0, 30, (h // f, w // f, 3))
+  corr = corr.astype(np.float32)
+  got = s.prolong_dots(f, corr)
+  want = co.prolong_dots(mask, x, grad, corr, f)
+  np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-9)
+  assert np.array_equal(_bits(s.state()), _bits(x))  # state untouched
+
+
 @pytest.mark.parametrize("omega,sweeps", [(0.5, 3), (1.0, 2), (0.8, 1)])
 def test_relax_matches_oracle(hip, omega, sweeps):
   print(f"relax omega {omega} sweeps {sweeps}", flush=True)
