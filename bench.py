@@ -305,7 +305,8 @@ def main():
       "config": {"workload": w.name, "method": w.method,
                  "image": list(w.src.shape), "solved_pixels": masked,
                  "gradient": w.gradient, "sweeps_per_step": iters,
-                 "schedule": a.schedule, "order": "ulrd",
+                 "schedule": a.schedule,
+                 "order": "ulrd" if w.method == "grid" else "udlr",
                  "tile": {"sweeps_per_launch": tile[0],
                           "cols_per_lane": tile[1], "rows_per_task": tile[2]},
                  "parallelism": "single GPU"},

@@ -6,9 +6,11 @@
 // Network (no counterpart in the reference): 4 resolution levels, base 32
 // channels, 3x3 convolutions with bias and ReLU, 2x2 average pooling,
 // nearest 2x upsampling, skip concatenation (upsampled channels first), a
-// 1x1 head to 3 channels.  Input: 8 channels (3 divergence / 64,
-// 3 (target - 128) / 64, mask, one zero channel); output: the correction
-// (x - target) / 64 on the solved pixels.
+// 1x1 head to 3 channels.  Input: 8 channels (the warm start feeds the
+// rms-normalised coarse residual of the correction problem in channels
+// 0-2, zeros in 3-5, the mask in 6 and a zero channel); output: 3 channels
+// (the shape of the coarse correction), trained with a masked mean squared
+// error.
 //
 // Parameters are one fp32 array, layer by layer: conv w[tap][cin][cout]
 // then b[cout]; the head w[32][3] then b[3].

@@ -6,8 +6,8 @@ counterpart in the reference; its definition is the one in
 fast-poisson-image-editing_amd/fpie_amd/unet.py (restated here, not
 imported):
 
-  input 7 channels (3 divergence / 64, 3 (target - 128) / 64, 1 mask),
-  channels padded with zeros to 8;
+  input 8 channels (the warm start's features: normalised coarse
+  residual, zeros, mask, zero);
   encoder  e0a 8->32, e0b 32->32 | pool | e1a 32->64, e1b 64->64 | pool |
            e2a 64->128, e2b 128->128 | pool | bna 128->256, bnb 256->256
   decoder  up(bnb) ++ e2b -> d2a 384->128, d2b 128->128 | up ++ e1b ->
