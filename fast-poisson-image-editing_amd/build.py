@@ -31,11 +31,14 @@ FLAGS = [
 
 def tiles() -> list[tuple[int, int]]:
   text = open(os.path.join(CSRC, "kernels.h")).read()
-  line = re.search(r"#define PIE_TILES\(X\)(.*)", text).group(1)
+  line = re.search(r"#define PIE_TILES\(X\)((?:.*\\\n)*.*)", text).group(1)
   return [(int(a), int(b)) for a, b in re.findall(r"X\((\d+), (\d+)\)", line)]
 
 
 def _compile(src: str, obj: str, defines: list[str]) -> str:
+  # PIE_DEFINES: extra -D flags for tuning builds (e.g. the sweep kernel's
+  # PIE_PREFETCH / PIE_WAVES_PER_EU / PIE_GQ_LDS knobs in csrc/march.h)
+  defines = defines + os.environ.get("PIE_DEFINES", "").split()
   cmd = [HIPCC, *FLAGS, *defines, "-c", src, "-o", obj]
   r = subprocess.run(cmd, capture_output=True, text=True)
   if r.returncode != 0:

@@ -34,6 +34,7 @@ struct ItemBuffers {
   DevBuf<uint8_t> m;
   int H = 0, W = 0, P = 0;
   long long plane = 0, masked = 0;
+  long long bufsz = 0, mbytes = 0;
   int max_sweeps = 0;
   float tol = 0.0f;
 };
@@ -97,7 +98,10 @@ int prepare(pie_batch* b) {
     d.t_end = it.max_sweeps;
     d.max_sweeps = it.max_sweeps;
     d.tol = it.tol;
-    task_item.insert(task_item.end(), (size_t)d.stripes * d.segs, i);
+    task_item.insert(task_item.end(),
+                     (size_t)d.stripes * d.segs *
+                         pie::tile_groups(b->cfg.S, b->cfg.C),
+                     i);
   }
   b->ntask = (int)task_item.size();
   PIE_HIP(b->d_items.alloc(std::max(n, 1)));
@@ -118,7 +122,7 @@ int prepare(pie_batch* b) {
 int restart(pie_batch* b) {
   for (size_t i = 0; i < b->items.size(); ++i) {
     ItemBuffers& it = *b->items[i];
-    PIE_HIP(hipMemcpyAsync(it.xa.p, it.x0.p, 3 * it.plane * sizeof(float),
+    PIE_HIP(hipMemcpyAsync(it.xa.p, it.x0.p, it.bufsz * sizeof(float),
                            hipMemcpyDeviceToDevice, b->stream));
     BatchItem& d = b->desc[i];
     d.launches = d.sweeps = d.done = 0;
@@ -221,21 +225,24 @@ int pie_batch_add(pie_batch* b, int h, int w, const int32_t* mask,
   std::unique_ptr<ItemBuffers> it(new ItemBuffers());
   it->H = h;
   it->W = w;
-  it->P = pitch_of(w);
-  it->plane = (long long)h * it->P;
-  if (it->plane > (1LL << 31) - 1)
-    return fail(PIE_EINVAL, "grid larger than 2^31 elements per plane");
+  const pie::host::GridLayout L = pie::host::grid_layout(h, w);
+  if ((long long)h * L.P > (1LL << 31) - 1)
+    return fail(PIE_EINVAL, "grid larger than 2^31 elements");
+  it->P = L.P;
+  it->plane = L.plane;
+  it->bufsz = L.floats;
+  it->mbytes = L.mask;
   it->max_sweeps = max_sweeps;
   it->tol = tol;
-  const size_t fl = 3 * (size_t)it->plane;
+  const size_t fl = (size_t)it->bufsz;
   PIE_HIP(it->xa.alloc(fl));
   PIE_HIP(it->xb.alloc(fl));
   PIE_HIP(it->g.alloc(fl));
   PIE_HIP(it->x0.alloc(fl));
-  PIE_HIP(it->m.alloc(it->plane));
+  PIE_HIP(it->m.alloc(it->mbytes));
   PIE_HIP(hipMemsetAsync(it->x0.p, 0, fl * sizeof(float), b->stream));
   PIE_HIP(hipMemsetAsync(it->g.p, 0, fl * sizeof(float), b->stream));
-  PIE_HIP(hipMemsetAsync(it->m.p, 0, it->plane, b->stream));
+  PIE_HIP(hipMemsetAsync(it->m.p, 0, it->mbytes, b->stream));
   const size_t px = (size_t)h * w;
   DevBuf<char> stage;
   PIE_HIP(stage.alloc(px * 12));

@@ -61,6 +61,29 @@ inline int pitch_of(int W) {
   return round_up(kPadLeft + W + kPadRight, 64);
 }
 
+// Layout of a solve's three float channels and mask on an H x W grid.
+// Row-interleaved (default): row i holds channel 0, 1, 2 segments of
+// pitch_of(W) floats each, so the channel stride ("plane") is one segment
+// and the row pitch P three; the loads of one row of all channels then fall
+// in one stretch of memory.  Planar: channel planes of H x pitch_of(W).
+// The mask uses the same row pitch (only its first segment is used).
+#ifndef PIE_ROW_INTERLEAVED
+#define PIE_ROW_INTERLEAVED 1
+#endif
+struct GridLayout {
+  int P;             // row pitch (elements)
+  long long plane;   // channel stride (elements)
+  long long floats;  // elements of one three-channel buffer
+  long long mask;    // bytes of the mask buffer
+};
+
+inline GridLayout grid_layout(int H, int W) {
+  const int p = pitch_of(W);
+  if (PIE_ROW_INTERLEAVED)
+    return {3 * p, p, 3LL * H * p, 3LL * H * p};
+  return {p, (long long)H * p, 3LL * H * p, (long long)H * p};
+}
+
 inline int upload(const void* host, size_t bytes, void* dev, hipStream_t st) {
   PIE_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, st));
   return PIE_OK;

@@ -1,15 +1,17 @@
 // Device kernels of the Poisson solver and their host-side launchers.
 //
 // Layout in HBM: a solve works on a pitched grid of H rows x W columns.
-// Each row holds kPadLeft padding elements, the W values and at least
-// kPadRight more padding (pitch P, a multiple of 64); the base pointers the
-// kernels get point at column 0 of row 0, so element (i, j) is base[i*P+j]
-// for j in [-kPadLeft, P - kPadLeft).  The float32 RGB state is kept as
-// three planes (channel c at offset c * plane, plane = H * P), the guidance
-// (grad for the GridSolver, b for the EquSolver) likewise, and the mask as
-// one uint8 plane.  Planar float rows make every row load of a wavefront one
-// contiguous 256-byte (or 512-byte) transaction, and the padding lets the
-// sweep kernel load whole stripes without bounds tests.
+// Every channel row is a segment of kPadLeft padding elements, the W values
+// and at least kPadRight more padding; the base pointers the kernels get
+// point at column 0 of channel 0 of row 0, so element (i, j) of channel c is
+// base[c * plane + i * P + j] for j in [-kPadLeft, segment - kPadLeft).
+// The host picks the layout (host_util.h: grid_layout): row-interleaved
+// (plane = one segment, P = three segments) or planar (plane = H
+// segments, P = one segment).  The float32 RGB state, the guidance (grad
+// for the GridSolver, b for the EquSolver) and the uint8 mask (row pitch P,
+// first segment) all use it.  Channel rows make every row load of a
+// wavefront one contiguous 256-byte (or 512-byte) transaction, and the
+// padding lets the sweep kernel load whole stripes without bounds tests.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -76,7 +78,13 @@ struct TileCfg {
 
 // Compiled (sweeps, columns per lane) variants of the sweep kernel; each is
 // built from march_inst.hip with -DPIE_S=s -DPIE_C=c.
-#define PIE_TILES(X) X(4, 1) X(8, 1) X(4, 2) X(8, 2)
+// Tiles of two columns per lane keep twice as many values per level in
+// registers, so their wave tasks carry one colour channel each (three task
+// groups per tile; the channels are independent problems); one-column
+// tiles carry all three channels.
+#define PIE_TILES(X) X(4, 1) X(6, 1) X(8, 1) X(4, 2) X(6, 2) X(8, 2) X(12, 2)
+constexpr int tile_channels(int S, int C) { return C >= 2 ? 1 : 3; }
+constexpr int tile_groups(int S, int C) { return 3 / tile_channels(S, C); }
 #define PIE_DECL(s, c)                                             \
   hipError_t launch_march_##s##_##c(int sched, int order, const MarchArgs& a, \
                                     hipStream_t st);                         \
@@ -117,9 +125,10 @@ hipError_t launch_interleave(const float* src, float* dst, long long plane,
 // counts[1] += solved pixels on the outermost rows/columns.
 hipError_t launch_mask(const int32_t* src, uint8_t* dst, int H, int W, int P,
                        unsigned long long* counts, hipStream_t st);
-// fill three planes (H x P each) with a per-channel constant
-hipError_t launch_fill3(float* dst, long long plane, float v0, float v1,
-                        float v2, hipStream_t st);
+// fill the n elements of a three-channel buffer (row pitch P, channel
+// stride plane) with a per-channel constant
+hipError_t launch_fill3(float* dst, long long n, int P, long long plane,
+                        float v0, float v1, float v2, hipStream_t st);
 // Equ system on the grid layout: X[id], B[id] (float[N][3]) to the pixel
 // lin[id] = i * P + j of each id >= 1.
 hipError_t launch_equ_scatter(const float* X, const float* B,

@@ -65,13 +65,13 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 __global__ void __launch_bounds__(kBlock)
-    fill3_kernel(float* __restrict__ dst, long long plane, float v0, float v1,
-                 float v2) {
-  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < plane;
+    fill3_kernel(float* __restrict__ dst, long long n, int P, long long plane,
+                 float v0, float v1, float v2) {
+  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < n;
        id += (long long)gridDim.x * kBlock) {
-    dst[id] = v0;
-    dst[plane + id] = v1;
-    dst[2 * plane + id] = v2;
+    // channel of element id: planar (plane >= P) or row-interleaved layout
+    const long long c = plane >= P ? id / plane : (id % P) / plane;
+    dst[id] = c == 0 ? v0 : (c == 1 ? v1 : v2);
   }
 }
 
@@ -532,7 +532,7 @@ hipError_t launch_march(const TileCfg& cfg, int sched, int order, MarchArgs a,
                         hipStream_t st) {
   tile_tasks(cfg, a.H, a.W, &a.stripes, &a.segs);
   a.seg_rows = cfg.seg_rows;
-  a.ntask = a.stripes * a.segs;
+  a.ntask = a.stripes * a.segs * tile_groups(cfg.S, cfg.C);
 #define PIE_T(s, c) \
   if (cfg.S == s && cfg.C == c) return launch_march_##s##_##c(sched, order, a, st);
   PIE_TILES(PIE_T)
@@ -592,10 +592,10 @@ hipError_t launch_mask(const int32_t* src, uint8_t* dst, int H, int W, int P,
   return hipGetLastError();
 }
 
-hipError_t launch_fill3(float* dst, long long plane, float v0, float v1,
-                        float v2, hipStream_t st) {
-  hipLaunchKernelGGL(fill3_kernel, dim3(grid_blocks(plane, 8192)), dim3(kBlock),
-                     0, st, dst, plane, v0, v1, v2);
+hipError_t launch_fill3(float* dst, long long n, int P, long long plane,
+                        float v0, float v1, float v2, hipStream_t st) {
+  hipLaunchKernelGGL(fill3_kernel, dim3(grid_blocks(n, 8192)), dim3(kBlock),
+                     0, st, dst, n, P, plane, v0, v1, v2);
   return hipGetLastError();
 }
 

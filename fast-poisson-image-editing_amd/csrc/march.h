@@ -79,13 +79,13 @@ __device__ __forceinline__ int xcd_block() {
 // col0 in [-S, W + 64C) always stays inside the row: loads need no per-lane
 // bounds test, and a prefetched value is not forced through a branch.
 // Padding and clamped rows only ever feed halo lanes or unsolved pixels.
-template <int C>
+template <int C, int NCH>
 __device__ __forceinline__ void load_row(const float* __restrict__ base,
                                          long long plane, int P, int row,
-                                         int col0, float (&v)[C][3]) {
+                                         int col0, float (&v)[C][NCH]) {
   const float* p = base + (long long)row * P + col0;
 #pragma unroll
-  for (int ch = 0; ch < 3; ++ch) {
+  for (int ch = 0; ch < NCH; ++ch) {
     if constexpr (C == 2) {
       const float2 q = *reinterpret_cast<const float2*>(p + ch * plane);
       v[0][ch] = q.x;
@@ -120,13 +120,13 @@ __device__ __forceinline__ uint32_t load_mask(const uint8_t* __restrict__ m,
 
 // Stores may land in the right padding (col >= W): those values are finite
 // and only ever read by halo lanes.
-template <int C>
+template <int C, int NCH>
 __device__ __forceinline__ void store_row(float* __restrict__ base,
                                           long long plane, int P, int row,
-                                          int col0, const float (&v)[C][3]) {
+                                          int col0, const float (&v)[C][NCH]) {
   float* p = base + (long long)row * P + col0;
 #pragma unroll
-  for (int ch = 0; ch < 3; ++ch) {
+  for (int ch = 0; ch < NCH; ++ch) {
     if constexpr (C == 2) {
       *reinterpret_cast<float2*>(p + ch * plane) =
           make_float2(v[0][ch], v[1][ch]);
@@ -141,8 +141,19 @@ __device__ __forceinline__ void store_row(float* __restrict__ base,
 
 // Row steps of look-ahead of the sweep kernel's loads (divides S, so the
 // ring slot of every unrolled step is a compile-time constant).
+#ifndef PIE_PREFETCH
 template <int S>
 constexpr int kPrefetch = S % 4 == 0 ? 4 : 2;
+#else
+template <int S>
+constexpr int kPrefetch = PIE_PREFETCH;
+#endif
+
+// Guidance ring of the last S rows: registers (0) or LDS (1, each wave a
+// private region, so no barrier is needed and fewer VGPRs are live).
+#ifndef PIE_GQ_LDS
+#define PIE_GQ_LDS 0
+#endif
 
 template <bool B, class T>
 __device__ __forceinline__ T* pick(T* x, T* y) {
@@ -157,9 +168,16 @@ __device__ __forceinline__ T* pick(T* x, T* y) {
 // the task writes is merged into *dmax (float bits, non-negative).
 template <int S, int C, int SCHED, int ORDER>
 __device__ __forceinline__ void march_task(const MarchArgs& a, int stripe,
-                                           int seg, unsigned* dmax) {
+                                           int seg, int group,
+                                           unsigned* dmax) {
   static_assert(S % 2 == 0 && S % C == 0, "S even and a multiple of C");
   static_assert(S * C <= 32, "mask queue is one 32-bit word");
+  // channels of this task: all three, or (deep tiles) channel `group` only
+  constexpr int NCH = tile_channels(S, C);
+  const long long choff = (long long)group * NCH * a.plane;
+  const float* __restrict__ xin = a.xin + choff;
+  float* __restrict__ xout = a.xout + choff;
+  const float* __restrict__ gin = a.g + choff;
   constexpr int U = kWave * C - 2 * S;  // written columns per stripe
   const int lane = threadIdx.x & (kWave - 1);
   const int col0 = stripe * U - S + lane * C;
@@ -174,9 +192,9 @@ __device__ __forceinline__ void march_task(const MarchArgs& a, int stripe,
     const int c_lo = stripe * U, c_hi = min(c_lo + U, a.W) - 1;
     if (r0 + c_lo > t_last || (r1 - 1) + c_hi <= a.t0 - a.band) {
       for (int r = r0; r < r1; ++r) {
-        float v[C][3];
-        load_row<C>(a.xin, a.plane, a.P, r, col0, v);
-        if (lane_out) store_row<C>(a.xout, a.plane, a.P, r, col0, v);
+        float v[C][NCH];
+        load_row<C, NCH>(xin, a.plane, a.P, r, col0, v);
+        if (lane_out) store_row<C, NCH>(xout, a.plane, a.P, r, col0, v);
       }
       return;
     }
@@ -189,27 +207,38 @@ __device__ __forceinline__ void march_task(const MarchArgs& a, int stripe,
   // Level j (0..S-1) keeps two rows: wa/wb alternate as the older and the
   // newer one, so a step renames instead of moving.  gq/mq hold the
   // guidance and mask of the last S rows (slot = row step mod S).
-  float wa[S][C][3], wb[S][C][3], gq[S][C][3];
+  float wa[S][C][NCH], wb[S][C][NCH];
+#if PIE_GQ_LDS
+  __shared__ float gl_all[kBlockWaves][S][NCH][kWave * C];
+  float(&gl)[S][NCH][kWave * C] =
+      gl_all[(threadIdx.x / kWave) % kBlockWaves];
+#define PIE_GQ(slot, cc, ch) gl[slot][ch][lane * C + (cc)]
+#else
+  float gq[S][C][NCH];
+#define PIE_GQ(slot, cc, ch) gq[slot][cc][ch]
+#endif
   uint32_t mq = 0;
 #pragma unroll
   for (int j = 0; j < S; ++j)
 #pragma unroll
     for (int cc = 0; cc < C; ++cc)
 #pragma unroll
-      for (int ch = 0; ch < 3; ++ch)
-        wa[j][cc][ch] = wb[j][cc][ch] = gq[j][cc][ch] = 0.0f;
+      for (int ch = 0; ch < NCH; ++ch) {
+        wa[j][cc][ch] = wb[j][cc][ch] = 0.0f;
+        PIE_GQ(j, cc, ch) = 0.0f;
+      }
 
   // Rows are loaded PF row steps before they are used (a register ring of
   // PF slots), so each wave keeps PF rows of loads in flight instead of
   // waiting for every row it consumes.
   constexpr int PF = kPrefetch<S>;
-  float px[PF][C][3], pg[PF][C][3];
+  float px[PF][C][NCH], pg[PF][C][NCH];
   uint32_t pm[PF];
   static_for<0, PF>([&](auto pc) {
     constexpr int q = decltype(pc)::value;
     const int row = min(rb + q, a.H - 1);
-    load_row<C>(a.xin, a.plane, a.P, row, col0, px[q]);
-    load_row<C>(a.g, a.plane, a.P, row, col0, pg[q]);
+    load_row<C, NCH>(xin, a.plane, a.P, row, col0, px[q]);
+    load_row<C, NCH>(gin, a.plane, a.P, row, col0, pg[q]);
     pm[q] = rb + q < a.H ? load_mask<C>(a.m, a.P, row, col0) : 0u;
   });
 
@@ -218,19 +247,19 @@ __device__ __forceinline__ void march_task(const MarchArgs& a, int stripe,
       constexpr int u = decltype(uc)::value;
       constexpr int q = u % PF;
       const int rho = rb + base + u;
-      float fr[C][3], gn[C][3];
+      float fr[C][NCH], gn[C][NCH];
 #pragma unroll
       for (int cc = 0; cc < C; ++cc)
 #pragma unroll
-        for (int ch = 0; ch < 3; ++ch) {
+        for (int ch = 0; ch < NCH; ++ch) {
           fr[cc][ch] = px[q][cc][ch];
           gn[cc][ch] = pg[q][cc][ch];
         }
       const uint32_t mn = pm[q];
       {
         const int row = min(rho + PF, a.H - 1);  // wave-uniform
-        load_row<C>(a.xin, a.plane, a.P, row, col0, px[q]);
-        load_row<C>(a.g, a.plane, a.P, row, col0, pg[q]);
+        load_row<C, NCH>(xin, a.plane, a.P, row, col0, px[q]);
+        load_row<C, NCH>(gin, a.plane, a.P, row, col0, pg[q]);
         pm[q] = rho + PF < a.H ? load_mask<C>(a.m, a.P, row, col0) : 0u;
       }
 
@@ -238,54 +267,89 @@ __device__ __forceinline__ void march_task(const MarchArgs& a, int stripe,
         constexpr int k = decltype(kc)::value;
         constexpr int j = k - 1;
         constexpr int slot = ((u - k) % S + S) % S;
-        float(&older)[C][3] = *pick<(u & 1) != 0>(&wa[j], &wb[j]);
-        float(&newer)[C][3] = *pick<(u & 1) != 0>(&wb[j], &wa[j]);
+        float(&older)[C][NCH] = *pick<(u & 1) != 0>(&wa[j], &wb[j]);
+        float(&newer)[C][NCH] = *pick<(u & 1) != 0>(&wb[j], &wa[j]);
         const int r = rho - k;
         const int t = a.t0 + j;
         // Steps past t_end keep their value; folding that (wave-uniform)
         // test into the predicate keeps the level free of branches, so
         // the window registers are renamed rather than copied.
         const bool live = t < a.t_end;
-        float out[C][3];
+        float out[C][NCH];
         const uint32_t mrow = mq >> (slot * C);
+        bool upd[C];
 #pragma unroll
         for (int cc = 0; cc < C; ++cc) {
           const int col = col0 + cc;
-          bool upd = live && ((mrow >> cc) & 1u);
+          upd[cc] = live && ((mrow >> cc) & 1u);
           if constexpr (SCHED == SCHED_REDBLACK) {
-            upd = upd && ((r + col + t) & 1);
+            upd[cc] = upd[cc] && ((r + col + t) & 1);
           } else if constexpr (SCHED == SCHED_GS_LEX) {
             const int d = t - r - col;
-            upd = upd && d >= 0 && d < a.band && !(d & 1);
+            upd[cc] = upd[cc] && d >= 0 && d < a.band && !(d & 1);
           }
+        }
+        // The C x NCH sums are independent chains; adding them in lockstep
+        // (one term of every chain, then the next term) keeps other work
+        // between an accumulator's write and the lane-shifted (DPP) add
+        // that reads it, which otherwise stalls on the DPP read hazard.
+        // Each chain still adds its own terms in the reference's order.
+        float acc[C][NCH];
+        auto left = [&](int cc, int ch) {
+          return cc > 0 ? newer[cc > 0 ? cc - 1 : 0][ch]
+                        : from_left(newer[C - 1][ch]);
+        };
+        auto right = [&](int cc, int ch) {
+          return cc < C - 1 ? newer[cc < C - 1 ? cc + 1 : 0][ch]
+                            : from_right(newer[0][ch]);
+        };
 #pragma unroll
-          for (int ch = 0; ch < 3; ++ch) {
-            const float ctr = newer[cc][ch];
-            const float up = older[cc][ch];
-            const float dn = fr[cc][ch];
-            const float lf = cc > 0 ? newer[cc > 0 ? cc - 1 : 0][ch]
-                                    : from_left(newer[C - 1][ch]);
-            const float rt = cc < C - 1 ? newer[cc < C - 1 ? cc + 1 : 0][ch]
-                                        : from_right(newer[0][ch]);
-            float s = gq[slot][cc][ch];
-            if constexpr (ORDER == ORDER_UDLR) {
-              s = s + up;
-              s = s + dn;
-              s = s + lf;
-              s = s + rt;
-            } else {
-              s = s + up;
-              s = s + lf;
-              s = s + rt;
-              s = s + dn;
-            }
-            out[cc][ch] = upd ? s * 0.25f : ctr;
-          }
+        for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+          for (int ch = 0; ch < NCH; ++ch)
+            acc[cc][ch] = PIE_GQ(slot, cc, ch) + older[cc][ch];
+        if constexpr (ORDER == ORDER_UDLR) {
+#pragma unroll
+          for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch)
+              acc[cc][ch] = acc[cc][ch] + fr[cc][ch];
+#pragma unroll
+          for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch)
+              acc[cc][ch] = acc[cc][ch] + left(cc, ch);
+#pragma unroll
+          for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch)
+              acc[cc][ch] = acc[cc][ch] + right(cc, ch);
+        } else {
+#pragma unroll
+          for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch)
+              acc[cc][ch] = acc[cc][ch] + left(cc, ch);
+#pragma unroll
+          for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch)
+              acc[cc][ch] = acc[cc][ch] + right(cc, ch);
+#pragma unroll
+          for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch)
+              acc[cc][ch] = acc[cc][ch] + fr[cc][ch];
         }
 #pragma unroll
         for (int cc = 0; cc < C; ++cc)
 #pragma unroll
-          for (int ch = 0; ch < 3; ++ch) {
+          for (int ch = 0; ch < NCH; ++ch)
+            out[cc][ch] = upd[cc] ? acc[cc][ch] * 0.25f : newer[cc][ch];
+#pragma unroll
+        for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+          for (int ch = 0; ch < NCH; ++ch) {
             older[cc][ch] = fr[cc][ch];
             fr[cc][ch] = out[cc][ch];
           }
@@ -293,15 +357,15 @@ __device__ __forceinline__ void march_task(const MarchArgs& a, int stripe,
 
       const int rs = rho - S;
       if (lane_out && rs >= r0 && rs < r1) {
-        store_row<C>(a.xout, a.plane, a.P, rs, col0, fr);
+        store_row<C, NCH>(xout, a.plane, a.P, rs, col0, fr);
         if (dmax) {
           // level S - 1 of row rs is still the newer row of that level
-          const float(&prev)[C][3] =
+          const float(&prev)[C][NCH] =
               *pick<(u & 1) != 0>(&wb[S - 1], &wa[S - 1]);
 #pragma unroll
           for (int cc = 0; cc < C; ++cc)
 #pragma unroll
-            for (int ch = 0; ch < 3; ++ch)
+            for (int ch = 0; ch < NCH; ++ch)
               dm = fmaxf(dm, fabsf(fr[cc][ch] - prev[cc][ch]));
         }
       }
@@ -311,10 +375,11 @@ __device__ __forceinline__ void march_task(const MarchArgs& a, int stripe,
 #pragma unroll
       for (int cc = 0; cc < C; ++cc)
 #pragma unroll
-        for (int ch = 0; ch < 3; ++ch) gq[u][cc][ch] = gn[cc][ch];
+        for (int ch = 0; ch < NCH; ++ch) PIE_GQ(u, cc, ch) = gn[cc][ch];
       mq = (mq & ~(((1u << C) - 1u) << (u * C))) | (mn << (u * C));
     });
   }
+#undef PIE_GQ
   if (dmax) {
     // differences are non-negative, so float order is unsigned-int order
     const unsigned wave_max = wave_max_u32(__float_as_uint(dm));
@@ -322,11 +387,19 @@ __device__ __forceinline__ void march_task(const MarchArgs& a, int stripe,
   }
 }
 
+#ifndef PIE_WAVES_PER_EU
+#define PIE_WAVES_PER_EU 1
+#endif
 template <int S, int C, int SCHED, int ORDER>
-__global__ void __launch_bounds__(kBlock) march_kernel(MarchArgs a) {
+__global__ void __launch_bounds__(kBlock)
+    __attribute__((amdgpu_waves_per_eu(PIE_WAVES_PER_EU)))
+    march_kernel(MarchArgs a) {
   const int task = xcd_block() * kBlockWaves + (threadIdx.x / kWave);
   if (task >= a.ntask) return;  // whole wave leaves together
-  march_task<S, C, SCHED, ORDER>(a, task % a.stripes, task / a.stripes,
+  // task = (channel group, segment, stripe), stripe fastest
+  const int per_group = a.stripes * a.segs;
+  const int group = task / per_group, rem = task % per_group;
+  march_task<S, C, SCHED, ORDER>(a, rem % a.stripes, rem / a.stripes, group,
                                  a.dmax);
 }
 
@@ -362,8 +435,10 @@ __global__ void __launch_bounds__(kBlock) march_batch_kernel(BatchArgs b) {
     a.segs = it.segs;
     a.seg_rows = b.seg_rows;
     const int local = t - it.task0;
-    march_task<S, C, SCHED, ORDER>(a, local % it.stripes, local / it.stripes,
-                                   &b.items[id].dmax_bits);
+    const int per_group = it.stripes * it.segs;
+    const int group = local / per_group, rem = local % per_group;
+    march_task<S, C, SCHED, ORDER>(a, rem % it.stripes, rem / it.stripes,
+                                   group, &b.items[id].dmax_bits);
   }
 }
 

@@ -49,6 +49,7 @@ struct pie_solver {
   bool grid_layout = false;  // Grid, or Equ solved on the grid layout
   int H = 0, W = 0, P = 0;
   long long plane = 0, masked = 0;
+  long long bufsz = 0, mbytes = 0;  // elements per state buffer, mask bytes
   int cur = 0;  // which of xa/xb holds the state
   DevBuf<float> xa, xb, g;
   DevBuf<uint8_t> m, img;
@@ -104,6 +105,7 @@ void free_problem(pie_solver* s) {
   s->X4b.release();
   s->H = s->W = s->P = 0;
   s->plane = s->masked = 0;
+  s->bufsz = s->mbytes = 0;
   s->N = 0;
   s->cur = 0;
   s->tuned = false;
@@ -141,19 +143,22 @@ int alloc_grid(pie_solver* s, int H, int W) {
   if (H < 1 || W < 1) return fail(PIE_EINVAL, "empty grid");
   s->H = H;
   s->W = W;
-  s->P = pitch_of(W);
-  s->plane = (long long)H * s->P;
-  if (s->plane > (1LL << 31) - 1)
-    return fail(PIE_EINVAL, "grid larger than 2^31 elements per plane");
-  PIE_HIP(s->xa.alloc(3 * s->plane));
-  PIE_HIP(s->xb.alloc(3 * s->plane));
-  PIE_HIP(s->g.alloc(3 * s->plane));
-  PIE_HIP(s->m.alloc(s->plane));
+  const pie::host::GridLayout L = pie::host::grid_layout(H, W);
+  if ((long long)H * L.P > (1LL << 31) - 1)
+    return fail(PIE_EINVAL, "grid larger than 2^31 elements");
+  s->P = L.P;
+  s->plane = L.plane;
+  s->bufsz = L.floats;
+  s->mbytes = L.mask;
+  PIE_HIP(s->xa.alloc(s->bufsz));
+  PIE_HIP(s->xb.alloc(s->bufsz));
+  PIE_HIP(s->g.alloc(s->bufsz));
+  PIE_HIP(s->m.alloc(s->mbytes));
   // padding must hold finite values (zeros) and an unsolved mask
-  PIE_HIP(hipMemsetAsync(s->xa.p, 0, 3 * s->plane * sizeof(float), s->stream));
-  PIE_HIP(hipMemsetAsync(s->xb.p, 0, 3 * s->plane * sizeof(float), s->stream));
-  PIE_HIP(hipMemsetAsync(s->g.p, 0, 3 * s->plane * sizeof(float), s->stream));
-  PIE_HIP(hipMemsetAsync(s->m.p, 0, s->plane, s->stream));
+  PIE_HIP(hipMemsetAsync(s->xa.p, 0, s->bufsz * sizeof(float), s->stream));
+  PIE_HIP(hipMemsetAsync(s->xb.p, 0, s->bufsz * sizeof(float), s->stream));
+  PIE_HIP(hipMemsetAsync(s->g.p, 0, s->bufsz * sizeof(float), s->stream));
+  PIE_HIP(hipMemsetAsync(s->m.p, 0, s->mbytes, s->stream));
   const int nb = pie::finalize_blocks(H, W);
   PIE_HIP(s->partial.alloc(3 * (size_t)nb));
   PIE_HIP(s->err.alloc(3));
@@ -406,7 +411,7 @@ int pie_reset_equ(pie_solver* s, int n, const int32_t* A, const float* X,
   std::vector<int32_t> lin;
   bool structured = s->part_count == n - 1 && s->ph >= 3 && s->pw >= 3;
   if (structured) {
-    const int h = s->ph, w = s->pw, P = pitch_of(w);
+    const int h = s->ph, w = s->pw, P = pie::host::grid_layout(h, w).P;
     const int32_t* ids = s->part_ids.data();
     lin.assign(n, 0);
     for (int i = 0; i < h && structured; ++i)
@@ -431,7 +436,7 @@ int pie_reset_equ(pie_solver* s, int n, const int32_t* A, const float* X,
 
   if (structured) {
     PIE_CHECK(alloc_grid(s, s->ph, s->pw));
-    std::vector<uint8_t> mh((size_t)s->plane, 0);
+    std::vector<uint8_t> mh((size_t)s->mbytes, 0);
     for (int i = 0; i < s->H; ++i)
       for (int j = 0; j < s->W; ++j)
         mh[(size_t)i * s->P + pie::kPadLeft + j] =
@@ -445,10 +450,11 @@ int pie_reset_equ(pie_solver* s, int n, const int32_t* A, const float* X,
     PIE_CHECK(upload(B, 12 * (size_t)n, Bd.p, s->stream));
     PIE_CHECK(upload(lin.data(), 4 * (size_t)n, s->lin.p, s->stream));
     // unsolved pixels hold X[0]: the value the reference reads via id 0
-    PIE_HIP(pie::launch_fill3(s->xa.p, s->plane, X[0], X[1], X[2], s->stream));
+    PIE_HIP(pie::launch_fill3(s->xa.p, s->bufsz, s->P, s->plane, X[0], X[1],
+                              X[2], s->stream));
     PIE_HIP(pie::launch_equ_scatter(Xd.p, Bd.p, s->lin.p, n, s->xa0(),
                                     s->g0(), s->plane, s->stream));
-    PIE_HIP(hipMemcpyAsync(s->xb.p, s->xa.p, 3 * s->plane * sizeof(float),
+    PIE_HIP(hipMemcpyAsync(s->xb.p, s->xa.p, s->bufsz * sizeof(float),
                            hipMemcpyDeviceToDevice, s->stream));
     PIE_HIP(hipStreamSynchronize(s->stream));
     s->masked = n - 1;
@@ -509,7 +515,7 @@ int pie_reset_grid(pie_solver* s, int h, int w, const int32_t* mask,
   PIE_CHECK(upload(tgt, px * 12, stage.p, s->stream));
   PIE_HIP(pie::launch_deinterleave(reinterpret_cast<const float*>(stage.p),
                                    s->xa0(), s->plane, h, w, s->P, s->stream));
-  PIE_HIP(hipMemcpyAsync(s->xb.p, s->xa.p, 3 * s->plane * sizeof(float),
+  PIE_HIP(hipMemcpyAsync(s->xb.p, s->xa.p, s->bufsz * sizeof(float),
                          hipMemcpyDeviceToDevice, s->stream));
   PIE_CHECK(upload(grad, px * 12, stage.p, s->stream));
   PIE_HIP(pie::launch_deinterleave(reinterpret_cast<const float*>(stage.p),
@@ -565,7 +571,7 @@ int pie_set_state(pie_solver* s, const float* x) {
   // both buffers: padding and unsolved pixels keep their values
   PIE_HIP(pie::launch_deinterleave(stage.p, s->xa0(), s->plane, s->H, s->W,
                                    s->P, s->stream));
-  PIE_HIP(hipMemcpyAsync(s->xb.p, s->xa.p, 3 * s->plane * sizeof(float),
+  PIE_HIP(hipMemcpyAsync(s->xb.p, s->xa.p, s->bufsz * sizeof(float),
                          hipMemcpyDeviceToDevice, s->stream));
   PIE_HIP(hipStreamSynchronize(s->stream));
   s->cur = 0;

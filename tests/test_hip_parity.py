@@ -162,7 +162,7 @@ def _tiles():
   import os
   from conftest import PKG
   text = open(os.path.join(PKG, "csrc", "kernels.h")).read()
-  line = re.search(r"#define PIE_TILES\(X\)(.*)", text).group(1)
+  line = re.search(r"#define PIE_TILES\(X\)((?:.*\\\n)*.*)", text).group(1)
   return [(int(a), int(b)) for a, b in re.findall(r"X\((\d+), (\d+)\)", line)]
 
 
