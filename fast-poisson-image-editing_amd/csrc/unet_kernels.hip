@@ -145,7 +145,8 @@ constexpr int kWgPix = 32;
 __global__ void __launch_bounds__(kBlock)
     wgrad_kernel(const bf16_t* __restrict__ x, int x_stride, int x_off,
                  int cin, int n, int h, int w, const bf16_t* __restrict__ dy,
-                 int cout, float* __restrict__ dw, int chunks_per_block) {
+                 int cout, float* __restrict__ dw, float* __restrict__ db,
+                 int chunks_per_block) {
   __shared__ __attribute__((aligned(16))) bf16_t xs[9][32][kWgPix];
   __shared__ __attribute__((aligned(16))) bf16_t ds[32][kWgPix];
   const int lane = threadIdx.x & (kWave - 1);
@@ -165,6 +166,10 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
       for (int b = 0; b < 2; ++b) acc[t][a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
 
+  // bias gradient: the blocks of the first input-channel tile also sum
+  // their staged output-gradient columns (thread t < 32: channel co0 + t)
+  const bool bias = db && ci0 == 0 && threadIdx.x < 32;
+  float bsum = 0.0f;
   const long long c_begin = (long long)blockIdx.x * chunks_per_block;
   const long long c_end = min(nchunk, c_begin + chunks_per_block);
   for (long long c = c_begin; c < c_end; ++c) {
@@ -202,6 +207,9 @@ __global__ void __launch_bounds__(kBlock)
       for (int k = 0; k < 8; ++k) ds[cg * 8 + k][pix] = e[k];
     }
     __syncthreads();
+    if (bias)
+#pragma unroll 8
+      for (int pix = 0; pix < kWgPix; ++pix) bsum += bf2f(ds[threadIdx.x][pix]);
     bf16x8 B[2];
 #pragma unroll
     for (int b = 0; b < 2; ++b)
@@ -225,6 +233,8 @@ __global__ void __launch_bounds__(kBlock)
     }
     __syncthreads();
   }
+  if (bias && co0 + (int)threadIdx.x < cout)
+    atomicAdd(&db[co0 + threadIdx.x], bsum);
   // D[ci][co]: lane holds col co = co0 + 16b + i, rows ci = ci0 + 16a + 4g + r
 #pragma unroll
   for (int t = 0; t < 3; ++t) {
@@ -243,19 +253,6 @@ __global__ void __launch_bounds__(kBlock)
                       acc[t][a][b][r]);
         }
       }
-  }
-}
-
-__global__ void __launch_bounds__(kBlock)
-    bias_grad_kernel(const bf16_t* __restrict__ dy, long long P, int cout,
-                     float* db, int rows_per_block) {
-  // thread t sums channel t % cout over its block's rows
-  const long long r0 = (long long)blockIdx.x * rows_per_block;
-  const long long r1 = min(P, r0 + rows_per_block);
-  for (int co = threadIdx.x; co < cout; co += kBlock) {
-    float s = 0.0f;
-    for (long long p = r0; p < r1; ++p) s += bf2f(dy[p * cout + co]);
-    atomicAdd(&db[co], s);
   }
 }
 
@@ -347,6 +344,26 @@ __global__ void __launch_bounds__(kBlock)
     const long long img = q / ((long long)W * H);
     const long long s = (img * h + y / 2) * w + x / 2;
     out[q * out_stride + out_off + c] = in[s * in_stride + in_off + c];
+  }
+}
+
+// up_fwd_kernel with eight channels (16 bytes) per thread: channel counts,
+// strides and offsets are multiples of 8.
+__global__ void __launch_bounds__(kBlock)
+    up_fwd8_kernel(const bf16_t* __restrict__ in, int in_stride, int in_off,
+                   int n, int h, int w, int ch, bf16_t* __restrict__ out,
+                   int out_stride, int out_off) {
+  const int H = 2 * h, W = 2 * w, cg = ch / 8;
+  const long long total = (long long)n * H * W * cg;
+  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < total;
+       id += (long long)gridDim.x * kBlock) {
+    const int c = (int)(id % cg) * 8;
+    const long long q = id / cg;
+    const int x = (int)(q % W), y = (int)((q / W) % H);
+    const long long img = q / ((long long)W * H);
+    const long long sp = (img * h + y / 2) * w + x / 2;
+    *reinterpret_cast<uint4*>(out + q * out_stride + out_off + c) =
+        *reinterpret_cast<const uint4*>(in + sp * in_stride + in_off + c);
   }
 }
 
@@ -561,12 +578,7 @@ hipError_t conv3x3_wgrad(const Slice& x, int n, int h, int w,
   const int xb = (int)((nchunk + per - 1) / per);
   hipLaunchKernelGGL(wgrad_kernel, dim3(xb, tiles), dim3(kBlock), 0, st,
                      static_cast<const bf16_t*>(x.base), x.stride, x.off,
-                     x.ch, n, h, w, dy, cout, dw, (int)per);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess || !db) return e;
-  const int rows = 4096;
-  hipLaunchKernelGGL(bias_grad_kernel, dim3((unsigned)((P + rows - 1) / rows)),
-                     dim3(kBlock), 0, st, dy, P, cout, db, rows);
+                     x.ch, n, h, w, dy, cout, dw, db, (int)per);
   return hipGetLastError();
 }
 
@@ -605,6 +617,16 @@ hipError_t pool_bwd(const float* dy, int n, int h2, int w2, int ch, float* dx,
 
 hipError_t up_fwd(const Slice& in, int n, int h, int w, const Slice& out,
                   hipStream_t st) {
+  if (in.ch % 8 == 0 && in.stride % 8 == 0 && in.off % 8 == 0 &&
+      out.stride % 8 == 0 && out.off % 8 == 0) {
+    hipLaunchKernelGGL(up_fwd8_kernel,
+                       dim3(blocks_for((long long)n * h * w * 4 * in.ch / 8)),
+                       dim3(kBlock), 0, st,
+                       static_cast<const bf16_t*>(in.base), in.stride, in.off,
+                       n, h, w, in.ch, static_cast<bf16_t*>(out.base),
+                       out.stride, out.off);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(up_fwd_kernel,
                      dim3(blocks_for((long long)n * h * w * 4 * in.ch)),
                      dim3(kBlock), 0, st, static_cast<const bf16_t*>(in.base),

@@ -204,3 +204,41 @@ def train(M, X, Y, epochs: int = 4, batch: int = 64, lr: float = 1e-3,
   st = net.stats()
   return net, {"losses": losses, "seconds": time.perf_counter() - t0,
                "steps": len(losses), **st}
+
+
+def main(argv=None) -> None:
+  """Time Adam steps of the network on random inputs:
+  python -m fpie_amd.unet [--batch 64] [--size 256] [--steps 5]"""
+  import argparse
+  import json
+  ap = argparse.ArgumentParser()
+  ap.add_argument("--batch", type=int, default=64)
+  ap.add_argument("--size", type=int, default=256)
+  ap.add_argument("--steps", type=int, default=5)
+  a = ap.parse_args(argv)
+  rng = np.random.default_rng(0)
+  b, s = a.batch, a.size
+  x = np.zeros((b, s, s, 8), np.float32)
+  x[..., :3] = rng.normal(0, 1, (b, s, s, 3))
+  x[..., 6] = 1.0
+  y = rng.normal(0, 1, (b, s, s, 3)).astype(np.float32)
+  m = np.ones((b, s, s), np.uint8)
+  net = UNet(b, s, s, seed=0)
+  net.train_step(x, y, m, 1e-3)  # untimed: kernel loading
+  before = net.stats()
+  t0 = time.perf_counter()
+  for _ in range(a.steps):
+    net.train_step(x, y, m, 1e-3)
+  dt = time.perf_counter() - t0
+  after = net.stats()
+  flops = after["conv_flops"] - before["conv_flops"]
+  print(json.dumps({"batch": b, "size": s, "steps": a.steps,
+                    "ms_per_step": dt * 1e3 / a.steps,
+                    "conv_tflops": flops / dt / 1e12,
+                    "mfma_utilisation_vs_2500_tflops": flops / dt / 2.5e15}),
+        flush=True)
+
+
+if __name__ == "__main__":
+  import sys
+  main(sys.argv[1:])
