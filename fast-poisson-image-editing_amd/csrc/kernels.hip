@@ -40,6 +40,44 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// fpie/process.py:113-122: the source difference ('src'), the mean of both
+// ('avg'), or the larger in magnitude, ties keeping the source ('max').
+__device__ __forceinline__ float mixgrad(float a, float b, int mode) {
+  if (mode == 0) return a;
+  if (mode == 1) return (a + b) / 2.0f;
+  return fabsf(a) < fabsf(b) ? b : a;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    divergence_kernel(const uint8_t* __restrict__ src,
+                      const uint8_t* __restrict__ tgt,
+                      const uint8_t* __restrict__ m, float* __restrict__ x,
+                      float* __restrict__ g, long long plane, int H, int W,
+                      int P, int mode) {
+  const long long n = (long long)H * W;
+  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < n;
+       id += (long long)gridDim.x * kBlock) {
+    const int i = (int)(id / W), j = (int)(id % W);
+    const long long o = (long long)i * P + j;
+    const bool solved = m[o] != 0;  // never on the outermost rows/columns
+    const long long nb[4] = {id - W, id + W, id - 1, id + 1};  // U, D, L, R
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float tc = (float)tgt[id * 3 + c];
+      x[c * plane + o] = tc;
+      float acc = 0.0f;
+      if (solved) {
+        const float sc = (float)src[id * 3 + c];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          acc = acc + mixgrad(sc - (float)src[nb[k] * 3 + c],
+                              tc - (float)tgt[nb[k] * 3 + c], mode);
+      }
+      g[c * plane + o] = acc;
+    }
+  }
+}
+
 __global__ void __launch_bounds__(kBlock)
     mask_kernel(const int32_t* __restrict__ src, uint8_t* __restrict__ dst,
                 int H, int W, int P, unsigned long long* counts) {
@@ -589,6 +627,16 @@ hipError_t launch_mask(const int32_t* src, uint8_t* dst, int H, int W, int P,
                        unsigned long long* counts, hipStream_t st) {
   hipLaunchKernelGGL(mask_kernel, dim3(grid_blocks((long long)H * W, 8192)),
                      dim3(kBlock), 0, st, src, dst, H, W, P, counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_divergence(const uint8_t* src, const uint8_t* tgt,
+                             const uint8_t* m, float* x, float* g,
+                             long long plane, int H, int W, int P, int mode,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(divergence_kernel,
+                     dim3(grid_blocks((long long)H * W, 8192)), dim3(kBlock),
+                     0, st, src, tgt, m, x, g, plane, H, W, P, mode);
   return hipGetLastError();
 }
 

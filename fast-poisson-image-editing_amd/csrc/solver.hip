@@ -487,16 +487,15 @@ int pie_reset_equ(pie_solver* s, int n, const int32_t* A, const float* X,
   return PIE_OK;
 }
 
-int pie_reset_grid(pie_solver* s, int h, int w, const int32_t* mask,
-                   const float* tgt, const float* grad) {
-  if (!s || !mask || !tgt || !grad) return fail(PIE_EINVAL, "NULL argument");
-  if (s->kind != PIE_GRID) return fail(PIE_EINVAL, "reset_grid on an EquSolver");
-  PIE_HIP(hipStreamSynchronize(s->stream));
-  free_problem(s);
+namespace {
+
+// Allocate the grid of a GridSolver problem and load its mask (staged
+// through `stage`, at least h * w * 4 bytes); rejects solved pixels on the
+// outermost rows/columns.  *masked gets the solved-pixel count.
+int load_grid_mask(pie_solver* s, int h, int w, const int32_t* mask,
+                   DevBuf<char>& stage, long long* masked) {
   PIE_CHECK(alloc_grid(s, h, w));
   const size_t px = (size_t)h * w;
-  DevBuf<char> stage;
-  PIE_HIP(stage.alloc(px * 12));
   DevBuf<unsigned long long> counts;
   PIE_HIP(counts.alloc(2));
   PIE_HIP(hipMemsetAsync(counts.p, 0, 2 * sizeof(unsigned long long), s->stream));
@@ -512,6 +511,24 @@ int pie_reset_grid(pie_solver* s, int h, int w, const int32_t* mask,
     return fail(PIE_EINVAL,
                 "mask has solved pixels on the outermost rows/columns");
   }
+  *masked = (long long)cnt[0];
+  return PIE_OK;
+}
+
+}  // namespace
+
+int pie_reset_grid(pie_solver* s, int h, int w, const int32_t* mask,
+                   const float* tgt, const float* grad) {
+  if (!s || !mask || !tgt || !grad) return fail(PIE_EINVAL, "NULL argument");
+  if (s->kind != PIE_GRID) return fail(PIE_EINVAL, "reset_grid on an EquSolver");
+  if (h < 1 || w < 1) return fail(PIE_EINVAL, "empty grid");
+  PIE_HIP(hipStreamSynchronize(s->stream));
+  free_problem(s);
+  const size_t px = (size_t)h * w;
+  DevBuf<char> stage;
+  PIE_HIP(stage.alloc(px * 12));
+  long long masked = 0;
+  PIE_CHECK(load_grid_mask(s, h, w, mask, stage, &masked));
   PIE_CHECK(upload(tgt, px * 12, stage.p, s->stream));
   PIE_HIP(pie::launch_deinterleave(reinterpret_cast<const float*>(stage.p),
                                    s->xa0(), s->plane, h, w, s->P, s->stream));
@@ -521,7 +538,37 @@ int pie_reset_grid(pie_solver* s, int h, int w, const int32_t* mask,
   PIE_HIP(pie::launch_deinterleave(reinterpret_cast<const float*>(stage.p),
                                    s->g0(), s->plane, h, w, s->P, s->stream));
   PIE_HIP(hipStreamSynchronize(s->stream));
-  s->masked = (long long)cnt[0];
+  s->masked = masked;
+  s->N = h * w;
+  PIE_CHECK(default_tile(s));
+  s->loaded = true;
+  return PIE_OK;
+}
+
+int pie_reset_grid_images(pie_solver* s, int h, int w, const int32_t* mask,
+                          const uint8_t* src, const uint8_t* tgt,
+                          int gradient) {
+  if (!s || !mask || !src || !tgt) return fail(PIE_EINVAL, "NULL argument");
+  if (s->kind != PIE_GRID) return fail(PIE_EINVAL, "reset_grid on an EquSolver");
+  if (h < 1 || w < 1) return fail(PIE_EINVAL, "empty grid");
+  if (gradient < PIE_GRAD_SRC || gradient > PIE_GRAD_MAX)
+    return fail(PIE_EINVAL, "gradient must be PIE_GRAD_SRC, _AVG or _MAX");
+  PIE_HIP(hipStreamSynchronize(s->stream));
+  free_problem(s);
+  const size_t px = (size_t)h * w;
+  DevBuf<char> stage;  // the int32 mask, then both uint8 images
+  PIE_HIP(stage.alloc(px * 6 > px * 4 ? px * 6 : px * 4));
+  long long masked = 0;
+  PIE_CHECK(load_grid_mask(s, h, w, mask, stage, &masked));
+  uint8_t* su = reinterpret_cast<uint8_t*>(stage.p);
+  PIE_CHECK(upload(src, px * 3, su, s->stream));
+  PIE_CHECK(upload(tgt, px * 3, su + px * 3, s->stream));
+  PIE_HIP(pie::launch_divergence(su, su + px * 3, s->m0(), s->xa0(), s->g0(),
+                                 s->plane, h, w, s->P, gradient, s->stream));
+  PIE_HIP(hipMemcpyAsync(s->xb.p, s->xa.p, s->bufsz * sizeof(float),
+                         hipMemcpyDeviceToDevice, s->stream));
+  PIE_HIP(hipStreamSynchronize(s->stream));
+  s->masked = masked;
   s->N = h * w;
   PIE_CHECK(default_tile(s));
   s->loaded = true;

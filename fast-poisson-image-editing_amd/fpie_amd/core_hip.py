@@ -184,6 +184,26 @@ class GridSolver(_Core):
     native.check(self._lib.pie_get_state(self._h, _ptr(out)))
     return out
 
+  GRADIENT_MODES = {"src": 0, "avg": 1, "max": 2}
+
+  def reset_images(self, mask: np.ndarray, src: np.ndarray, tgt: np.ndarray,
+                   gradient: str) -> None:
+    """reset() built on the device from the uint8 source and target crops
+    [h, w, 3] and the prepared crop mask [h, w] (extension): the target as
+    the state, the mixed-gradient divergence as the guidance, exactly what
+    GridProcessor.reset computes on the host (fpie/process.py:352-386)."""
+    m = np.ascontiguousarray(mask, dtype=np.int32)
+    s8 = np.ascontiguousarray(src, dtype=np.uint8)
+    t8 = np.ascontiguousarray(tgt, dtype=np.uint8)
+    if s8.shape != m.shape + (3,) or t8.shape != m.shape + (3,):
+      raise ValueError("src and tgt must be uint8 [h, w, 3] like the mask")
+    if gradient not in self.GRADIENT_MODES:
+      raise ValueError(f"gradient must be one of {list(self.GRADIENT_MODES)}")
+    native.check(self._lib.pie_reset_grid_images(
+        self._h, m.shape[0], m.shape[1], _ptr(m), _ptr(s8), _ptr(t8),
+        self.GRADIENT_MODES[gradient]))
+    self.hw = m.shape
+
   def set_state(self, x: np.ndarray) -> None:
     """Replace the state (a warm start) by float32 x[h][w][3] (extension)."""
     x = np.ascontiguousarray(x, dtype=np.float32)

@@ -256,15 +256,23 @@ class GridProcessor(BaseProcessor):
   def reset(self, src, mask, tgt, mask_on_src, mask_on_tgt) -> int:
     m, x0, x1, y0, y1 = prepare_mask(mask)
     h, w = m.shape
-    sc = src[mask_on_src[0] + x0:mask_on_src[0] + x1,
-             mask_on_src[1] + y0:mask_on_src[1] + y1].astype(np.float32)
-    tc = tgt[mask_on_tgt[0] + x0:mask_on_tgt[0] + x1,
-             mask_on_tgt[1] + y0:mask_on_tgt[1] + y1].astype(np.float32)
-    grad = divergence(sc, tc, m, self.mixgrad)
+    s_crop = src[mask_on_src[0] + x0:mask_on_src[0] + x1,
+                 mask_on_src[1] + y0:mask_on_src[1] + y1]
+    t_crop = tgt[mask_on_tgt[0] + x0:mask_on_tgt[0] + x1,
+                 mask_on_tgt[1] + y0:mask_on_tgt[1] + y1]
     self.x0, self.x1 = mask_on_tgt[0] + x0, mask_on_tgt[0] + x1
     self.y0, self.y1 = mask_on_tgt[1] + y0, mask_on_tgt[1] + y1
     self.tgt = tgt.copy()
     n = h * w
+    if (hasattr(self.core, "reset_images") and s_crop.dtype == np.uint8
+        and t_crop.dtype == np.uint8 and s_crop.ndim == 3):
+      # the core builds target and divergence on the device from the uint8
+      # crops: the same values as below, a quarter of the bytes to copy
+      self.core.reset_images(m, s_crop, t_crop, self.gradient)
+      return int(n)
+    sc = s_crop.astype(np.float32)
+    tc = t_crop.astype(np.float32)
+    grad = divergence(sc, tc, m, self.mixgrad)
     self.core.reset(n, m, tc, grad)
     return int(n)
 

@@ -96,6 +96,21 @@ int pie_reset_equ(pie_solver* s, int n, const int32_t* A, const float* X,
 int pie_reset_grid(pie_solver* s, int h, int w, const int32_t* mask,
                    const float* tgt, const float* grad);
 
+/* GridSolver.reset built on the device from the uint8 crops: what
+ * GridProcessor.reset (fpie/process.py:352-386) computes on the host and
+ * then passes to pie_reset_grid.  src and tgt uint8[h][w][3] are the source
+ * and target crops, mask int32[h][w] the prepared crop mask; the state is
+ * the target as float and the guidance, at each solved pixel, the sum over
+ * the up, down, left, right neighbours of mixgrad(src_c - src_n,
+ * tgt_c - tgt_n) (process.py:113-122, 362-378), 0 elsewhere.  Integer and
+ * half-integer values: identical to the host computation bit for bit. */
+#define PIE_GRAD_SRC 0
+#define PIE_GRAD_AVG 1
+#define PIE_GRAD_MAX 2
+int pie_reset_grid_images(pie_solver* s, int h, int w, const int32_t* mask,
+                          const uint8_t* src, const uint8_t* tgt,
+                          int gradient);
+
 /* sync() (base_solver.h:61): waits for queued work; no data moves. */
 int pie_sync(pie_solver* s);
 
