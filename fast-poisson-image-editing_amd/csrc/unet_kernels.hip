@@ -19,6 +19,9 @@ constexpr int kBlock = 256;
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+// staging registers: a clang vector, not HIP's uint4 struct, so that
+// fixed-index arrays of them stay in VGPRs
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(bf16_t v) {
   return __uint_as_float((uint32_t)v << 16);
@@ -133,6 +136,158 @@ __global__ void __launch_bounds__(kBlock) conv3x3_kernel(ConvArgs a) {
         }
       }
     }
+}
+
+// LDS-tiled variant (input channels a multiple of 32): a block computes a
+// (4 MT) x 16-pixel spatial tile of one image for 16 NT output channels.
+// For every 32 input channels it stages the input window (tile plus a
+// one-pixel halo) and the weights of all nine taps in LDS once; each wave
+// then runs its MT x NT MFMA tiles for the nine taps from shifted views of
+// the window (lane group g holds channels 8g .. 8g+7 of the 32). The global
+// loads of the next 32 channels are issued before the MFMAs of the current
+// ones (addresses clamped into the image, halo values selected to zero, so
+// every load is unconditional and they all overlap).
+// Global loads of one 32-channel stage of conv3x3_lds_kernel: the input
+// window (WH x WW pixels x 4 groups of 8 channels) and the weights of the
+// nine taps for CT output channels, into registers. Addresses are clamped
+// into the image and every load is issued; bit k of the returned mask says
+// whether window word k lies inside the image (the halo outside is zero,
+// applied when the registers are stored to LDS, so that the compiler keeps
+// the loads unconditional and in flight together).
+template <int WH, int WW, int CT>
+__device__ __forceinline__ uint32_t conv_stage_load(
+    const ConvArgs& a, long long img_base, int row0, int col0, int co0,
+    int c32, u32x4 (&wv)[(WH * WW * 4 + kBlock - 1) / kBlock],
+    u32x4 (&wt)[(9 * 4 * CT + kBlock - 1) / kBlock]) {
+  constexpr int kWinJobs = WH * WW * 4, kWinIt = (kWinJobs + kBlock - 1) / kBlock;
+  constexpr int kWtJobs = 9 * 4 * CT, kWtIt = (kWtJobs + kBlock - 1) / kBlock;
+  const int CB = a.cin / 8;
+  uint32_t ok = 0;
+#pragma unroll
+  for (int k = 0; k < kWinIt; ++k) {
+    const int job = threadIdx.x + k * kBlock;
+    const int jj = min(job, kWinJobs - 1);
+    const int pix = jj >> 2, gg = jj & 3;
+    const int y = row0 - 1 + pix / WW, x = col0 - 1 + pix % WW;
+    if (y >= 0 && y < a.h && x >= 0 && x < a.w) ok |= 1u << k;
+    const int yc = min(max(y, 0), a.h - 1), xc = min(max(x, 0), a.w - 1);
+    wv[k] = *reinterpret_cast<const u32x4*>(
+        a.in + (img_base + (long long)yc * a.w + xc) * a.in_stride +
+        a.in_off + c32 * 32 + gg * 8);
+  }
+#pragma unroll
+  for (int k = 0; k < kWtIt; ++k) {
+    const int jj = min((int)threadIdx.x + k * kBlock, kWtJobs - 1);
+    const int co = jj % CT, rest = jj / CT;
+    const int kb = (rest >> 2) * CB + c32 * 4 + (rest & 3);
+    wt[k] = *reinterpret_cast<const u32x4*>(
+        a.wk + ((long long)kb * a.cout + co0 + co) * 8);
+  }
+  return ok;
+}
+
+template <int WH, int WW, int CT>
+__device__ __forceinline__ void conv_stage_store(
+    const u32x4 (&wv)[(WH * WW * 4 + kBlock - 1) / kBlock],
+    const u32x4 (&wt)[(9 * 4 * CT + kBlock - 1) / kBlock], uint32_t ok,
+    u32x4 (&win)[WH][WW][4], u32x4 (&ws)[9][4][CT]) {
+  constexpr int kWinJobs = WH * WW * 4, kWinIt = (kWinJobs + kBlock - 1) / kBlock;
+  constexpr int kWtJobs = 9 * 4 * CT, kWtIt = (kWtJobs + kBlock - 1) / kBlock;
+#pragma unroll
+  for (int k = 0; k < kWinIt; ++k) {
+    const int job = threadIdx.x + k * kBlock;
+    if (job < kWinJobs)
+      win[(job >> 2) / WW][(job >> 2) % WW][job & 3] =
+          ((ok >> k) & 1u) ? wv[k] : u32x4{0u, 0u, 0u, 0u};
+  }
+#pragma unroll
+  for (int k = 0; k < kWtIt; ++k) {
+    const int job = threadIdx.x + k * kBlock;
+    if (job < kWtJobs) {
+      const int co = job % CT, rest = job / CT;
+      ws[rest >> 2][rest & 3][co] = wt[k];
+    }
+  }
+}
+
+template <int MT, int NT>
+__global__ void __launch_bounds__(kBlock) conv3x3_lds_kernel(ConvArgs a) {
+  constexpr int TH = 4 * MT, TW = 16, WH = TH + 2, WW = TW + 2;
+  constexpr int CT = 16 * NT;
+  constexpr int kWinJobs = WH * WW * 4, kWinIt = (kWinJobs + kBlock - 1) / kBlock;
+  constexpr int kWtJobs = 9 * 4 * CT, kWtIt = (kWtJobs + kBlock - 1) / kBlock;
+  __shared__ u32x4 win[WH][WW][4];
+  __shared__ u32x4 ws[9][4][CT];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int g = lane >> 4, i = lane & 15;
+  const int tiles_x = (a.w + TW - 1) / TW, tiles_y = (a.h + TH - 1) / TH;
+  const int tile = blockIdx.x;
+  const int img = tile / (tiles_x * tiles_y);
+  const int trem = tile % (tiles_x * tiles_y);
+  const int row0 = (trem / tiles_x) * TH, col0 = (trem % tiles_x) * TW;
+  const int co0 = blockIdx.y * CT;
+  const long long img_base = (long long)img * a.h * a.w;
+  u32x4 wv[kWinIt], wt[kWtIt];
+  f32x4 acc[MT][NT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  const int nc32 = a.cin / 32;
+  uint32_t ok = conv_stage_load<WH, WW, CT>(a, img_base, row0, col0, co0, 0,
+                                            wv, wt);
+  for (int c32 = 0; c32 < nc32; ++c32) {
+    conv_stage_store<WH, WW, CT>(wv, wt, ok, win, ws);
+    __syncthreads();
+    if (c32 + 1 < nc32)
+      ok = conv_stage_load<WH, WW, CT>(a, img_base, row0, col0, co0, c32 + 1,
+                                       wv, wt);
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+      bf16x8 A[MT], B[NT];
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+        B[nt] = __builtin_bit_cast(bf16x8, ws[tap][g][nt * 16 + i]);
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+        A[mt] = __builtin_bit_cast(
+            bf16x8, win[wave * MT + mt + 1 + dy][i + 1 + dx][g]);
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+              A[mt], B[nt], acc[mt][nt], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    const int y = row0 + wave * MT + mt;
+    if (y >= a.h) continue;
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const int co = co0 + nt * 16 + i;
+      const float b = a.bias ? a.bias[co] : 0.0f;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int x = col0 + 4 * g + r;
+        if (x >= a.w) continue;
+        const long long p = img_base + (long long)y * a.w + x;
+        float v = acc[mt][nt][r] + b;
+        if (a.relu) v = fmaxf(v, 0.0f);
+        if (a.out_bf) a.out_bf[p * a.out_stride + a.out_off + co] = f2bf(v);
+        if (a.out_f32) {
+          float* o = a.out_f32 + p * a.out_f32_stride + co;
+          *o = a.accumulate ? *o + v : v;
+        }
+      }
+    }
+  }
 }
 
 // Weight gradient: a block owns 32 input x 32 output channels for all nine
@@ -254,6 +409,195 @@ __global__ void __launch_bounds__(kBlock)
         }
       }
   }
+}
+
+// Weight gradient over spatial tiles (input channels a multiple of 8, output
+// channels of 32): a block owns 32 input x 32 output channels (input
+// channels past cin read as zero) for all nine taps and
+// sums over 8 x 16-pixel tiles of the images. For each tile it stages,
+// channel-major with the pixels of a row contiguous, three copies of the
+// input window (tile plus one-pixel halo) shifted by one column each, so
+// that every tap reads the 8 pixels of a lane group as one aligned 16-byte
+// word, and the output gradient of the tile. Wave w owns the 16 x 16 (ci,
+// co) block (w / 2, w % 2) of all nine taps; one MFMA step covers two rows
+// of the tile (lane group g: row g / 2, columns 8 (g % 2) .. +7).
+//
+// Staging jobs: threads 0..63 load 8 pixels x 8 channels of the gradient
+// (and keep the bias sums of their channels); threads 64..143 load 10
+// pixels x 8 channels of an input window row, from which the three shifted
+// copies are cut. A job's global loads are unconditional (addresses clamped
+// into the image, values outside selected to zero) and the loads of the
+// next tile are in flight while the MFMAs of the current one run.
+constexpr int kWtH = 8, kWtW = 16;
+constexpr int kWtRowsX = kWtH + 2;
+// 16-byte words per channel row block, padded to an odd count so that the
+// 16 lanes of a group hit distinct banks
+constexpr int kWtXStride = kWtRowsX * 2 + 1;
+constexpr int kWtDStride = kWtH * 2 + 1;
+constexpr int kWtDJobs = kWtH * 2 * 4, kWtXJobs = kWtRowsX * 2 * 4;
+
+__device__ inline uint32_t pack_lo(uint32_t lo, uint32_t hi) {
+  return (lo & 0xffffu) | (hi << 16);
+}
+__device__ inline uint32_t pack_hi(uint32_t lo, uint32_t hi) {
+  return (lo >> 16) | (hi & 0xffff0000u);
+}
+
+// 8 pixels x 8 channels (v[p0 + p] holds channels 0..7 of pixel p) to 8
+// channels x 8 pixels (o[c] holds pixels 0..7 of channel c)
+template <int N>
+__device__ __forceinline__ void transpose8(const u32x4 (&v)[N], int p0,
+                                           u32x4 (&o)[8]) {
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t lo = v[p0 + 2 * j][c >> 1], hi = v[p0 + 2 * j + 1][c >> 1];
+      o[c][j] = (c & 1) ? pack_hi(lo, hi) : pack_lo(lo, hi);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    wgrad_lds_kernel(const bf16_t* __restrict__ x, int x_stride, int x_off,
+                     int cin, int n, int h, int w,
+                     const bf16_t* __restrict__ dy, int cout,
+                     float* __restrict__ dw, float* __restrict__ db,
+                     int tiles_per_block) {
+  __shared__ u32x4 xw[3 * 32 * kWtXStride];
+  __shared__ u32x4 dt[32 * kWtDStride];
+  __shared__ float bsum[32];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int g = lane >> 4, i = lane & 15;
+  const int ta = wave >> 1, tb = wave & 1;
+  const int nci = (cin + 31) / 32;
+  const int ci0 = (blockIdx.y % nci) * 32, co0 = (blockIdx.y / nci) * 32;
+  const int tiles_x = (w + kWtW - 1) / kWtW, tiles_y = (h + kWtH - 1) / kWtH;
+  const long long ntiles = (long long)n * tiles_x * tiles_y;
+  const bool bias = db && ci0 == 0;
+  const int tid = threadIdx.x;
+  const bool djob = tid < kWtDJobs;
+  const bool xjob = tid >= kWtDJobs && tid < kWtDJobs + kWtXJobs;
+  // job coordinates: channel group, half row, row
+  const int jid = djob ? tid : tid - kWtDJobs;
+  const int cg = jid & 3, hx = (jid >> 2) & 1, jr = jid >> 3;
+  // input channel group inside cin (the loads of the others are clamped to
+  // the last group and zeroed)
+  const bool cok = ci0 + cg * 8 < cin;
+  const int xch = min(ci0 + cg * 8, cin - 8);
+  if (tid < 32) bsum[tid] = 0.0f;
+  float bpart[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) bpart[k] = 0.0f;
+
+  // staging registers of the thread's job and the in-image bit of each
+  u32x4 v[10];
+  uint32_t ok = 0;
+  auto load = [&](long long t) {
+    const int img = (int)(t / (tiles_x * tiles_y));
+    const int trem = (int)(t % (tiles_x * tiles_y));
+    const int row0 = (trem / tiles_x) * kWtH, col0 = (trem % tiles_x) * kWtW;
+    ok = 0;
+    if (djob) {
+      const int y = row0 + jr;
+      const int yc = min(y, h - 1);
+      const bf16_t* row = dy + ((long long)img * h + yc) * w * cout + co0 + cg * 8;
+#pragma unroll
+      for (int p = 0; p < 8; ++p) {
+        const int xx = col0 + 8 * hx + p;
+        if (y < h && xx < w) ok |= 1u << p;
+        v[p] = *reinterpret_cast<const u32x4*>(row + min(xx, w - 1) * cout);
+      }
+    } else if (xjob) {
+      const int y = row0 + jr - 1;
+      const int yc = min(max(y, 0), h - 1);
+      const bf16_t* row = x + ((long long)img * h + yc) * w * x_stride + x_off +
+                          xch;
+#pragma unroll
+      for (int p = 0; p < 10; ++p) {
+        const int xx = col0 + 8 * hx - 1 + p;
+        const int xc = min(max(xx, 0), w - 1);
+        if (cok && y >= 0 && y < h && xx >= 0 && xx < w) ok |= 1u << p;
+        v[p] = *reinterpret_cast<const u32x4*>(row + xc * x_stride);
+      }
+    }
+  };
+  auto store = [&]() {
+    u32x4 o[8];
+#pragma unroll
+    for (int p = 0; p < 10; ++p)
+      if (!((ok >> p) & 1u)) v[p] = u32x4{0u, 0u, 0u, 0u};
+    if (djob) {
+      if (bias) {
+#pragma unroll
+        for (int p = 0; p < 8; ++p)
+#pragma unroll
+          for (int k = 0; k < 8; ++k)
+            bpart[k] += __uint_as_float((k & 1) ? (v[p][k >> 1] & 0xffff0000u)
+                                                : (v[p][k >> 1] << 16));
+      }
+      transpose8(v, 0, o);
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+        dt[(cg * 8 + c) * kWtDStride + jr * 2 + hx] = o[c];
+    } else if (xjob) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        transpose8(v, d, o);
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+          xw[(d * 32 + cg * 8 + c) * kWtXStride + jr * 2 + hx] = o[c];
+      }
+    }
+  };
+
+  f32x4 acc[9];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  const long long t_begin = (long long)blockIdx.x * tiles_per_block;
+  const long long t_end = min(ntiles, t_begin + tiles_per_block);
+  if (t_begin < t_end) load(t_begin);
+  for (long long t = t_begin; t < t_end; ++t) {
+    store();
+    __syncthreads();
+    if (t + 1 < t_end) load(t + 1);
+#pragma unroll
+    for (int s = 0; s < kWtH / 2; ++s) {
+      const int row = 2 * s + (g >> 1), half = g & 1;
+      const bf16x8 B = __builtin_bit_cast(
+          bf16x8, dt[(tb * 16 + i) * kWtDStride + row * 2 + half]);
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap) {
+        const int ddy = tap / 3 - 1, d = tap % 3;
+        const bf16x8 A = __builtin_bit_cast(
+            bf16x8, xw[(d * 32 + ta * 16 + i) * kWtXStride +
+                       (row + 1 + ddy) * 2 + half]);
+        acc[tap] =
+            __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[tap], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  // bias: the gradient jobs hold per-channel partial sums
+  if (bias && djob) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) atomicAdd(&bsum[cg * 8 + k], bpart[k]);
+  }
+  __syncthreads();
+  if (bias && tid < 32) atomicAdd(&db[co0 + tid], bsum[tid]);
+  // D[ci][co]: lane holds col co = co0 + 16 tb + i, rows ci = ci0 + 16 ta +
+  // 4 g + r
+  const int co = co0 + tb * 16 + i;
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int ci = ci0 + ta * 16 + 4 * g + r;
+      if (ci < cin)
+        atomicAdd(&dw[((long long)tap * cin + ci) * cout + co], acc[tap][r]);
+    }
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -399,6 +743,34 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// relu_grad_kernel on 8 channels per thread (channels, strides and offsets
+// multiples of 8 for y, of 4 for g; p * ch / 8 below 2^31)
+__global__ void __launch_bounds__(kBlock)
+    relu_grad8_kernel(const float* __restrict__ g, int g_stride, int g_off,
+                      const bf16_t* __restrict__ y, int y_stride, int y_off,
+                      int total8, int ch8, bf16_t* __restrict__ out) {
+  for (int id = blockIdx.x * kBlock + threadIdx.x; id < total8;
+       id += gridDim.x * kBlock) {
+    const int q = id / ch8, c = (id - q * ch8) * 8;
+    const u32x4 yv = *reinterpret_cast<const u32x4*>(
+        y + (long long)q * y_stride + y_off + c);
+    const f32x4* gp = reinterpret_cast<const f32x4*>(
+        g + (long long)q * g_stride + g_off + c);
+    const f32x4 g0 = gp[0], g1 = gp[1];
+    u32x4 o;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float ga = j < 2 ? g0[2 * j] : g1[2 * j - 4];
+      const float gb = j < 2 ? g0[2 * j + 1] : g1[2 * j - 3];
+      const float ya = __uint_as_float(yv[j] << 16);
+      const float yb = __uint_as_float(yv[j] & 0xffff0000u);
+      o[j] = (uint32_t)f2bf(ya > 0.0f ? ga : 0.0f) |
+             ((uint32_t)f2bf(yb > 0.0f ? gb : 0.0f) << 16);
+    }
+    *reinterpret_cast<u32x4*>(out + (long long)id * 8) = o;
+  }
+}
+
 __global__ void __launch_bounds__(kBlock)
     head_fwd_kernel(const bf16_t* __restrict__ x, int x_stride, int x_off,
                     long long p, const float* __restrict__ w,
@@ -426,41 +798,68 @@ __global__ void __launch_bounds__(kBlock)
                     long long p, const float* __restrict__ w,
                     const float* __restrict__ dout, float* dw, float* db,
                     float* __restrict__ dx) {
+  // four threads per pixel, eight channels each: one 16-byte load of the
+  // input, two 16-byte stores of its gradient
   __shared__ float ws[32 * 3];
-  __shared__ float red[kBlock];
+  __shared__ float wsum[kBlock / kWave][99];
   for (int k = threadIdx.x; k < 96; k += kBlock) ws[k] = bf2f(f2bf(w[k]));
   __syncthreads();
-  float gw[96];
+  const int c8 = threadIdx.x & 3;
+  float gw[24];
 #pragma unroll
-  for (int k = 0; k < 96; ++k) gw[k] = 0.0f;
+  for (int k = 0; k < 24; ++k) gw[k] = 0.0f;
   float gb[3] = {0.f, 0.f, 0.f};
-  for (long long q = blockIdx.x * (long long)kBlock + threadIdx.x; q < p;
-       q += (long long)gridDim.x * kBlock) {
+  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < 4 * p;
+       id += (long long)gridDim.x * kBlock) {
+    const long long q = id >> 2;
     const float d0 = dout[q * 3 + 0], d1 = dout[q * 3 + 1],
                 d2 = dout[q * 3 + 2];
-    gb[0] += d0;
-    gb[1] += d1;
-    gb[2] += d2;
+    if (c8 == 0) {
+      gb[0] += d0;
+      gb[1] += d1;
+      gb[2] += d2;
+    }
+    const u32x4 xv =
+        *reinterpret_cast<const u32x4*>(x + q * x_stride + x_off + c8 * 8);
+    f32x4 o[2];
 #pragma unroll
-    for (int c = 0; c < 32; ++c) {
-      const float v = bf2f(x[q * x_stride + x_off + c]);
-      gw[c * 3 + 0] += v * d0;
-      gw[c * 3 + 1] += v * d1;
-      gw[c * 3 + 2] += v * d2;
-      dx[q * 32 + c] = d0 * ws[c * 3 + 0] + d1 * ws[c * 3 + 1] +
-                       d2 * ws[c * 3 + 2];
+    for (int k = 0; k < 8; ++k) {
+      const int c = c8 * 8 + k;
+      const float v = __uint_as_float((k & 1) ? (xv[k >> 1] & 0xffff0000u)
+                                              : (xv[k >> 1] << 16));
+      gw[k * 3 + 0] += v * d0;
+      gw[k * 3 + 1] += v * d1;
+      gw[k * 3 + 2] += v * d2;
+      o[k >> 2][k & 3] =
+          d0 * ws[c * 3 + 0] + d1 * ws[c * 3 + 1] + d2 * ws[c * 3 + 2];
+    }
+    f32x4* dp = reinterpret_cast<f32x4*>(dx + q * 32 + c8 * 8);
+    dp[0] = o[0];
+    dp[1] = o[1];
+  }
+  // block sums of the 99 parameter gradients: butterfly sums over the lanes
+  // of a wave that hold the same channels (fixed indices, so the
+  // accumulators stay in registers), the four wave sums through LDS, one
+  // atomic per parameter and block
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    float v = k < 24 ? gw[k] : gb[k - 24];
+#pragma unroll
+    for (int o = kWave / 2; o >= 4; o >>= 1) v += __shfl_xor(v, o);
+    if (lane < 4) {
+      if (k < 24)
+        wsum[wave][(lane * 8 + k / 3) * 3 + k % 3] = v;
+      else if (lane == 0)
+        wsum[wave][96 + k - 24] = v;
     }
   }
-  // block sums of the 99 parameter gradients, one atomic each
-  for (int k = 0; k < 99; ++k) {
-    red[threadIdx.x] = k < 96 ? gw[k] : gb[k - 96];
-    __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd(k < 96 ? &dw[k] : &db[k - 96], red[0]);
-    __syncthreads();
+  __syncthreads();
+  if (threadIdx.x < 99) {
+    float v = 0.0f;
+#pragma unroll
+    for (int q = 0; q < kBlock / kWave; ++q) v += wsum[q][threadIdx.x];
+    atomicAdd(threadIdx.x < 96 ? &dw[threadIdx.x] : &db[threadIdx.x - 96], v);
   }
 }
 
@@ -552,6 +951,19 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
   a.out_f32_stride = out_f32_stride;
   a.accumulate = accumulate;
   const long long P = (long long)n * h * w;
+  if (in.ch % 32 == 0 && cout % 32 == 0) {
+    // LDS-tiled: 8 x 16-pixel tiles, 64 (or 32) output channels per block
+    constexpr int LMT = 2;
+    const unsigned tiles = (unsigned)n * ((h + 4 * LMT - 1) / (4 * LMT)) *
+                           ((w + 15) / 16);
+    if (cout % 64 == 0)
+      hipLaunchKernelGGL((conv3x3_lds_kernel<LMT, 4>), dim3(tiles, cout / 64),
+                         dim3(kBlock), 0, st, a);
+    else
+      hipLaunchKernelGGL((conv3x3_lds_kernel<LMT, 2>), dim3(tiles, cout / 32),
+                         dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+  }
   constexpr int MT = 2;
   if (cout % 32 == 0) {
     constexpr int NT = 2;
@@ -569,6 +981,19 @@ hipError_t conv3x3_wgrad(const Slice& x, int n, int h, int w,
                          hipStream_t st) {
   if (x.ch % 8 || cout % 8 || x.stride % 8 || x.off % 8)
     return hipErrorInvalidValue;
+  if (cout % 32 == 0) {
+    const long long ntiles = (long long)n * ((h + kWtH - 1) / kWtH) *
+                             ((w + kWtW - 1) / kWtW);
+    const int tiles = ((x.ch + 31) / 32) * (cout / 32);
+    // aim at about 1024 blocks in all
+    long long per = (ntiles * tiles + 1023) / 1024;
+    if (per < 1) per = 1;
+    const int xb = (int)((ntiles + per - 1) / per);
+    hipLaunchKernelGGL(wgrad_lds_kernel, dim3(xb, tiles), dim3(kBlock), 0, st,
+                       static_cast<const bf16_t*>(x.base), x.stride, x.off,
+                       x.ch, n, h, w, dy, cout, dw, db, (int)per);
+    return hipGetLastError();
+  }
   const long long P = (long long)n * h * w;
   const long long nchunk = (P + kWgPix - 1) / kWgPix;
   const int tiles = ((x.ch + 31) / 32) * ((cout + 31) / 32);
@@ -645,6 +1070,15 @@ hipError_t up_bwd(const float* dy, int dy_stride, int n, int h, int w, int ch,
 
 hipError_t relu_grad(const float* g, int g_stride, int g_off, const Slice& y,
                      int p, bf16_t* out, hipStream_t st) {
+  const long long total8 = (long long)p * y.ch / 8;
+  if (y.ch % 8 == 0 && y.stride % 8 == 0 && y.off % 8 == 0 &&
+      g_stride % 4 == 0 && g_off % 4 == 0 && total8 < (1LL << 31)) {
+    hipLaunchKernelGGL(relu_grad8_kernel, dim3(blocks_for(total8)),
+                       dim3(kBlock), 0, st, g, g_stride, g_off,
+                       static_cast<const bf16_t*>(y.base), y.stride, y.off,
+                       (int)total8, y.ch / 8, out);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(relu_grad_kernel, dim3(blocks_for((long long)p * y.ch)),
                      dim3(kBlock), 0, st, g, g_stride, g_off,
                      static_cast<const bf16_t*>(y.base), y.stride, y.off,
@@ -662,7 +1096,9 @@ hipError_t head_fwd(const Slice& x, int p, const float* w, const float* b,
 
 hipError_t head_bwd(const Slice& x, int p, const float* w, const float* dout,
                     float* dw, float* db, float* dx, hipStream_t st) {
-  hipLaunchKernelGGL(head_bwd_kernel, dim3(blocks_for(p, 1024)), dim3(kBlock),
+  if (x.ch != 32 || x.stride % 8 || x.off % 8) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(head_bwd_kernel, dim3(blocks_for(4LL * p, 1024)),
+                     dim3(kBlock),
                      0, st, static_cast<const bf16_t*>(x.base), x.stride,
                      x.off, (long long)p, w, dout, dw, db, dx);
   return hipGetLastError();

@@ -232,10 +232,16 @@ def main(argv=None) -> None:
   dt = time.perf_counter() - t0
   after = net.stats()
   flops = after["conv_flops"] - before["conv_flops"]
+  # device: forward + backward between HIP events on the network's stream
+  # (the batch already uploaded); wall: whole train_step calls including
+  # the host-to-device upload of the batch, Adam and the weight packing
+  dev = (after["ms"] - before["ms"]) / 1e3
   print(json.dumps({"batch": b, "size": s, "steps": a.steps,
                     "ms_per_step": dt * 1e3 / a.steps,
-                    "conv_tflops": flops / dt / 1e12,
-                    "mfma_utilisation_vs_2500_tflops": flops / dt / 2.5e15}),
+                    "device_ms_per_step": dev * 1e3 / a.steps,
+                    "conv_tflops": flops / dev / 1e12,
+                    "conv_tflops_wall": flops / dt / 1e12,
+                    "mfma_utilisation_vs_2500_tflops": flops / dev / 2.5e15}),
         flush=True)
 
 
