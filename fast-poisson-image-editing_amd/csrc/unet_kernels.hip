@@ -265,26 +265,62 @@ __global__ void __launch_bounds__(kBlock) conv3x3_lds_kernel(ConvArgs a) {
     __syncthreads();
   }
 
+  // epilogue through LDS (the staging arrays are free after the last
+  // barrier): bias and ReLU applied per lane into a [pixel][channel] fp32
+  // tile, then 16-byte stores of 8 bf16 or 4 fp32 channels per thread
+  constexpr int TS = CT + 4;  // padded row: lane groups on distinct banks
+  static_assert(TH * TW * TS * 4 <= (int)sizeof(ws), "output tile fits");
+  float* T = reinterpret_cast<float*>(&ws[0][0][0]);
 #pragma unroll
-  for (int mt = 0; mt < MT; ++mt) {
-    const int y = row0 + wave * MT + mt;
-    if (y >= a.h) continue;
+  for (int nt = 0; nt < NT; ++nt) {
+    const int co = nt * 16 + i;
+    const float b = a.bias ? a.bias[co0 + co] : 0.0f;
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-      const int co = co0 + nt * 16 + i;
-      const float b = a.bias ? a.bias[co] : 0.0f;
+    for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int x = col0 + 4 * g + r;
-        if (x >= a.w) continue;
-        const long long p = img_base + (long long)y * a.w + x;
         float v = acc[mt][nt][r] + b;
         if (a.relu) v = fmaxf(v, 0.0f);
-        if (a.out_bf) a.out_bf[p * a.out_stride + a.out_off + co] = f2bf(v);
-        if (a.out_f32) {
-          float* o = a.out_f32 + p * a.out_f32_stride + co;
-          *o = a.accumulate ? *o + v : v;
-        }
+        T[((wave * MT + mt) * TW + 4 * g + r) * TS + co] = v;
+      }
+  }
+  __syncthreads();
+  if (a.out_bf) {
+    constexpr int kJobs = TH * TW * (CT / 8);
+#pragma unroll
+    for (int k = 0; k < (kJobs + kBlock - 1) / kBlock; ++k) {
+      const int job = threadIdx.x + k * kBlock;
+      const int pix = job / (CT / 8), c8 = job % (CT / 8);
+      const int y = row0 + pix / TW, x = col0 + pix % TW;
+      if (job < kJobs && y < a.h && x < a.w) {
+        const f32x4 v0 = *reinterpret_cast<const f32x4*>(&T[pix * TS + c8 * 8]);
+        const f32x4 v1 =
+            *reinterpret_cast<const f32x4*>(&T[pix * TS + c8 * 8 + 4]);
+        u32x4 o;
+        o[0] = (uint32_t)f2bf(v0[0]) | ((uint32_t)f2bf(v0[1]) << 16);
+        o[1] = (uint32_t)f2bf(v0[2]) | ((uint32_t)f2bf(v0[3]) << 16);
+        o[2] = (uint32_t)f2bf(v1[0]) | ((uint32_t)f2bf(v1[1]) << 16);
+        o[3] = (uint32_t)f2bf(v1[2]) | ((uint32_t)f2bf(v1[3]) << 16);
+        const long long p = img_base + (long long)y * a.w + x;
+        *reinterpret_cast<u32x4*>(a.out_bf + p * a.out_stride + a.out_off +
+                                  co0 + c8 * 8) = o;
+      }
+    }
+  }
+  if (a.out_f32) {
+    constexpr int kJobs = TH * TW * (CT / 4);
+#pragma unroll
+    for (int k = 0; k < (kJobs + kBlock - 1) / kBlock; ++k) {
+      const int job = threadIdx.x + k * kBlock;
+      const int pix = job / (CT / 4), c4 = job % (CT / 4);
+      const int y = row0 + pix / TW, x = col0 + pix % TW;
+      if (job < kJobs && y < a.h && x < a.w) {
+        f32x4 v = *reinterpret_cast<const f32x4*>(&T[pix * TS + c4 * 4]);
+        const long long p = img_base + (long long)y * a.w + x;
+        f32x4* o = reinterpret_cast<f32x4*>(a.out_f32 + p * a.out_f32_stride +
+                                            co0 + c4 * 4);
+        if (a.accumulate) v += *o;
+        *o = v;
       }
     }
   }
@@ -729,6 +765,84 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// Vector forms of the pooling and upsampling kernels: 8 bf16 or 4 fp32
+// channels per thread with 16-byte accesses and 32-bit index arithmetic
+// (element counts below 2^31; same sums in the same order as above).
+__global__ void __launch_bounds__(kBlock)
+    pool_fwd8_kernel(const bf16_t* __restrict__ in, int in_stride, int in_off,
+                     int h, int w, int cg, int total,
+                     bf16_t* __restrict__ out, int out_stride, int out_off) {
+  const int h2 = h / 2, w2 = w / 2;
+  for (int id = blockIdx.x * kBlock + threadIdx.x; id < total;
+       id += gridDim.x * kBlock) {
+    const int c = (id % cg) * 8, q = id / cg;
+    const int x2 = q % w2, t = q / w2;
+    const int y2 = t % h2, img = t / h2;
+    const long long b = ((long long)img * h + 2 * y2) * w + 2 * x2;
+    const u32x4 v0 = *reinterpret_cast<const u32x4*>(in + b * in_stride + in_off + c);
+    const u32x4 v1 =
+        *reinterpret_cast<const u32x4*>(in + (b + 1) * in_stride + in_off + c);
+    const u32x4 v2 =
+        *reinterpret_cast<const u32x4*>(in + (b + w) * in_stride + in_off + c);
+    const u32x4 v3 = *reinterpret_cast<const u32x4*>(
+        in + (b + w + 1) * in_stride + in_off + c);
+    u32x4 o;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float lo = __uint_as_float(v0[j] << 16) +
+                       __uint_as_float(v1[j] << 16) +
+                       __uint_as_float(v2[j] << 16) +
+                       __uint_as_float(v3[j] << 16);
+      const float hi = __uint_as_float(v0[j] & 0xffff0000u) +
+                       __uint_as_float(v1[j] & 0xffff0000u) +
+                       __uint_as_float(v2[j] & 0xffff0000u) +
+                       __uint_as_float(v3[j] & 0xffff0000u);
+      o[j] = (uint32_t)f2bf(lo * 0.25f) | ((uint32_t)f2bf(hi * 0.25f) << 16);
+    }
+    *reinterpret_cast<u32x4*>(out + (long long)q * out_stride + out_off + c) = o;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    pool_bwd4_kernel(const float* __restrict__ dy, int h2, int w2, int cg,
+                     int total, float* __restrict__ dx, int dx_stride) {
+  const int w = 2 * w2;
+  for (int id = blockIdx.x * kBlock + threadIdx.x; id < total;
+       id += gridDim.x * kBlock) {
+    const int c = (id % cg) * 4, s = id / cg;  // source pixel [n][h2][w2]
+    const int x2 = s % w2, t = s / w2;
+    const int y2 = t % h2, img = t / h2;
+    const f32x4 v = *reinterpret_cast<const f32x4*>(dy + (long long)s * cg * 4 + c) *
+                    0.25f;
+    const long long b = ((long long)img * 2 * h2 + 2 * y2) * w + 2 * x2;
+    const long long q[4] = {b, b + 1, b + w, b + w + 1};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      f32x4* o = reinterpret_cast<f32x4*>(dx + q[k] * dx_stride + c);
+      *o = *o + v;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    up_bwd4_kernel(const float* __restrict__ dy, int dy_stride, int h, int w,
+                   int cg, int total, float* __restrict__ dx) {
+  const int W = 2 * w;
+  for (int id = blockIdx.x * kBlock + threadIdx.x; id < total;
+       id += gridDim.x * kBlock) {
+    const int c = (id % cg) * 4, q = id / cg;
+    const int x = q % w, t = q / w;
+    const int y = t % h, img = t / h;
+    const long long b = ((long long)img * 2 * h + 2 * y) * W + 2 * x;
+    const f32x4 d0 = *reinterpret_cast<const f32x4*>(dy + b * dy_stride + c);
+    const f32x4 d1 = *reinterpret_cast<const f32x4*>(dy + (b + 1) * dy_stride + c);
+    const f32x4 d2 = *reinterpret_cast<const f32x4*>(dy + (b + W) * dy_stride + c);
+    const f32x4 d3 =
+        *reinterpret_cast<const f32x4*>(dy + (b + W + 1) * dy_stride + c);
+    *reinterpret_cast<f32x4*>(dx + (long long)q * cg * 4 + c) = d0 + d1 + d2 + d3;
+  }
+}
+
 __global__ void __launch_bounds__(kBlock)
     relu_grad_kernel(const float* __restrict__ g, int g_stride, int g_off,
                      const bf16_t* __restrict__ y, int y_stride, int y_off,
@@ -951,7 +1065,10 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
   a.out_f32_stride = out_f32_stride;
   a.accumulate = accumulate;
   const long long P = (long long)n * h * w;
-  if (in.ch % 32 == 0 && cout % 32 == 0) {
+  const bool vec_out = (!out_bf || (out_bf->stride % 8 == 0 &&
+                                    out_bf->off % 8 == 0)) &&
+                       (!out_f32 || out_f32_stride % 4 == 0);
+  if (in.ch % 32 == 0 && cout % 32 == 0 && vec_out) {
     // LDS-tiled: 8 x 16-pixel tiles, 64 (or 32) output channels per block
     constexpr int LMT = 2;
     const unsigned tiles = (unsigned)n * ((h + 4 * LMT - 1) / (4 * LMT)) *
@@ -1024,6 +1141,15 @@ hipError_t to_bf16(const float* x, int p, int ch, const Slice& out,
 
 hipError_t pool_fwd(const Slice& in, int n, int h, int w, const Slice& out,
                     hipStream_t st) {
+  const long long total8 = (long long)n * (h / 2) * (w / 2) * (in.ch / 8);
+  if (in.ch % 8 == 0 && in.stride % 8 == 0 && in.off % 8 == 0 &&
+      out.stride % 8 == 0 && out.off % 8 == 0 && total8 < (1LL << 31)) {
+    hipLaunchKernelGGL(pool_fwd8_kernel, dim3(blocks_for(total8)), dim3(kBlock),
+                       0, st, static_cast<const bf16_t*>(in.base), in.stride,
+                       in.off, h, w, in.ch / 8, (int)total8,
+                       static_cast<bf16_t*>(out.base), out.stride, out.off);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(pool_fwd_kernel,
                      dim3(blocks_for((long long)n * h * w / 4 * in.ch)),
                      dim3(kBlock), 0, st, static_cast<const bf16_t*>(in.base),
@@ -1034,6 +1160,14 @@ hipError_t pool_fwd(const Slice& in, int n, int h, int w, const Slice& out,
 
 hipError_t pool_bwd(const float* dy, int n, int h2, int w2, int ch, float* dx,
                     int dx_stride, hipStream_t st) {
+  const long long total4 = (long long)n * h2 * w2 * (ch / 4);
+  if (ch % 4 == 0 && dx_stride % 4 == 0 &&
+      reinterpret_cast<uintptr_t>(dx) % 16 == 0 &&
+      reinterpret_cast<uintptr_t>(dy) % 16 == 0 && total4 < (1LL << 31)) {
+    hipLaunchKernelGGL(pool_bwd4_kernel, dim3(blocks_for(total4)), dim3(kBlock),
+                       0, st, dy, h2, w2, ch / 4, (int)total4, dx, dx_stride);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(pool_bwd_kernel,
                      dim3(blocks_for((long long)n * h2 * w2 * 4 * ch)),
                      dim3(kBlock), 0, st, dy, n, h2, w2, ch, dx, dx_stride);
@@ -1062,6 +1196,14 @@ hipError_t up_fwd(const Slice& in, int n, int h, int w, const Slice& out,
 
 hipError_t up_bwd(const float* dy, int dy_stride, int n, int h, int w, int ch,
                   float* dx, hipStream_t st) {
+  const long long total4 = (long long)n * h * w * (ch / 4);
+  if (ch % 4 == 0 && dy_stride % 4 == 0 &&
+      reinterpret_cast<uintptr_t>(dx) % 16 == 0 &&
+      reinterpret_cast<uintptr_t>(dy) % 16 == 0 && total4 < (1LL << 31)) {
+    hipLaunchKernelGGL(up_bwd4_kernel, dim3(blocks_for(total4)), dim3(kBlock),
+                       0, st, dy, dy_stride, h, w, ch / 4, (int)total4, dx);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(up_bwd_kernel,
                      dim3(blocks_for((long long)n * h * w * ch)),
                      dim3(kBlock), 0, st, dy, dy_stride, n, h, w, ch, dx);
