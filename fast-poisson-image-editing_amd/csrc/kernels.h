@@ -125,13 +125,19 @@ hipError_t launch_interleave(const float* src, float* dst, long long plane,
 // counts[1] += solved pixels on the outermost rows/columns.
 hipError_t launch_mask(const int32_t* src, uint8_t* dst, int H, int W, int P,
                        unsigned long long* counts, hipStream_t st);
+// img[H][W][3] = tgt[H][W][3] at the unsolved pixels of the mask plane.
+hipError_t launch_keep_unsolved(uint8_t* img, const uint8_t* tgt,
+                                const uint8_t* m, int H, int W, int P,
+                                hipStream_t st);
 // GridSolver inputs from uint8 crops src/tgt [H][W][3] and the mask plane:
 // x = tgt, g = sum over up, down, left, right of mixgrad(src_c - src_n,
 // tgt_c - tgt_n) at solved pixels (mode 0 src, 1 avg, 2 max), 0 elsewhere.
+// equ != 0 gives the EquSolver system on the grid instead: x = tgt at solved
+// pixels and 0 (X[0]) elsewhere, g (= B) adds tgt of each unsolved neighbour.
 hipError_t launch_divergence(const uint8_t* src, const uint8_t* tgt,
                              const uint8_t* m, float* x, float* g,
                              long long plane, int H, int W, int P, int mode,
-                             hipStream_t st);
+                             int equ, hipStream_t st);
 // fill the n elements of a three-channel buffer (row pitch P, channel
 // stride plane) with a per-channel constant
 hipError_t launch_fill3(float* dst, long long n, int P, long long plane,

@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(kBlock)
                       const uint8_t* __restrict__ tgt,
                       const uint8_t* __restrict__ m, float* __restrict__ x,
                       float* __restrict__ g, long long plane, int H, int W,
-                      int P, int mode) {
+                      int P, int mode, int equ) {
   const long long n = (long long)H * W;
   for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < n;
        id += (long long)gridDim.x * kBlock) {
@@ -61,10 +61,12 @@ __global__ void __launch_bounds__(kBlock)
     const long long o = (long long)i * P + j;
     const bool solved = m[o] != 0;  // never on the outermost rows/columns
     const long long nb[4] = {id - W, id + W, id - 1, id + 1};  // U, D, L, R
+    const long long nm[4] = {o - P, o + P, o - 1, o + 1};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const float tc = (float)tgt[id * 3 + c];
-      x[c * plane + o] = tc;
+      // equation form: unsolved pixels hold X[0] = 0 (id 0)
+      x[c * plane + o] = (equ && !solved) ? 0.0f : tc;
       float acc = 0.0f;
       if (solved) {
         const float sc = (float)src[id * 3 + c];
@@ -72,8 +74,29 @@ __global__ void __launch_bounds__(kBlock)
         for (int k = 0; k < 4; ++k)
           acc = acc + mixgrad(sc - (float)src[nb[k] * 3 + c],
                               tc - (float)tgt[nb[k] * 3 + c], mode);
+        // equation form: B adds the target of every unsolved neighbour
+        if (equ) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            if (!m[nm[k]]) acc = acc + (float)tgt[nb[k] * 3 + c];
+        }
       }
       g[c * plane + o] = acc;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    keep_unsolved_kernel(uint8_t* __restrict__ img,
+                         const uint8_t* __restrict__ tgt,
+                         const uint8_t* __restrict__ m, int H, int W, int P) {
+  const long long n = (long long)H * W;
+  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < n;
+       id += (long long)gridDim.x * kBlock) {
+    const int i = (int)(id / W), j = (int)(id % W);
+    if (!m[(long long)i * P + j]) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) img[id * 3 + c] = tgt[id * 3 + c];
     }
   }
 }
@@ -633,10 +656,19 @@ hipError_t launch_mask(const int32_t* src, uint8_t* dst, int H, int W, int P,
 hipError_t launch_divergence(const uint8_t* src, const uint8_t* tgt,
                              const uint8_t* m, float* x, float* g,
                              long long plane, int H, int W, int P, int mode,
-                             hipStream_t st) {
+                             int equ, hipStream_t st) {
   hipLaunchKernelGGL(divergence_kernel,
                      dim3(grid_blocks((long long)H * W, 8192)), dim3(kBlock),
-                     0, st, src, tgt, m, x, g, plane, H, W, P, mode);
+                     0, st, src, tgt, m, x, g, plane, H, W, P, mode, equ);
+  return hipGetLastError();
+}
+
+hipError_t launch_keep_unsolved(uint8_t* img, const uint8_t* tgt,
+                                const uint8_t* m, int H, int W, int P,
+                                hipStream_t st) {
+  hipLaunchKernelGGL(keep_unsolved_kernel,
+                     dim3(grid_blocks((long long)H * W, 8192)), dim3(kBlock),
+                     0, st, img, tgt, m, H, W, P);
   return hipGetLastError();
 }
 

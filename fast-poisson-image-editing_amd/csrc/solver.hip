@@ -53,6 +53,9 @@ struct pie_solver {
   int cur = 0;  // which of xa/xb holds the state
   DevBuf<float> xa, xb, g;
   DevBuf<uint8_t> m, img;
+  // uint8 target crop [H][W][3] kept by pie_reset_equ_images for the crop
+  // output of an EquSolver (unsolved pixels keep the target)
+  DevBuf<uint8_t> tgt8;
   DevBuf<double> partial;
   DevBuf<float> err;
   DevBuf<float> coarse;  // restriction / prolongation staging
@@ -95,6 +98,7 @@ void free_problem(pie_solver* s) {
   s->g.release();
   s->m.release();
   s->img.release();
+  s->tgt8.release();
   s->partial.release();
   s->err.release();
   s->coarse.release();
@@ -564,12 +568,68 @@ int pie_reset_grid_images(pie_solver* s, int h, int w, const int32_t* mask,
   PIE_CHECK(upload(src, px * 3, su, s->stream));
   PIE_CHECK(upload(tgt, px * 3, su + px * 3, s->stream));
   PIE_HIP(pie::launch_divergence(su, su + px * 3, s->m0(), s->xa0(), s->g0(),
-                                 s->plane, h, w, s->P, gradient, s->stream));
+                                 s->plane, h, w, s->P, gradient, 0,
+                                 s->stream));
   PIE_HIP(hipMemcpyAsync(s->xb.p, s->xa.p, s->bufsz * sizeof(float),
                          hipMemcpyDeviceToDevice, s->stream));
   PIE_HIP(hipStreamSynchronize(s->stream));
   s->masked = masked;
   s->N = h * w;
+  PIE_CHECK(default_tile(s));
+  s->loaded = true;
+  return PIE_OK;
+}
+
+int pie_reset_equ_images(pie_solver* s, int h, int w, const int32_t* mask,
+                         const uint8_t* src, const uint8_t* tgt,
+                         int gradient) {
+  if (!s || !mask || !src || !tgt) return fail(PIE_EINVAL, "NULL argument");
+  if (s->kind != PIE_EQU) return fail(PIE_EINVAL, "reset_equ on a GridSolver");
+  if (h < 3 || w < 3) return fail(PIE_EINVAL, "mask crop smaller than 3 x 3");
+  if (gradient < PIE_GRAD_SRC || gradient > PIE_GRAD_MAX)
+    return fail(PIE_EINVAL, "gradient must be PIE_GRAD_SRC, _AVG or _MAX");
+  PIE_HIP(hipStreamSynchronize(s->stream));
+  free_problem(s);
+  const size_t px = (size_t)h * w;
+  // fpie/core/gcc/equ.cc:14-32: row-major ids from 1 over mask > 0, as
+  // pie_partition; the system is then held on the grid (id -> i P + j)
+  s->part_ids.assign(px, 0);
+  int cnt = 0;
+  for (size_t k = 0; k < px; ++k) s->part_ids[k] = mask[k] > 0 ? ++cnt : 0;
+  s->ph = h;
+  s->pw = w;
+  s->part_count = cnt;
+  DevBuf<char> stage;  // the int32 mask, then both uint8 images
+  PIE_HIP(stage.alloc(px * 6 > px * 4 ? px * 6 : px * 4));
+  long long masked = 0;
+  PIE_CHECK(load_grid_mask(s, h, w, mask, stage, &masked));
+  const int n = cnt + 1;
+  std::vector<int32_t> lin(n, 0);
+  for (int i = 0; i < h; ++i)
+    for (int j = 0; j < w; ++j) {
+      const int id = s->part_ids[(size_t)i * w + j];
+      if (id) lin[id] = i * s->P + j;
+    }
+  PIE_HIP(s->lin.alloc(n));
+  PIE_CHECK(upload(lin.data(), 4 * (size_t)n, s->lin.p, s->stream));
+  uint8_t* su = reinterpret_cast<uint8_t*>(stage.p);
+  PIE_CHECK(upload(src, px * 3, su, s->stream));
+  PIE_CHECK(upload(tgt, px * 3, su + px * 3, s->stream));
+  PIE_HIP(s->tgt8.alloc(px * 3));
+  PIE_HIP(hipMemcpyAsync(s->tgt8.p, su + px * 3, px * 3,
+                         hipMemcpyDeviceToDevice, s->stream));
+  // fpie/process.py:227-270: X[0] = 0, X[id] = tgt, B = grad + boundary
+  PIE_HIP(pie::launch_fill3(s->xa.p, s->bufsz, s->P, s->plane, 0.0f, 0.0f,
+                            0.0f, s->stream));
+  PIE_HIP(pie::launch_divergence(su, su + px * 3, s->m0(), s->xa0(), s->g0(),
+                                 s->plane, h, w, s->P, gradient, 1,
+                                 s->stream));
+  PIE_HIP(hipMemcpyAsync(s->xb.p, s->xa.p, s->bufsz * sizeof(float),
+                         hipMemcpyDeviceToDevice, s->stream));
+  PIE_HIP(hipStreamSynchronize(s->stream));
+  for (int c = 0; c < 3; ++c) s->x0[c] = 0.0f;
+  s->N = n;
+  s->masked = masked;
   PIE_CHECK(default_tile(s));
   s->loaded = true;
   return PIE_OK;
@@ -594,6 +654,36 @@ int pie_iterate(pie_solver* s, int iterations) {
 int pie_read(pie_solver* s, uint8_t* img, float* err) {
   if (!s) return fail(PIE_EINVAL, "NULL solver");
   return read_outputs(s, img, err, nullptr);
+}
+
+int pie_read_crop(pie_solver* s, uint8_t* crop, float* err) {
+  if (!s || !crop) return fail(PIE_EINVAL, "NULL argument");
+  if (!s->loaded) return fail(PIE_ESTATE, "step before reset");
+  if (s->kind == PIE_GRID) return read_outputs(s, crop, err, nullptr);
+  if (!s->tgt8.p)
+    return fail(PIE_ESTATE, "the crop output needs pie_reset_equ_images");
+  // the clamped state on the grid (fpie/core/gcc/equ.cc:82-85), the target
+  // crop at unsolved pixels, and the residual as pie_read
+  const int nb = pie::finalize_blocks(s->H, s->W);
+  const size_t px = (size_t)s->H * s->W;
+  if (!s->img.p || s->img.n < px * 3) {
+    s->img.release();
+    PIE_HIP(s->img.alloc(px * 3));
+  }
+  PIE_HIP(pie::launch_finalize(s->eff_order(), s->state(), s->g0(), s->m0(),
+                               s->plane, s->H, s->W, s->P, s->img.p,
+                               s->partial.p, nb, s->stream));
+  PIE_HIP(pie::launch_reduce(s->partial.p, nb, s->err.p, s->stream));
+  PIE_HIP(pie::launch_keep_unsolved(s->img.p, s->tgt8.p, s->m0(), s->H, s->W,
+                                    s->P, s->stream));
+  PIE_HIP(hipMemcpyAsync(crop, s->img.p, px * 3, hipMemcpyDeviceToHost,
+                         s->stream));
+  float e[3];
+  PIE_HIP(hipMemcpyAsync(e, s->err.p, sizeof(e), hipMemcpyDeviceToHost,
+                         s->stream));
+  PIE_HIP(hipStreamSynchronize(s->stream));
+  if (err) std::memcpy(err, e, sizeof(e));
+  return PIE_OK;
 }
 
 int pie_step(pie_solver* s, int iterations, uint8_t* img, float* err) {

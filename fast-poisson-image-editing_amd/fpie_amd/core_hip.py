@@ -102,6 +102,10 @@ class _Core:
     return {"h": h.value, "w": w.value, "n": n.value, "masked": masked.value}
 
 
+# mixed-gradient modes of the device resets (fpie/process.py:113-122)
+GRADIENT_MODES = {"src": 0, "avg": 1, "max": 2}
+
+
 class EquSolver(_Core):
   """Equation-form solver (fpie/core/base_solver.h:12-75)."""
 
@@ -110,6 +114,7 @@ class EquSolver(_Core):
   def __init__(self, block_size: int = 1024, schedule: str = "jacobi"):
     super().__init__(schedule, "udlr", block_size, 0)
     self.N = 0
+    self.hw = None  # crop shape after reset_images
 
   def partition(self, mask: np.ndarray) -> np.ndarray:
     m = np.ascontiguousarray(mask, dtype=np.int32)
@@ -132,6 +137,43 @@ class EquSolver(_Core):
     native.check(self._lib.pie_reset_equ(self._h, N, _ptr(A), _ptr(X),
                                          _ptr(B)))
     self.N = N
+    self.hw = None
+
+  def reset_images(self, mask: np.ndarray, src: np.ndarray, tgt: np.ndarray,
+                   gradient: str) -> int:
+    """partition() + reset() built on the device from the uint8 source and
+    target crops [h, w, 3] and the prepared crop mask [h, w] (extension):
+    ids row-major from 1, X = the target, B = the mixed-gradient divergence
+    plus the target of unsolved neighbours, exactly the system that
+    EquProcessor.reset builds on the host (fpie/process.py:180-190,
+    227-270).  Returns N (solved pixels + 1); step() then returns the N x 3
+    values in id order."""
+    m = np.ascontiguousarray(mask, dtype=np.int32)
+    s8 = np.ascontiguousarray(src, dtype=np.uint8)
+    t8 = np.ascontiguousarray(tgt, dtype=np.uint8)
+    if m.ndim != 2 or s8.shape != m.shape + (3,) or t8.shape != m.shape + (3,):
+      raise ValueError("src and tgt must be uint8 [h, w, 3] like the mask")
+    if gradient not in GRADIENT_MODES:
+      raise ValueError(f"gradient must be one of {list(GRADIENT_MODES)}")
+    native.check(self._lib.pie_reset_equ_images(
+        self._h, m.shape[0], m.shape[1], _ptr(m), _ptr(s8), _ptr(t8),
+        GRADIENT_MODES[gradient]))
+    self.N = int(np.count_nonzero(m > 0)) + 1
+    self.hw = m.shape
+    return self.N
+
+  def step_crop(self, iteration: int) -> tuple[np.ndarray, np.ndarray]:
+    """step() after reset_images, with the output as the uint8 crop
+    [h, w, 3] assembled on the device (extension): the clamped values at
+    solved pixels, the target crop elsewhere.  Returns (crop, err[3])."""
+    if self.hw is None:
+      raise RuntimeError("the crop output needs reset_images "
+                         "(pie_reset_equ_images)")
+    native.check(self._lib.pie_iterate(self._h, int(iteration)))
+    crop = np.empty(self.hw + (3,), np.uint8)
+    err = np.empty(3, np.float32)
+    native.check(self._lib.pie_read_crop(self._h, _ptr(crop), _ptr(err)))
+    return crop, err
 
   def step(self, iteration: int) -> tuple[np.ndarray, np.ndarray]:
     img = np.empty((self.N, 3), np.uint8)
@@ -184,7 +226,7 @@ class GridSolver(_Core):
     native.check(self._lib.pie_get_state(self._h, _ptr(out)))
     return out
 
-  GRADIENT_MODES = {"src": 0, "avg": 1, "max": 2}
+  GRADIENT_MODES = GRADIENT_MODES
 
   def reset_images(self, mask: np.ndarray, src: np.ndarray, tgt: np.ndarray,
                    gradient: str) -> None:

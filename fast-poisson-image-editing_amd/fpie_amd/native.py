@@ -54,6 +54,8 @@ _SIGNATURES = {
                                ctypes.POINTER(ctypes.c_longlong),
                                ctypes.POINTER(ctypes.c_double)], _ip),
     "pie_reset_grid_images": ([_vp, _ip, _ip, _vp, _vp, _vp, _ip], _ip),
+"pie_reset_equ_images": ([_vp, _ip, _ip, _vp, _vp, _vp, _ip], _ip),
+    "pie_read_crop": ([_vp, _vp, _vp], _ip),
     "pie_restrict": ([_vp, _ip, _vp], _ip),
     "pie_prolong": ([_vp, _ip, _ip, _ip, _vp], _ip),
     "pie_relax": ([_vp, _ip, ctypes.c_float], _ip),

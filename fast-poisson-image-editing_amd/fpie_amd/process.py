@@ -201,10 +201,29 @@ class EquProcessor(BaseProcessor):
     return ids, xs.size + 1, xs[by_id], ys[by_id]
 
   def reset(self, src, mask, tgt, mask_on_src, mask_on_tgt) -> int:
-    m, x0, _, y0, _ = prepare_mask(mask)
-    ids, n, ix, iy = self.mask2index(m)
+    m, x0, x1, y0, y1 = prepare_mask(mask)
     so = (x0 + mask_on_src[0], y0 + mask_on_src[1])
     to = (x0 + mask_on_tgt[0], y0 + mask_on_tgt[1])
+    if (hasattr(self.core, "reset_images") and src.dtype == np.uint8
+        and tgt.dtype == np.uint8 and src.ndim == 3 and tgt.ndim == 3
+        and min(so + to) >= 0):
+      # the core numbers the pixels and builds X and B on the device from
+      # the uint8 crops: the same system as below (ids row-major, so the
+      # coordinates in id order are np.nonzero's)
+      h, w = m.shape
+      s_crop = src[so[0]:so[0] + h, so[1]:so[1] + w]
+      t_crop = tgt[to[0]:to[0] + h, to[1]:to[1] + w]
+      if s_crop.shape[:2] == (h, w) and t_crop.shape[:2] == (h, w):
+        n = self.core.reset_images(m, s_crop, t_crop, self.gradient)
+        ix, iy = np.nonzero(m)
+        self.tgt = tgt.copy()
+        self.tgt_index = (ix + to[0], iy + to[1])
+        # step() writes the crop the device assembles (a slice copy, not a
+        # scatter through the index arrays)
+        self.crop_window = (to[0], to[0] + h, to[1], to[1] + w)
+        return int(n)
+    self.crop_window = None
+    ids, n, ix, iy = self.mask2index(m)
     sx, sy = ix + so[0], iy + so[1]
     tx, ty = ix + to[0], iy + to[1]
 
@@ -235,6 +254,11 @@ class EquProcessor(BaseProcessor):
     return n
 
   def step(self, iteration: int):
+    if getattr(self, "crop_window", None) is not None:
+      crop, err = self.core.step_crop(iteration)
+      x0, x1, y0, y1 = self.crop_window
+      self.tgt[x0:x1, y0:y1] = crop
+      return self.tgt, err
     x, err = self.core.step(iteration)
     self.tgt[self.tgt_index] = x[1:]
     return self.tgt, err

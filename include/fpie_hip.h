@@ -111,6 +111,27 @@ int pie_reset_grid_images(pie_solver* s, int h, int w, const int32_t* mask,
                           const uint8_t* src, const uint8_t* tgt,
                           int gradient);
 
+/* EquSolver.partition + EquSolver.reset built on the device from the uint8
+ * crops: what EquProcessor.reset (fpie/process.py:180-190, 227-270)
+ * computes on the host and passes to pie_partition / pie_reset_equ.  Ids
+ * are row-major from 1 over mask > 0 (fpie/core/gcc/equ.cc:14-32),
+ * N = count + 1, X[0] = 0, X[id] = tgt, B[id] = the mixed-gradient sum of
+ * pie_reset_grid_images plus the target of every unsolved up, left, right,
+ * down neighbour.  Integer and half-integer values: identical to the host
+ * system bit for bit, for every schedule.  pie_step then returns N x 3
+ * values in id order. */
+int pie_reset_equ_images(pie_solver* s, int h, int w, const int32_t* mask,
+                         const uint8_t* src, const uint8_t* tgt,
+                         int gradient);
+
+/* The output of EquProcessor.step (fpie/process.py:273-280: the solved
+ * values clamped and truncated to uint8, fpie/core/gcc/equ.cc:82-85,
+ * scattered into the target) as the h x w x 3 crop, assembled on the
+ * device: the clamped state at solved pixels, the target crop elsewhere;
+ * err as pie_step.  Needs pie_reset_equ_images (which keeps the target
+ * crop); on a GridSolver it is pie_read. */
+int pie_read_crop(pie_solver* s, uint8_t* crop, float* err);
+
 /* sync() (base_solver.h:61): waits for queued work; no data moves. */
 int pie_sync(pie_solver* s);
 
