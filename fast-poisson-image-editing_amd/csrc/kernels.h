@@ -199,4 +199,41 @@ hipError_t launch_equ_finalize(const int4* A, const float4* B,
                                float* state, double* partial, int nblocks,
                                hipStream_t st);
 
+// ---- coarse-grid algebra of the learned warm start (coarse.hip) ----
+// Fields double [hc][wc][3]; mc uint8 [hc][wc] with an unsolved outer ring.
+constexpr int kMaxDots = 4;
+struct CoarseDots {  // per-channel sums of a[q] * b[q], q < count
+  const double* a[kMaxDots];
+  const double* b[kMaxDots];
+  int count;
+};
+int coarse_dots_blocks(int n);
+// r = rc (float) on solved cells, 0 elsewhere; c = 0
+hipError_t launch_coarse_load(const float* rc, const uint8_t* mc, int n,
+                              double* r, double* c, hipStream_t st);
+// network input x [hp][wp][8]: r / sig in channels 0..2, the mask in 6
+hipError_t launch_net_input(const double* r, const uint8_t* mc, int hc,
+                            int wc, int hp, int wp, const double sig[3],
+                            float* x, hipStream_t st);
+// z = network output [hp][wp][3] on solved cells
+hipError_t launch_net_output(const float* out, const uint8_t* mc, int hc,
+                             int wc, int wp, double* z, hipStream_t st);
+// one damped (weight 1/2) sweep of the homogeneous coarse problem
+hipError_t launch_lowpass(const double* q, const uint8_t* mc, int hc, int wc,
+                          double* out, hipStream_t st);
+// out = A_c p = 4 p - sum of the four neighbours, on solved cells
+hipError_t launch_coarse_apply(const double* p, const uint8_t* mc, int hc,
+                               int wc, double* out, hipStream_t st);
+// partial[block][q][3] for blocks < coarse_dots_blocks(n)
+hipError_t launch_coarse_dots(const CoarseDots& d, int n, double* partial,
+                              hipStream_t st);
+// y += a_c x
+hipError_t launch_coarse_axpy(double* y, const double* x, const double a[3],
+                              int n, hipStream_t st);
+hipError_t launch_coarse_to_float(const double* c, int n3, float* out,
+                                  hipStream_t st);
+// v = float(double(v) * a_c); *maxbits = max(*maxbits, bits of max |v|)
+hipError_t launch_coarse_scale(float* v, const double a[3], int n,
+                               unsigned* maxbits, hipStream_t st);
+
 }  // namespace pie

@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <initializer_list>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -59,6 +61,15 @@ struct pie_solver {
   DevBuf<double> partial;
   DevBuf<float> err;
   DevBuf<float> coarse;  // restriction / prolongation staging
+  // learned coarse correction (pie_warm_net_start): coarse mask, double
+  // fields (residual, correction, smoothing scratch, directions p_k and
+  // A_c p_k), network input / output, dot-product partials; warm_factor is
+  // the factor of the correction left in `coarse` (0: none)
+  DevBuf<uint8_t> wmc;
+  DevBuf<double> wf, wpart;
+  DevBuf<float> wx, wout;
+  DevBuf<unsigned> wmax;
+  int warm_factor = 0;
 
   // EquSolver
   int N = 0;
@@ -102,6 +113,13 @@ void free_problem(pie_solver* s) {
   s->partial.release();
   s->err.release();
   s->coarse.release();
+  s->wmc.release();
+  s->wf.release();
+  s->wpart.release();
+  s->wx.release();
+  s->wout.release();
+  s->wmax.release();
+  s->warm_factor = 0;
   s->lin.release();
   s->A4.release();
   s->B4.release();
@@ -787,18 +805,14 @@ int pie_prolong(pie_solver* s, int factor, int hc, int wc, const float* corr) {
   return PIE_OK;
 }
 
-int pie_prolong_dots(pie_solver* s, int factor, int hc, int wc,
-                     const float* corr, double* out) {
-  int ec = 0, ew = 0;
-  if (!corr || !out) return fail(PIE_EINVAL, "NULL argument");
-  PIE_CHECK(coarse_grid(s, factor, &ec, &ew));
-  if (hc != ec || wc != ew)
-    return fail(PIE_EINVAL, "coarse grid must be (h // factor, w // factor)");
+namespace {
+
+// pie_prolong_dots of the correction in s->coarse
+int prolong_dots_device(pie_solver* s, int factor, int hc, int wc,
+                        double* out) {
   const int nb = pie::prolong_dots_blocks(s->H, s->W);
   DevBuf<double> part;
   PIE_HIP(part.alloc((size_t)nb * 6));
-  PIE_CHECK(upload(corr, (size_t)hc * wc * 3 * sizeof(float), s->coarse.p,
-                   s->stream));
   PIE_HIP(pie::launch_prolong_dots(s->state(), s->g0(), s->m0(), s->plane,
                                    s->H, s->W, s->P, factor, hc, wc,
                                    s->coarse.p, part.p, nb, s->stream));
@@ -809,6 +823,184 @@ int pie_prolong_dots(pie_solver* s, int factor, int hc, int wc,
   for (int k = 0; k < 6; ++k) out[k] = 0.0;
   for (int b = 0; b < nb; ++b)  // fixed order: deterministic
     for (int k = 0; k < 6; ++k) out[k] += h[(size_t)b * 6 + k];
+  return PIE_OK;
+}
+
+// Per-channel inner products of up to kMaxDots pairs of coarse fields
+// (blocks added on the host in order: deterministic).
+int coarse_dots(pie_solver* s, int n, std::initializer_list<
+                std::pair<const double*, const double*>> pairs,
+                double (*out)[3]) {
+  pie::CoarseDots d{};
+  d.count = 0;
+  for (const auto& pr : pairs) {
+    d.a[d.count] = pr.first;
+    d.b[d.count] = pr.second;
+    ++d.count;
+  }
+  const int nb = pie::coarse_dots_blocks(n);
+  PIE_HIP(pie::launch_coarse_dots(d, n, s->wpart.p, s->stream));
+  std::vector<double> h((size_t)nb * pie::kMaxDots * 3);
+  PIE_HIP(hipMemcpyAsync(h.data(), s->wpart.p, h.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, s->stream));
+  PIE_HIP(hipStreamSynchronize(s->stream));
+  for (int q = 0; q < d.count; ++q)
+    for (int c = 0; c < 3; ++c) {
+      double v = 0.0;
+      for (int b = 0; b < nb; ++b)
+        v += h[((size_t)b * pie::kMaxDots + q) * 3 + c];
+      out[q][c] = v;
+    }
+  return PIE_OK;
+}
+
+}  // namespace
+
+int pie_prolong_dots(pie_solver* s, int factor, int hc, int wc,
+                     const float* corr, double* out) {
+  int ec = 0, ew = 0;
+  if (!corr || !out) return fail(PIE_EINVAL, "NULL argument");
+  PIE_CHECK(coarse_grid(s, factor, &ec, &ew));
+  if (hc != ec || wc != ew)
+    return fail(PIE_EINVAL, "coarse grid must be (h // factor, w // factor)");
+  s->warm_factor = 0;
+  PIE_CHECK(upload(corr, (size_t)hc * wc * 3 * sizeof(float), s->coarse.p,
+                   s->stream));
+  return prolong_dots_device(s, factor, hc, wc, out);
+}
+
+int pie_warm_net_start(pie_solver* s, pie_unet* net, int factor,
+                       const uint8_t* mc, int net_steps, int lowpass_sweeps,
+                       float* corr_out) {
+  constexpr int kMaxSteps = 8;
+  int hc = 0, wc = 0;
+  if (!net || !mc) return fail(PIE_EINVAL, "NULL argument");
+  if (net_steps < 0 || net_steps > kMaxSteps)
+    return fail(PIE_EINVAL, "net_steps must be in [0, 8]");
+  if (lowpass_sweeps < 0) return fail(PIE_EINVAL, "negative lowpass_sweeps");
+  PIE_CHECK(coarse_grid(s, factor, &hc, &wc));
+  const int n = hc * wc;
+  const int hp = (hc + 7) / 8 * 8, wp = (wc + 7) / 8 * 8;
+  const size_t fs = (size_t)n * 3;  // doubles per field
+  // fields: r, c, t, then p_k and A_c p_k for each step
+  const size_t nfields = 3 + 2 * (size_t)kMaxSteps;
+  if (s->wf.n < nfields * fs) PIE_HIP(s->wf.alloc(nfields * fs));
+  if (s->wmc.n < (size_t)n) PIE_HIP(s->wmc.alloc(n));
+  const size_t np = (size_t)pie::coarse_dots_blocks(n) * pie::kMaxDots * 3;
+  if (s->wpart.n < np) PIE_HIP(s->wpart.alloc(np));
+  if (s->wx.n < (size_t)hp * wp * 8) PIE_HIP(s->wx.alloc((size_t)hp * wp * 8));
+  if (s->wout.n < (size_t)hp * wp * 3) PIE_HIP(s->wout.alloc((size_t)hp * wp * 3));
+  double* r = s->wf.p;
+  double* c = r + fs;
+  double* t = c + fs;
+  auto P = [&](int k) { return t + fs * (1 + 2 * (size_t)k); };
+  auto AP = [&](int k) { return t + fs * (2 + 2 * (size_t)k); };
+  int cells = 0;
+  for (int k = 0; k < n; ++k) cells += mc[k] != 0;
+
+  s->warm_factor = 0;
+  PIE_CHECK(upload(mc, (size_t)n, s->wmc.p, s->stream));
+  // r_c: the restricted residual on the solved cells
+  PIE_HIP(pie::launch_restrict(s->state(), s->g0(), s->m0(), s->plane, s->P,
+                               factor, hc, wc, s->coarse.p, s->stream));
+  PIE_HIP(pie::launch_coarse_load(s->coarse.p, s->wmc.p, n, r, c, s->stream));
+  double pap[kMaxSteps][3];  // <p_k, A_c p_k> per channel (1 if it vanished)
+  for (int k = 0; k < net_steps; ++k) {
+    double* z = P(k);
+    double* az = AP(k);
+    // network direction for the current residual (zero if r is zero)
+    double d[pie::kMaxDots][3];
+    PIE_CHECK(coarse_dots(s, n, {{r, r}}, d));
+    double sig[3];
+    bool any = false;
+    for (int ch = 0; ch < 3; ++ch) {
+      sig[ch] = cells ? std::sqrt(d[0][ch] / cells) : 0.0;
+      any = any || sig[ch] > 0.0;
+      if (!(sig[ch] > 0.0)) sig[ch] = 1.0;
+    }
+    if (any) {
+      PIE_HIP(pie::launch_net_input(r, s->wmc.p, hc, wc, hp, wp, sig, s->wx.p,
+                                    s->stream));
+      PIE_HIP(hipStreamSynchronize(s->stream));
+      PIE_CHECK(pie_unet_forward_device(net, hp, wp, s->wx.p, s->wout.p));
+      PIE_HIP(pie::launch_net_output(s->wout.p, s->wmc.p, hc, wc, wp, z,
+                                     s->stream));
+    } else {
+      PIE_HIP(hipMemsetAsync(z, 0, fs * sizeof(double), s->stream));
+    }
+    // low-pass: damped homogeneous sweeps (z -> t -> z)
+    for (int q = 0; q < lowpass_sweeps; ++q) {
+      PIE_HIP(pie::launch_lowpass(z, s->wmc.p, hc, wc, t, s->stream));
+      PIE_HIP(hipMemcpyAsync(z, t, fs * sizeof(double),
+                             hipMemcpyDeviceToDevice, s->stream));
+    }
+    // A_c-orthogonal to the earlier directions, one after the other
+    for (int j = 0; j < k; ++j) {
+      PIE_CHECK(coarse_dots(s, n, {{z, AP(j)}}, d));
+      const double a[3] = {-(d[0][0] / pap[j][0]), -(d[0][1] / pap[j][1]),
+                           -(d[0][2] / pap[j][2])};
+      PIE_HIP(pie::launch_coarse_axpy(z, P(j), a, n, s->stream));
+    }
+    PIE_HIP(pie::launch_coarse_apply(z, s->wmc.p, hc, wc, az, s->stream));
+    // <z, A z>, <z, z>, <r, z>: the energy-optimal step along z
+    PIE_CHECK(coarse_dots(s, n, {{z, az}, {z, z}, {r, z}}, d));
+    double a[3], na[3];
+    for (int ch = 0; ch < 3; ++ch) {
+      const bool ok = d[0][ch] > 1e-30 * std::max(d[1][ch], 1e-300);
+      a[ch] = ok ? d[2][ch] / d[0][ch] : 0.0;
+      na[ch] = -a[ch];
+      pap[k][ch] = ok ? d[0][ch] : 1.0;
+    }
+    PIE_HIP(pie::launch_coarse_axpy(c, z, a, n, s->stream));
+    PIE_HIP(pie::launch_coarse_axpy(r, az, na, n, s->stream));
+  }
+  PIE_HIP(pie::launch_coarse_to_float(c, (int)fs, s->coarse.p, s->stream));
+  if (corr_out)
+    PIE_HIP(hipMemcpyAsync(corr_out, s->coarse.p, fs * sizeof(float),
+                           hipMemcpyDeviceToHost, s->stream));
+  PIE_HIP(hipStreamSynchronize(s->stream));
+  s->warm_factor = factor;
+  return PIE_OK;
+}
+
+int pie_warm_apply(pie_solver* s, int factor, const float* corr,
+                   double* alpha_out, double* max_corr) {
+  int hc = 0, wc = 0;
+  PIE_CHECK(coarse_grid(s, factor, &hc, &wc));
+  const size_t fs = (size_t)hc * wc * 3;
+  if (corr) {
+    PIE_CHECK(upload(corr, fs * sizeof(float), s->coarse.p, s->stream));
+  } else if (s->warm_factor != factor) {
+    return fail(PIE_ESTATE, "no device correction of this factor: call "
+                            "pie_warm_net_start first or pass corr");
+  }
+  s->warm_factor = 0;
+  // line search: alpha_c = max(0, <r, d> / <d, A d>)
+  double dots[6];
+  PIE_CHECK(prolong_dots_device(s, factor, hc, wc, dots));
+  double alpha[3];
+  for (int ch = 0; ch < 3; ++ch) {
+    const double den = dots[3 + ch];
+    alpha[ch] = den > 0.0 ? dots[ch] / den : 0.0;
+    alpha[ch] = std::max(alpha[ch], 0.0);
+  }
+  if (!s->wmax.p) PIE_HIP(s->wmax.alloc(1));
+  PIE_HIP(hipMemsetAsync(s->wmax.p, 0, sizeof(unsigned), s->stream));
+  PIE_HIP(pie::launch_coarse_scale(s->coarse.p, alpha, hc * wc, s->wmax.p,
+                                   s->stream));
+  PIE_HIP(pie::launch_prolong(s->state(), s->m0(), s->plane, s->H, s->W, s->P,
+                              factor, hc, wc, s->coarse.p, s->stream));
+  unsigned mb = 0;
+  PIE_HIP(hipMemcpyAsync(&mb, s->wmax.p, sizeof(mb), hipMemcpyDeviceToHost,
+                         s->stream));
+  PIE_HIP(hipStreamSynchronize(s->stream));
+  if (alpha_out)
+    for (int ch = 0; ch < 3; ++ch) alpha_out[ch] = alpha[ch];
+  if (max_corr) {
+    float m;
+    std::memcpy(&m, &mb, sizeof(m));
+    *max_corr = m;
+  }
   return PIE_OK;
 }
 

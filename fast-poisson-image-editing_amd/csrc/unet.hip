@@ -381,6 +381,21 @@ int pie_unet_forward(pie_unet* u, const float* x, float* out) {
   return PIE_OK;
 }
 
+int pie_unet_forward_device(pie_unet* u, int h, int w, const float* x,
+                            float* out) {
+  if (!u || !x || !out) return fail(PIE_EINVAL, "NULL argument");
+  if (u->B != 1 || u->H != h || u->W != w)
+    return fail(PIE_EINVAL, "network shape differs from (1, h, w)");
+  const long long P0 = u->P[0];
+  PIE_HIP(hipMemcpyAsync(u->x32.p, x, P0 * 8 * sizeof(float),
+                         hipMemcpyDeviceToDevice, u->stream));
+  PIE_CHECK(forward(u));
+  PIE_HIP(hipMemcpyAsync(out, u->out.p, P0 * 3 * sizeof(float),
+                         hipMemcpyDeviceToDevice, u->stream));
+  PIE_HIP(hipStreamSynchronize(u->stream));
+  return PIE_OK;
+}
+
 int pie_unet_loss_grad(pie_unet* u, const float* x, const float* target,
                        const uint8_t* mask, float* loss, float* grads) {
   if (!u || !x || !target || !mask) return fail(PIE_EINVAL, "NULL argument");

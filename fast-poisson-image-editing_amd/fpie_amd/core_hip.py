@@ -303,6 +303,44 @@ class GridSolver(_Core):
                                             _ptr(corr), _ptr(out)))
     return out.reshape(2, 3)
 
+  def warm_net_start(self, factor: int, mc: np.ndarray, net,
+                     net_steps: int = 2, lowpass_sweeps: int = 4,
+                     want: bool = False):
+    """The learned coarse correction on the device (extension,
+    pie_warm_net_start): restrict the residual, then `net_steps` flexible
+    conjugate-gradient steps over the network's low-passed directions.
+    mc: bool [hc, wc] coarse mask; net: a callable returning the batch-1
+    fpie_amd.unet.UNet of the padded coarse shape.  The correction stays on
+    the device for warm_apply(); with want=True a float32 copy
+    [hc, wc, 3] is returned too."""
+    hc, wc = self.coarse_shape(factor)
+    m = np.ascontiguousarray(mc, dtype=np.uint8)
+    if m.shape != (hc, wc):
+      raise ValueError(f"coarse mask must be {(hc, wc)}, not {m.shape}")
+    u = net()
+    corr = np.empty((hc, wc, 3), np.float32) if want else None
+    native.check(self._lib.pie_warm_net_start(
+        self._h, u._h, int(factor), _ptr(m), int(net_steps),
+        int(lowpass_sweeps), _ptr(corr) if want else None))
+    return corr
+
+  def warm_apply(self, factor: int, corr: np.ndarray | None = None):
+    """state += alpha * prolong(corr) with the per-channel energy-optimal
+    alpha >= 0 (extension, pie_warm_apply); corr None uses the correction
+    warm_net_start left on the device.  Returns (alpha [3], max |alpha
+    corr|)."""
+    c = None
+    if corr is not None:
+      c = np.ascontiguousarray(corr, dtype=np.float32)
+      if c.shape != self.coarse_shape(factor) + (3,):
+        raise ValueError("correction must be [hc, wc, 3] of the factor")
+    alpha = np.zeros(3, np.float64)
+    size = ctypes.c_double()
+    native.check(self._lib.pie_warm_apply(self._h, int(factor),
+                                          _ptr(c) if c is not None else None,
+                                          _ptr(alpha), ctypes.byref(size)))
+    return alpha, size.value
+
   def relax(self, sweeps: int = 1, omega: float = 0.5) -> None:
     """`sweeps` weighted Jacobi sweeps x <- x + omega ((g + sum) / 4 - x)."""
     native.check(self._lib.pie_relax(self._h, int(sweeps), float(omega)))

@@ -60,12 +60,15 @@ _SIGNATURES = {
     "pie_prolong": ([_vp, _ip, _ip, _ip, _vp], _ip),
     "pie_relax": ([_vp, _ip, ctypes.c_float], _ip),
     "pie_prolong_dots": ([_vp, _ip, _ip, _ip, _vp, _vp], _ip),
+    "pie_warm_net_start": ([_vp, _vp, _ip, _vp, _ip, _ip, _vp], _ip),
+    "pie_warm_apply": ([_vp, _ip, _vp, _vp, _vp], _ip),
     "pie_unet_create": ([_ip, _ip, _ip, ctypes.POINTER(_vp)], _ip),
     "pie_unet_destroy": ([_vp], _ip),
     "pie_unet_param_count": ([_vp, ctypes.POINTER(ctypes.c_longlong)], _ip),
     "pie_unet_set_params": ([_vp, _vp], _ip),
     "pie_unet_get_params": ([_vp, _vp], _ip),
     "pie_unet_forward": ([_vp, _vp, _vp], _ip),
+    "pie_unet_forward_device": ([_vp, _ip, _ip, _vp, _vp], _ip),
     "pie_unet_loss_grad": ([_vp, _vp, _vp, _vp,
                             ctypes.POINTER(ctypes.c_float), _vp], _ip),
     "pie_unet_train_step": ([_vp, _vp, _vp, _vp, ctypes.c_float,

@@ -11,8 +11,11 @@ the residual waits for the checkerboard modes.  A start state that is only
 smooth-accurate (an upsampled prediction) therefore gains nothing; it has to
 be free of both.
 
-Algorithm (device work through the GridSolver calls restrict / prolong /
-relax / iterate; the coarse problem is itself a GridSolver problem):
+Algorithm (all on the device: GridSolver.iterate / relax smooth,
+GridSolver.warm_net_start computes a cycle's coarse correction --
+restriction, network and coarse algebra, pie_warm_net_start -- and
+GridSolver.warm_apply the line search and prolongation; the optional polish
+solves the coarse problem with a second GridSolver):
 
   smooth: 24 f^2 plain Jacobi sweeps, then one weighted sweep with
           omega 1/2 (the mean of two consecutive Jacobi iterates), which
@@ -78,7 +81,7 @@ from .core_hip import BatchSolver, GridSolver
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
                        "warmstart_unet.npz")
 
-# ------------------------------------------------------- coarse algebra ---
+# ---------------------------------------------------------- coarse mask ---
 
 
 def coarse_mask(mask: np.ndarray, f: int) -> np.ndarray:
@@ -90,47 +93,6 @@ def coarse_mask(mask: np.ndarray, f: int) -> np.ndarray:
   mc[[0, -1], :] = False
   mc[:, [0, -1]] = False
   return mc
-
-
-def signed_residual(mask, x, grad) -> np.ndarray:
-  """float64 [h, w, 3]: g + four neighbours - 4 x at solved pixels."""
-  t = np.asarray(x, np.float64)
-  r = np.zeros_like(t)
-  r[1:-1, 1:-1] = (np.asarray(grad, np.float64)[1:-1, 1:-1] + t[:-2, 1:-1] +
-                   t[2:, 1:-1] + t[1:-1, :-2] + t[1:-1, 2:] -
-                   4.0 * t[1:-1, 1:-1])
-  return r * (np.asarray(mask)[..., None] != 0)
-
-
-def block_sum(a: np.ndarray, f: int) -> np.ndarray:
-  h, w = (a.shape[0] // f) * f, (a.shape[1] // f) * f
-  return a[:h, :w].reshape(h // f, f, w // f, f, -1).sum((1, 3))
-
-
-def coarse_apply(mc: np.ndarray, p: np.ndarray) -> np.ndarray:
-  """A_c p: 4 p - sum of the solved neighbours' p, on the solved cells."""
-  q = np.where(mc[..., None], p, 0.0).astype(np.float64)
-  z = np.pad(q, ((1, 1), (1, 1), (0, 0)))
-  out = 4.0 * q - (z[:-2, 1:-1] + z[2:, 1:-1] + z[1:-1, :-2] + z[1:-1, 2:])
-  return out * mc[..., None]
-
-
-def lowpass(mc: np.ndarray, p: np.ndarray, sweeps: int = 4) -> np.ndarray:
-  """`sweeps` damped Jacobi sweeps (weight 1/2) of the homogeneous coarse
-  problem (p = 0 off the solved cells): keeps the smooth part of p and
-  removes its cell-scale oscillation."""
-  m = mc[..., None]
-  q = np.where(m, p, 0.0).astype(np.float64)
-  for _ in range(sweeps):
-    z = np.pad(q, ((1, 1), (1, 1), (0, 0)))
-    avg = 0.25 * (z[:-2, 1:-1] + z[2:, 1:-1] + z[1:-1, :-2] + z[1:-1, 2:])
-    q = np.where(m, 0.5 * (q + avg), 0.0)
-  return q
-
-
-def _dot(a: np.ndarray, b: np.ndarray) -> np.ndarray:
-  """Per-channel inner products of [h, w, 3] fields (float64)."""
-  return np.einsum("ijc,ijc->c", a, b, dtype=np.float64)
 
 
 def mean_residual(solver: GridSolver) -> float:
@@ -184,50 +146,30 @@ class WarmStart:
       self._nets[key] = unet.UNet(1, key[0], key[1], params=self.params)
     return self._nets[key]
 
-  def predict(self, rc: np.ndarray, mc: np.ndarray) -> np.ndarray:
-    """Network output [hc, wc, 3] for the coarse residual rc (each channel
-    scaled to unit rms over the solved cells; output scale arbitrary)."""
-    hc, wc = mc.shape
-    sig = np.sqrt(np.mean(rc[mc] ** 2, axis=0)) if mc.any() else np.zeros(3)
-    if not (sig > 0).any():
-      return np.zeros((hc, wc, 3), np.float32)
-    net = self._net(hc, wc)
-    x = np.zeros(net.shape[1:] + (8,), np.float32)
-    x[:hc, :wc, 0:3] = rc / np.where(sig > 0, sig, 1.0)
-    x[:hc, :wc, 6] = mc
-    return net.forward(x[None])[0, :hc, :wc] * mc[..., None]
-
-  def net_start(self, rc: np.ndarray, mc: np.ndarray) -> np.ndarray:
-    """Learned start of the coarse solve: `net_steps` flexible
-    conjugate-gradient steps over low-passed network directions (the coarse
-    error's energy norm never grows, whatever the network returns)."""
-    m = mc[..., None]
-    r = np.asarray(rc, np.float64) * m
-    c = np.zeros_like(r)
-    dirs = []  # (p, A p, <p, A p>)
-    for _ in range(self.net_steps):
-      z = lowpass(mc, self.predict(r, mc), self.lowpass_sweeps) * m
-      for p, ap, pap in dirs:
-        z -= (_dot(z, ap) / pap) * p
-      az = coarse_apply(mc, z)
-      pap = _dot(z, az)
-      ok = pap > 1e-30 * np.maximum(_dot(z, z), 1e-300)
-      a = np.where(ok, _dot(r, z) / np.where(ok, pap, 1.0), 0.0)
-      c += a * z
-      r -= a * az
-      dirs.append((z, az, np.where(ok, pap, 1.0)))
-    return c.astype(np.float32)
-
-  def coarse_solve(self, rc: np.ndarray, mc: np.ndarray, cs=None):
-    """Coarse correction: the learned start, then (if `coarse_rtol` is set)
-    the coarse GridSolver `cs` (omega-1/2 sweep and plain Jacobi to
-    `coarse_rtol`).  Returns (c float32 [hc, wc, 3], info)."""
+  def coarse_solve(self, solver: GridSolver, f: int, mc: np.ndarray,
+                   cs=None):
+    """Coarse correction for the state of `solver` (factor f, coarse mask
+    mc): the learned start on the device (GridSolver.warm_net_start), then,
+    if `coarse_rtol` is set, the coarse GridSolver `cs` polish (one omega
+    1/2 sweep and plain Jacobi to `coarse_rtol` times the zero start's mean
+    residual).  Returns (c, info): c None when the correction stayed on
+    the device, else float32 [hc, wc, 3]."""
     hc, wc = mc.shape
     use_net = self.precond == "net" and self.net_steps > 0
-    c0 = (self.net_start(rc, mc) if use_net
-          else np.zeros((hc, wc, 3), np.float32))
+
+    def net():
+      return self._net(hc, wc)
+
     if self.coarse_rtol is None:
-      return c0, {"coarse_sweeps": 0}
+      if not use_net:
+        return np.zeros((hc, wc, 3), np.float32), {"coarse_sweeps": 0}
+      solver.warm_net_start(f, mc, net, self.net_steps, self.lowpass_sweeps)
+      return None, {"coarse_sweeps": 0}
+    rc = solver.restrict(f)
+    rc[~mc] = 0.0
+    c0 = (solver.warm_net_start(f, mc, net, self.net_steps,
+                                self.lowpass_sweeps, want=True)
+          if use_net else None)
     if cs is None:
       cs = self.coarse_solver()
     cs.reset(0, mc.astype(np.int32), np.zeros((hc, wc, 3), np.float32),
@@ -261,17 +203,10 @@ class WarmStart:
     steps = []
     r_prev = mean_residual(solver)
     for k in range(self.cycles):
-      rc = solver.restrict(f)
-      rc[~mc] = 0.0
-      cc, info = self.coarse_solve(rc, mc, cs)
-      num, den = solver.prolong_dots(f, cc)
-      alpha = np.where(den > 0, num / np.where(den > 0, den, 1.0), 0.0)
-      alpha = np.maximum(alpha, 0.0)
-      cc = (cc * alpha).astype(np.float32)
-      solver.prolong(f, cc)
+      cc, info = self.coarse_solve(solver, f, mc, cs)
+      alpha, size = solver.warm_apply(f, cc)
       smooth()
       r_now = mean_residual(solver)
-      size = float(np.abs(cc).max()) if cc.size else 0.0
       steps.append({"max_corr": size, "alpha": [float(a) for a in alpha],
                     **info, "mean_residual": r_now})
       if log:

@@ -179,6 +179,8 @@ int pie_solve_to_residual(pie_solver* s, double tol_mean, int check_every,
                           long long max_sweeps, long long* sweeps,
                           double* mean_residual);
 
+typedef struct pie_unet pie_unet;
+
 /* ---- coarse-grid correction (GridSolver warm start): no counterpart in
  * the reference.  The coarse grid of `factor` is (h // factor) x
  * (w // factor); block (I, J) covers rows I*factor.., cols J*factor.. */
@@ -199,6 +201,27 @@ int pie_prolong_dots(pie_solver* s, int factor, int hc, int wc,
 /* `sweeps` weighted Jacobi sweeps x <- x + omega * ((g + sum) / 4 - x),
  * 0 < omega <= 1 (omega 1/2: the mean of two consecutive Jacobi iterates). */
 int pie_relax(pie_solver* s, int sweeps, float omega);
+/* The learned coarse correction on the device.  Restricts the residual
+ * (pie_restrict), zeroes it off the coarse mask mc uint8[hc][wc] (outer
+ * ring unsolved), then takes `net_steps` (<= 8) flexible conjugate-gradient
+ * steps for A_c c = r_c (A_c the 5-point operator on mc) whose directions
+ * are the network's answers to the current coarse residual r (input:
+ * channels 0..2 = r per channel over its rms on mc, channel 6 = mc),
+ * smoothed by `lowpass_sweeps` damped (weight 1/2) homogeneous sweeps,
+ * A_c-orthogonalised against the earlier directions and taken with the
+ * energy-optimal step; all in double.  net: a batch-1 pie_unet of shape
+ * (hc rounded up to 8, wc rounded up to 8).  The float correction stays in
+ * the solver for pie_warm_apply; corr_out (may be NULL) receives a copy. */
+int pie_warm_net_start(pie_solver* s, pie_unet* net, int factor,
+                       const uint8_t* mc, int net_steps, int lowpass_sweeps,
+                       float* corr_out);
+/* state += alpha_c * (bilinear prolongation of the correction), alpha_c =
+ * max(0, out[c] / out[3 + c]) of pie_prolong_dots, with corr float[hc][wc][3]
+ * from the host, or NULL for the one pie_warm_net_start left on the device
+ * (an error if there is none for this factor).  alpha_out[3] receives
+ * alpha, max_corr the largest |alpha_c corr| (as float). */
+int pie_warm_apply(pie_solver* s, int factor, const float* corr,
+                   double* alpha_out, double* max_corr);
 
 /* ---- batched GridSolver: no counterpart in the reference ---------------
  * Many composites of uneven size and mask, each with its own stop rule,
@@ -232,13 +255,12 @@ int pie_batch_size(pie_batch* b, int* items, int* tasks, long long* masked);
 
 /* ---- warm-start U-Net: no counterpart in the reference -----------------
  * 4-level U-Net (base 32 channels, 3x3 convolutions, bf16 weights and
- * activations, fp32 accumulation on MFMA) predicting the solution's
- * correction from 8 input channels: divergence / 64, (target - 128) / 64,
- * mask, 0.  Tensors are NHWC float32 on the host: x [B][H][W][8], target
+ * activations, fp32 accumulation on MFMA) mapping 8 input channels to 3
+ * (the warm start: coarse residual scaled per channel in 0..2, coarse mask
+ * in 6, the rest 0 -> a coarse correction direction).  Tensors are NHWC float32 on the host: x [B][H][W][8], target
  * and output [B][H][W][3], mask uint8 [B][H][W].  Parameters: one float32
  * array (pie_unet_param_count), per conv layer w[tap][cin][cout] then
  * b[cout], then the 1x1 head w[32][3], b[3]. */
-typedef struct pie_unet pie_unet;
 /* batch >= 1; h, w multiples of 8.  Parameters start at zero. */
 int pie_unet_create(int batch, int h, int w, pie_unet** out);
 int pie_unet_destroy(pie_unet* u);
@@ -246,6 +268,11 @@ int pie_unet_param_count(pie_unet* u, long long* n);
 int pie_unet_set_params(pie_unet* u, const float* p);
 int pie_unet_get_params(pie_unet* u, float* p);
 int pie_unet_forward(pie_unet* u, const float* x, float* out);
+/* pie_unet_forward on device memory (no counterpart in the reference): x
+ * and out are device pointers of a batch-1 network of shape (h, w); the
+ * call returns when out is written. */
+int pie_unet_forward_device(pie_unet* u, int h, int w, const float* x,
+                            float* out);
 /* Masked mean squared error over solved pixels and its parameter gradient
  * (grads may be NULL). */
 int pie_unet_loss_grad(pie_unet* u, const float* x, const float* target,

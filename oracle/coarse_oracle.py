@@ -16,6 +16,12 @@ GridSolver system that the tests use as the converged solution:
 
 for every solved pixel p (the fixed point of the update in
 fpie/core/gcc/grid.cc:21-36).
+
+The learned coarse correction (pie_warm_net_start / pie_warm_apply) is
+restated in float64 numpy too: the network input, the damped homogeneous
+smoothing of its directions, the 5-point coarse operator, and the flexible
+conjugate-gradient steps with a caller-supplied `predict` (the tests pass
+the device network's forward pass, an exact coarse solve or noise).
 """
 
 from __future__ import annotations
@@ -140,3 +146,80 @@ def exact_solution(mask, tgt, grad) -> np.ndarray:
   for c in range(3):
     x[ii, jj, c] = lu.solve(b[:, c])
   return x
+
+
+# ------------------------------------------- learned coarse correction ---
+
+
+def coarse_apply(mc: np.ndarray, p: np.ndarray) -> np.ndarray:
+  """A_c p: 4 p - (up + down + left + right) on the solved cells."""
+  q = np.where(mc[..., None], p, 0.0).astype(np.float64)
+  z = np.pad(q, ((1, 1), (1, 1), (0, 0)))
+  out = 4.0 * q - (z[:-2, 1:-1] + z[2:, 1:-1] + z[1:-1, :-2] + z[1:-1, 2:])
+  return out * mc[..., None]
+
+
+def lowpass(mc: np.ndarray, p: np.ndarray, sweeps: int = 4) -> np.ndarray:
+  """`sweeps` damped sweeps q <- (q + (up + down + left + right) / 4) / 2 of
+  the homogeneous coarse problem (q = 0 off the solved cells)."""
+  m = mc[..., None]
+  q = np.where(m, p, 0.0).astype(np.float64)
+  for _ in range(sweeps):
+    z = np.pad(q, ((1, 1), (1, 1), (0, 0)))
+    avg = 0.25 * (z[:-2, 1:-1] + z[2:, 1:-1] + z[1:-1, :-2] + z[1:-1, 2:])
+    q = np.where(m, 0.5 * (q + avg), 0.0)
+  return q
+
+
+def dot(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+  """Per-channel inner products of [h, w, 3] fields (float64)."""
+  return np.einsum("ijc,ijc->c", a, b, dtype=np.float64)
+
+
+def net_input(r: np.ndarray, mc: np.ndarray, hp: int, wp: int):
+  """Network input [1, hp, wp, 8] float32 for the coarse residual r:
+  channels 0..2 = r over its per-channel rms on the solved cells, channel 6
+  = the mask; None when r vanishes."""
+  hc, wc = mc.shape
+  sig = np.sqrt(np.mean(r[mc] ** 2, axis=0)) if mc.any() else np.zeros(3)
+  if not (sig > 0).any():
+    return None
+  x = np.zeros((1, hp, wp, 8), np.float32)
+  x[0, :hc, :wc, 0:3] = r / np.where(sig > 0, sig, 1.0)
+  x[0, :hc, :wc, 6] = mc
+  return x
+
+
+def net_start(rc: np.ndarray, mc: np.ndarray, predict, net_steps: int = 2,
+              lowpass_sweeps: int = 4) -> np.ndarray:
+  """The learned coarse correction (float32 [hc, wc, 3]) for the coarse
+  residual rc: `net_steps` flexible CG steps whose directions are
+  predict(r, mc) low-passed, A_c-orthogonalised one after the other against
+  the earlier ones, and taken with the energy-optimal step."""
+  m = mc[..., None]
+  r = np.asarray(rc, np.float64) * m
+  c = np.zeros_like(r)
+  dirs = []  # (p, A p, <p, A p>)
+  for _ in range(net_steps):
+    z = lowpass(mc, predict(r, mc), lowpass_sweeps) * m
+    for p, ap, pap in dirs:
+      z -= (dot(z, ap) / pap) * p
+    az = coarse_apply(mc, z)
+    pap = dot(z, az)
+    ok = pap > 1e-30 * np.maximum(dot(z, z), 1e-300)
+    a = np.where(ok, dot(r, z) / np.where(ok, pap, 1.0), 0.0)
+    c += a * z
+    r -= a * az
+    dirs.append((z, az, np.where(ok, pap, 1.0)))
+  return c.astype(np.float32)
+
+
+def warm_apply(mask, x, grad, corr, f: int):
+  """x + alpha prolong(corr), alpha = max(0, <r, d> / <d, A d>) per
+  channel; returns (new x, alpha, max |alpha corr|)."""
+  num, den = prolong_dots(mask, x, grad, corr, f)
+  alpha = np.where(den > 0, num / np.where(den > 0, den, 1.0), 0.0)
+  alpha = np.maximum(alpha, 0.0)
+  cc = (np.asarray(corr, np.float32) * alpha).astype(np.float32)
+  size = float(np.abs(cc).max()) if cc.size else 0.0
+  return prolong(mask, x, cc, f), alpha, size

@@ -3,6 +3,16 @@
 pie_restrict is checked to float32 rounding of the block sums (the kernel
 and the oracle both add float64 terms, in different orders), pie_prolong
 and pie_relax bit for bit (same float32 operations, no contraction).
+
+pie_warm_net_start against the oracle's restatement driven by the same
+network (the packaged weights, forward pass on the host path of the same
+library): the two differ only where the per-channel rms of the residual,
+summed in another order, moves a network input across a bf16 rounding
+boundary, so the corrections agree to 2e-3 of their size; the device
+correction never raises the coarse error's energy norm (exact coarse
+solution by a sparse direct solve).  pie_warm_apply against the oracle's
+line search and prolongation of the same correction (alpha from double
+sums in another order: to 1e-6).
 """
 
 import numpy as np
@@ -103,3 +113,89 @@ def test_coarse_calls_reject_bad_arguments(hip):
   out = np.zeros(64 * 3, np.float32)
   with pytest.raises(RuntimeError):
     native.check(e._lib.pie_restrict(e._h, 2, out.ctypes.data))
+
+
+def _net_problem(hip, size=256, f=4, seed=300):
+  from fpie_amd import synth, unet, warmstart
+  mask, tgt, grad, _ = synth.composite((size, size), seed)
+  s = hip.GridSolver(schedule="jacobi")
+  s.reset(0, mask, tgt, grad)
+  s.iterate(24 * f * f)
+  s.relax(1, 0.5)
+  mc = warmstart.coarse_mask(mask, f)
+  hc, wc = mc.shape
+  net = unet.UNet(1, (hc + 7) // 8 * 8, (wc + 7) // 8 * 8,
+                  params=warmstart.load_params())
+  return s, mask, grad, mc, net
+
+
+def _predict(net, mc):
+  hc, wc = mc.shape
+  hp, wp = net.shape[1:]
+
+  def predict(r, m):
+    x = co.net_input(r, m, hp, wp)
+    if x is None:
+      return np.zeros((hc, wc, 3), np.float32)
+    return net.forward(x)[0, :hc, :wc] * m[..., None]
+
+  return predict
+
+
+@pytest.mark.parametrize("steps,lowpass", [(1, 0), (2, 4), (3, 2)])
+def test_warm_net_start_matches_oracle(hip, steps, lowpass):
+  print(f"warm_net_start steps {steps} lowpass {lowpass}", flush=True)
+  s, mask, grad, mc, net = _net_problem(hip)
+  f = 4
+  rc = s.restrict(f)
+  rc[~mc] = 0.0
+  want = co.net_start(rc, mc, _predict(net, mc), steps, lowpass)
+  got = s.warm_net_start(f, mc, lambda: net, steps, lowpass, want=True)
+  scale = float(np.abs(want).max())
+  assert scale > 0
+  err = float(np.abs(got - want).max())
+  print(f"max |device - oracle| = {err:.3g} of {scale:.3g}", flush=True)
+  assert err <= 2e-3 * scale
+  assert not got[~mc].any()
+  # the energy norm of the coarse error never grows
+  e = co.exact_solution(mc, np.zeros(mc.shape + (3,)), rc)
+
+  def energy(v):
+    return co.dot(v, co.coarse_apply(mc, v))
+
+  assert (energy(e - got) < energy(e)).all()
+
+
+def test_warm_apply_matches_oracle(hip):
+  print("warm_apply", flush=True)
+  s, mask, grad, mc, net = _net_problem(hip, seed=301)
+  f = 4
+  x = s.state().copy()
+  corr = s.warm_net_start(f, mc, lambda: net, 2, 4, want=True)
+  alpha, size = s.warm_apply(f)  # the correction left on the device
+  want_x, want_alpha, want_size = co.warm_apply(mask, x, grad, corr, f)
+  np.testing.assert_allclose(alpha, want_alpha, rtol=1e-6)
+  assert abs(size - want_size) <= 1e-6 * want_size
+  np.testing.assert_allclose(s.state(), want_x, rtol=1e-6, atol=1e-4)
+  # a host correction takes the same path
+  s.set_state(x)
+  alpha2, size2 = s.warm_apply(f, corr)
+  np.testing.assert_array_equal(alpha2, alpha)
+  assert size2 == size
+
+
+def test_warm_calls_reject_bad_input(hip):
+  from fpie_amd import unet
+  s, mask, grad, mc, net = _net_problem(hip, size=128, seed=302)
+  with pytest.raises(RuntimeError, match="no device correction"):
+    s.warm_apply(4)  # nothing left by warm_net_start
+  wrong = unet.UNet(1, 64, 64)
+  with pytest.raises(RuntimeError, match="network shape"):
+    s.warm_net_start(4, mc, lambda: wrong, 1, 0)
+  with pytest.raises(RuntimeError, match="net_steps"):
+    s.warm_net_start(4, mc, lambda: net, 9, 0)
+  with pytest.raises(ValueError):
+    s.warm_net_start(4, mc[:-1], lambda: net, 1, 0)
+  s.warm_net_start(4, mc, lambda: net, 1, 0)
+  with pytest.raises(RuntimeError, match="no device correction"):
+    s.warm_apply(8)  # the device correction is of factor 4

@@ -47,12 +47,19 @@ def test_fallback_restores_the_target_state(ws_mod, monkeypatch):
   rng = np.random.default_rng(0)
   monkeypatch.setattr(
       ws, "coarse_solve",
-      lambda rc, mc, cs=None: ((1e4 * rng.normal(0, 1, rc.shape) *
-                                mc[..., None]).astype(np.float32), {
-                                    "coarse_sweeps": 0}))
+      lambda solver, f, mc, cs=None: ((1e4 * rng.normal(0, 1, mc.shape + (3,))
+                                       * mc[..., None]).astype(np.float32),
+                                      {"coarse_sweeps": 0}))
   s = _solver(mask, tgt, grad)
   # and a forced huge step past the line search
-  s.prolong_dots = lambda f, c: np.array([[1e4] * 3, [1.0] * 3])
+  prolong = s.prolong
+
+  def bad_apply(f, cc=None):
+    c = (cc * 1e4).astype(np.float32)
+    prolong(f, c)
+    return np.full(3, 1e4), float(np.abs(c).max())
+
+  s.warm_apply = bad_apply
   info = ws.run(s, mask, tgt)
   assert not info["used"]
   assert np.array_equal(s.state(), tgt)
