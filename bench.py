@@ -186,7 +186,6 @@ def bench_warm_start(size: int = 1024, seed: int = 900_001,
   def solver():
     s = GridSolver(schedule="jacobi")
     s.reset(0, mask, tgt, grad)
-    s.set_tile(8, 1, 64)
     s.sync()
     return s
 
