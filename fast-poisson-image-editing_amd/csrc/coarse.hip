@@ -45,8 +45,9 @@ __global__ void __launch_bounds__(kBlock)
 __global__ void __launch_bounds__(kBlock)
     net_input_kernel(const double* __restrict__ r,
                      const uint8_t* __restrict__ mc, int hc, int wc, int hp,
-                     int wp, double s0, double s1, double s2,
+                     int wp, const double* __restrict__ sig,
                      float* __restrict__ x) {
+  const double s0 = sig[0], s1 = sig[1], s2 = sig[2];
   for (int id = blockIdx.x * kBlock + threadIdx.x; id < hp * wp;
        id += gridDim.x * kBlock) {
     const int i = id / wp, j = id % wp;
@@ -61,15 +62,18 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// z = the network output [hp][wp][3] on the solved cells
+// z = the network output [hp][wp][3] on the solved cells; 0 everywhere
+// when *any is 0 (the residual vanished: no direction)
 __global__ void __launch_bounds__(kBlock)
     net_output_kernel(const float* __restrict__ out,
                       const uint8_t* __restrict__ mc, int hc, int wc, int wp,
+                      const double* __restrict__ any,
                       double* __restrict__ z) {
+  const bool use = *any != 0.0;
   for (int k = blockIdx.x * kBlock + threadIdx.x; k < hc * wc;
        k += gridDim.x * kBlock) {
     const int i = k / wc, j = k % wc;
-    const bool on = mc[k] != 0;
+    const bool on = use && mc[k] != 0;
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch)
       z[k * 3 + ch] =
@@ -122,7 +126,8 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Per-channel inner products of up to kMaxDots field pairs: block partial
-// sums partial[block][pair][3] (the host adds the blocks in order).
+// sums partial[block][pair][3] (sum_partials_kernel adds the blocks in
+// order).
 __global__ void __launch_bounds__(kBlock)
     coarse_dots_kernel(CoarseDots d, int n, double* __restrict__ partial) {
   double acc[kMaxDots][3];
@@ -161,10 +166,11 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// y += a_c x
+// y += a_c x (a: three doubles in device memory)
 __global__ void __launch_bounds__(kBlock)
     coarse_axpy_kernel(double* __restrict__ y, const double* __restrict__ x,
-                       double a0, double a1, double a2, int n) {
+                       const double* __restrict__ a, int n) {
+  const double a0 = a[0], a1 = a[1], a2 = a[2];
   for (int k = blockIdx.x * kBlock + threadIdx.x; k < n;
        k += gridDim.x * kBlock) {
     y[k * 3 + 0] = y[k * 3 + 0] + a0 * x[k * 3 + 0];
@@ -184,8 +190,10 @@ __global__ void __launch_bounds__(kBlock)
 // v = float(double(v) * a_c); *maxbits = max |v| as float bits (atomicMax
 // on the bits of non-negative floats orders them as the floats)
 __global__ void __launch_bounds__(kBlock)
-    coarse_scale_kernel(float* __restrict__ v, double a0, double a1,
-                        double a2, int n, unsigned* __restrict__ maxbits) {
+    coarse_scale_kernel(float* __restrict__ v,
+                        const double* __restrict__ alpha, int n,
+                        unsigned* __restrict__ maxbits) {
+  const double a0 = alpha[0], a1 = alpha[1], a2 = alpha[2];
   unsigned m = 0;
   for (int k = blockIdx.x * kBlock + threadIdx.x; k < n;
        k += gridDim.x * kBlock) {
@@ -203,7 +211,73 @@ __global__ void __launch_bounds__(kBlock)
   if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicMax(maxbits, m);
 }
 
+// out[k] = sum over blocks b (in order) of partial[b * width + k]
+__global__ void __launch_bounds__(kWave)
+    sum_partials_kernel(const double* __restrict__ partial, int nb, int width,
+                        double* __restrict__ out) {
+  const int k = threadIdx.x;
+  if (k >= width) return;
+  double v = 0.0;
+  for (int b = 0; b < nb; ++b) v += partial[(long long)b * width + k];
+  out[k] = v;
+}
+
+// The scalar steps of the flexible CG on the device (one thread), on the
+// scalars of WarmScalars (see kernels.h)
+__global__ void warm_scalar_kernel(double* __restrict__ sc, int op, int k,
+                                   double cells) {
+  if (threadIdx.x != 0) return;
+  const double* d = sc + kWsDots;
+  if (op == kWarmSig) {
+    // per-channel rms of r over the solved cells (1 where it vanishes);
+    // any = whether some channel is non-zero
+    bool any = false;
+    for (int ch = 0; ch < 3; ++ch) {
+      double v = cells > 0.0 ? sqrt(d[ch] / cells) : 0.0;
+      any = any || v > 0.0;
+      sc[kWsSig + ch] = v > 0.0 ? v : 1.0;
+    }
+    sc[kWsAny] = any ? 1.0 : 0.0;
+  } else if (op == kWarmOrth) {
+    // coefficient removing the A_c-component along direction k
+    for (int ch = 0; ch < 3; ++ch)
+      sc[kWsCoef + ch] = -(d[ch] / sc[kWsPap + 3 * k + ch]);
+  } else if (op == kWarmStep) {
+    // <z, A z>, <z, z>, <r, z> -> the energy-optimal step along z
+    for (int ch = 0; ch < 3; ++ch) {
+      const double zaz = d[ch], zz = d[3 + ch], rz = d[6 + ch];
+      const bool ok = zaz > 1e-30 * fmax(zz, 1e-300);
+      const double a = ok ? rz / zaz : 0.0;
+      sc[kWsStep + ch] = a;
+      sc[kWsNegStep + ch] = -a;
+      sc[kWsPap + 3 * k + ch] = ok ? zaz : 1.0;
+    }
+  } else if (op == kWarmAlpha) {
+    // fine line search: alpha = max(0, <r, d> / <d, A d>)
+    for (int ch = 0; ch < 3; ++ch) {
+      const double den = d[3 + ch];
+      const double a = den > 0.0 ? d[ch] / den : 0.0;
+      sc[kWsAlpha + ch] = fmax(a, 0.0);
+    }
+  }
+}
+
 }  // namespace
+
+hipError_t launch_sum_partials(const double* partial, int nb, int width,
+                               double* out, hipStream_t st) {
+  if (width > kWave) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kWave), 0, st,
+                     partial, nb, width, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_warm_scalar(double* sc, int op, int k, double cells,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(warm_scalar_kernel, dim3(1), dim3(kWave), 0, st, sc, op,
+                     k, cells);
+  return hipGetLastError();
+}
 
 int coarse_dots_blocks(int n) { return blocks_for(n, 256); }
 
@@ -215,18 +289,18 @@ hipError_t launch_coarse_load(const float* rc, const uint8_t* mc, int n,
 }
 
 hipError_t launch_net_input(const double* r, const uint8_t* mc, int hc,
-                            int wc, int hp, int wp, const double sig[3],
+                            int wc, int hp, int wp, const double* sig,
                             float* x, hipStream_t st) {
   hipLaunchKernelGGL(net_input_kernel, dim3(blocks_for((long long)hp * wp, 4096)),
-                     dim3(kBlock), 0, st, r, mc, hc, wc, hp, wp, sig[0],
-                     sig[1], sig[2], x);
+                     dim3(kBlock), 0, st, r, mc, hc, wc, hp, wp, sig, x);
   return hipGetLastError();
 }
 
 hipError_t launch_net_output(const float* out, const uint8_t* mc, int hc,
-                             int wc, int wp, double* z, hipStream_t st) {
+                             int wc, int wp, const double* any, double* z,
+                             hipStream_t st) {
   hipLaunchKernelGGL(net_output_kernel, dim3(blocks_for((long long)hc * wc, 4096)),
-                     dim3(kBlock), 0, st, out, mc, hc, wc, wp, z);
+                     dim3(kBlock), 0, st, out, mc, hc, wc, wp, any, z);
   return hipGetLastError();
 }
 
@@ -252,10 +326,10 @@ hipError_t launch_coarse_dots(const CoarseDots& d, int n, double* partial,
   return hipGetLastError();
 }
 
-hipError_t launch_coarse_axpy(double* y, const double* x, const double a[3],
+hipError_t launch_coarse_axpy(double* y, const double* x, const double* a,
                               int n, hipStream_t st) {
   hipLaunchKernelGGL(coarse_axpy_kernel, dim3(blocks_for(n, 4096)),
-                     dim3(kBlock), 0, st, y, x, a[0], a[1], a[2], n);
+                     dim3(kBlock), 0, st, y, x, a, n);
   return hipGetLastError();
 }
 
@@ -266,10 +340,10 @@ hipError_t launch_coarse_to_float(const double* c, int n3, float* out,
   return hipGetLastError();
 }
 
-hipError_t launch_coarse_scale(float* v, const double a[3], int n,
+hipError_t launch_coarse_scale(float* v, const double* alpha, int n,
                                unsigned* maxbits, hipStream_t st) {
   hipLaunchKernelGGL(coarse_scale_kernel, dim3(blocks_for(n, 4096)),
-                     dim3(kBlock), 0, st, v, a[0], a[1], a[2], n, maxbits);
+                     dim3(kBlock), 0, st, v, alpha, n, maxbits);
   return hipGetLastError();
 }
 

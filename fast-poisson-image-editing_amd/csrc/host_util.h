@@ -90,4 +90,10 @@ inline int upload(const void* host, size_t bytes, void* dev, hipStream_t st) {
 }
 
 }  // namespace host
+
+// unet.hip: the warm-start network's forward pass on device buffers,
+// stream-ordered against `st` (no host wait)
+int unet_forward_on_stream(pie_unet* u, int h, int w, const float* x,
+                           float* out, hipStream_t st);
+
 }  // namespace pie

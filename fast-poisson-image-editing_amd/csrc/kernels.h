@@ -208,16 +208,37 @@ struct CoarseDots {  // per-channel sums of a[q] * b[q], q < count
   int count;
 };
 int coarse_dots_blocks(int n);
+// Device scalars of the learned coarse correction (doubles, offsets):
+// dot results (kMaxDots x 3), sig (3), any (1), the orthogonalisation
+// coefficient (3), the step and its negative (3 + 3), the fine line-search
+// alpha (3), <p_k, A p_k> per step (kWarmMaxSteps x 3).
+constexpr int kWarmMaxSteps = 8;
+constexpr int kWsDots = 0, kWsSig = kWsDots + kMaxDots * 3, kWsAny = kWsSig + 3,
+              kWsCoef = kWsAny + 1, kWsStep = kWsCoef + 3,
+              kWsNegStep = kWsStep + 3, kWsAlpha = kWsNegStep + 3,
+              kWsPap = kWsAlpha + 3, kWsCount = kWsPap + 3 * kWarmMaxSteps;
+constexpr int kWarmSig = 0, kWarmOrth = 1, kWarmStep = 2, kWarmAlpha = 3;
+// out[k] = sum over b < nb, in order, of partial[b * width + k] (width <= 64)
+hipError_t launch_sum_partials(const double* partial, int nb, int width,
+                               double* out, hipStream_t st);
+// one scalar step on sc (op kWarmSig: sig/any from the dots of (r, r) and
+// `cells`; kWarmOrth: coefficient from the dot (z, A p_k) and pap_k;
+// kWarmStep: step and pap_k from (z, Az), (z, z), (r, z); kWarmAlpha:
+// alpha from the six line-search sums)
+hipError_t launch_warm_scalar(double* sc, int op, int k, double cells,
+                              hipStream_t st);
 // r = rc (float) on solved cells, 0 elsewhere; c = 0
 hipError_t launch_coarse_load(const float* rc, const uint8_t* mc, int n,
                               double* r, double* c, hipStream_t st);
 // network input x [hp][wp][8]: r / sig in channels 0..2, the mask in 6
+// (sig: three doubles in device memory)
 hipError_t launch_net_input(const double* r, const uint8_t* mc, int hc,
-                            int wc, int hp, int wp, const double sig[3],
+                            int wc, int hp, int wp, const double* sig,
                             float* x, hipStream_t st);
-// z = network output [hp][wp][3] on solved cells
+// z = network output [hp][wp][3] on solved cells (0 if *any is 0)
 hipError_t launch_net_output(const float* out, const uint8_t* mc, int hc,
-                             int wc, int wp, double* z, hipStream_t st);
+                             int wc, int wp, const double* any, double* z,
+                             hipStream_t st);
 // one damped (weight 1/2) sweep of the homogeneous coarse problem
 hipError_t launch_lowpass(const double* q, const uint8_t* mc, int hc, int wc,
                           double* out, hipStream_t st);
@@ -227,13 +248,14 @@ hipError_t launch_coarse_apply(const double* p, const uint8_t* mc, int hc,
 // partial[block][q][3] for blocks < coarse_dots_blocks(n)
 hipError_t launch_coarse_dots(const CoarseDots& d, int n, double* partial,
                               hipStream_t st);
-// y += a_c x
-hipError_t launch_coarse_axpy(double* y, const double* x, const double a[3],
+// y += a_c x (a: three doubles in device memory)
+hipError_t launch_coarse_axpy(double* y, const double* x, const double* a,
                               int n, hipStream_t st);
 hipError_t launch_coarse_to_float(const double* c, int n3, float* out,
                                   hipStream_t st);
-// v = float(double(v) * a_c); *maxbits = max(*maxbits, bits of max |v|)
-hipError_t launch_coarse_scale(float* v, const double a[3], int n,
+// v = float(double(v) * alpha_c); *maxbits = max(*maxbits, bits of
+// max |v|) (alpha: three doubles in device memory)
+hipError_t launch_coarse_scale(float* v, const double* alpha, int n,
                                unsigned* maxbits, hipStream_t st);
 
 }  // namespace pie

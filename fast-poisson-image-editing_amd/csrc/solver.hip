@@ -66,7 +66,7 @@ struct pie_solver {
   // A_c p_k), network input / output, dot-product partials; warm_factor is
   // the factor of the correction left in `coarse` (0: none)
   DevBuf<uint8_t> wmc;
-  DevBuf<double> wf, wpart;
+  DevBuf<double> wf, wpart, wsc;  // fields, dot partials, scalars
   DevBuf<float> wx, wout;
   DevBuf<unsigned> wmax;
   int warm_factor = 0;
@@ -116,6 +116,7 @@ void free_problem(pie_solver* s) {
   s->wmc.release();
   s->wf.release();
   s->wpart.release();
+  s->wsc.release();
   s->wx.release();
   s->wout.release();
   s->wmax.release();
@@ -835,11 +836,10 @@ int prolong_dots_device(pie_solver* s, int factor, int hc, int wc,
   return PIE_OK;
 }
 
-// Per-channel inner products of up to kMaxDots pairs of coarse fields
-// (blocks added on the host in order: deterministic).
+// Per-channel inner products of up to kMaxDots pairs of coarse fields into
+// the device scalars sc[kWsDots..] (blocks added in order: deterministic).
 int coarse_dots(pie_solver* s, int n, std::initializer_list<
-                std::pair<const double*, const double*>> pairs,
-                double (*out)[3]) {
+                std::pair<const double*, const double*>> pairs) {
   pie::CoarseDots d{};
   d.count = 0;
   for (const auto& pr : pairs) {
@@ -849,17 +849,8 @@ int coarse_dots(pie_solver* s, int n, std::initializer_list<
   }
   const int nb = pie::coarse_dots_blocks(n);
   PIE_HIP(pie::launch_coarse_dots(d, n, s->wpart.p, s->stream));
-  std::vector<double> h((size_t)nb * pie::kMaxDots * 3);
-  PIE_HIP(hipMemcpyAsync(h.data(), s->wpart.p, h.size() * sizeof(double),
-                         hipMemcpyDeviceToHost, s->stream));
-  PIE_HIP(hipStreamSynchronize(s->stream));
-  for (int q = 0; q < d.count; ++q)
-    for (int c = 0; c < 3; ++c) {
-      double v = 0.0;
-      for (int b = 0; b < nb; ++b)
-        v += h[((size_t)b * pie::kMaxDots + q) * 3 + c];
-      out[q][c] = v;
-    }
+  PIE_HIP(pie::launch_sum_partials(s->wpart.p, nb, pie::kMaxDots * 3,
+                                   s->wsc.p + pie::kWsDots, s->stream));
   return PIE_OK;
 }
 
@@ -881,7 +872,7 @@ int pie_prolong_dots(pie_solver* s, int factor, int hc, int wc,
 int pie_warm_net_start(pie_solver* s, pie_unet* net, int factor,
                        const uint8_t* mc, int net_steps, int lowpass_sweeps,
                        float* corr_out) {
-  constexpr int kMaxSteps = 8;
+  constexpr int kMaxSteps = pie::kWarmMaxSteps;
   int hc = 0, wc = 0;
   if (!net || !mc) return fail(PIE_EINVAL, "NULL argument");
   if (net_steps < 0 || net_steps > kMaxSteps)
@@ -895,8 +886,13 @@ int pie_warm_net_start(pie_solver* s, pie_unet* net, int factor,
   const size_t nfields = 3 + 2 * (size_t)kMaxSteps;
   if (s->wf.n < nfields * fs) PIE_HIP(s->wf.alloc(nfields * fs));
   if (s->wmc.n < (size_t)n) PIE_HIP(s->wmc.alloc(n));
-  const size_t np = (size_t)pie::coarse_dots_blocks(n) * pie::kMaxDots * 3;
+  // partials of the coarse dots here and of the line search in
+  // pie_warm_apply (sized for both: no reallocation between the two)
+  const size_t np = std::max(
+      (size_t)pie::coarse_dots_blocks(n) * pie::kMaxDots * 3,
+      (size_t)pie::prolong_dots_blocks(s->H, s->W) * 6);
   if (s->wpart.n < np) PIE_HIP(s->wpart.alloc(np));
+  if (s->wsc.n < (size_t)pie::kWsCount) PIE_HIP(s->wsc.alloc(pie::kWsCount));
   if (s->wx.n < (size_t)hp * wp * 8) PIE_HIP(s->wx.alloc((size_t)hp * wp * 8));
   if (s->wout.n < (size_t)hp * wp * 3) PIE_HIP(s->wout.alloc((size_t)hp * wp * 3));
   double* r = s->wf.p;
@@ -904,39 +900,31 @@ int pie_warm_net_start(pie_solver* s, pie_unet* net, int factor,
   double* t = c + fs;
   auto P = [&](int k) { return t + fs * (1 + 2 * (size_t)k); };
   auto AP = [&](int k) { return t + fs * (2 + 2 * (size_t)k); };
+  double* sc = s->wsc.p;
   int cells = 0;
   for (int k = 0; k < n; ++k) cells += mc[k] != 0;
 
   s->warm_factor = 0;
-  PIE_CHECK(upload(mc, (size_t)n, s->wmc.p, s->stream));
+  // mc is pageable host memory: the copy completes before this returns
+  PIE_HIP(hipMemcpy(s->wmc.p, mc, (size_t)n, hipMemcpyHostToDevice));
   // r_c: the restricted residual on the solved cells
   PIE_HIP(pie::launch_restrict(s->state(), s->g0(), s->m0(), s->plane, s->P,
                                factor, hc, wc, s->coarse.p, s->stream));
   PIE_HIP(pie::launch_coarse_load(s->coarse.p, s->wmc.p, n, r, c, s->stream));
-  double pap[kMaxSteps][3];  // <p_k, A_c p_k> per channel (1 if it vanished)
+  // every scalar of the steps stays on the device: no host round trip
   for (int k = 0; k < net_steps; ++k) {
     double* z = P(k);
     double* az = AP(k);
     // network direction for the current residual (zero if r is zero)
-    double d[pie::kMaxDots][3];
-    PIE_CHECK(coarse_dots(s, n, {{r, r}}, d));
-    double sig[3];
-    bool any = false;
-    for (int ch = 0; ch < 3; ++ch) {
-      sig[ch] = cells ? std::sqrt(d[0][ch] / cells) : 0.0;
-      any = any || sig[ch] > 0.0;
-      if (!(sig[ch] > 0.0)) sig[ch] = 1.0;
-    }
-    if (any) {
-      PIE_HIP(pie::launch_net_input(r, s->wmc.p, hc, wc, hp, wp, sig, s->wx.p,
+    PIE_CHECK(coarse_dots(s, n, {{r, r}}));
+    PIE_HIP(pie::launch_warm_scalar(sc, pie::kWarmSig, k, (double)cells,
                                     s->stream));
-      PIE_HIP(hipStreamSynchronize(s->stream));
-      PIE_CHECK(pie_unet_forward_device(net, hp, wp, s->wx.p, s->wout.p));
-      PIE_HIP(pie::launch_net_output(s->wout.p, s->wmc.p, hc, wc, wp, z,
-                                     s->stream));
-    } else {
-      PIE_HIP(hipMemsetAsync(z, 0, fs * sizeof(double), s->stream));
-    }
+    PIE_HIP(pie::launch_net_input(r, s->wmc.p, hc, wc, hp, wp,
+                                  sc + pie::kWsSig, s->wx.p, s->stream));
+    PIE_CHECK(pie::unet_forward_on_stream(net, hp, wp, s->wx.p, s->wout.p,
+                                          s->stream));
+    PIE_HIP(pie::launch_net_output(s->wout.p, s->wmc.p, hc, wc, wp,
+                                   sc + pie::kWsAny, z, s->stream));
     // low-pass: damped homogeneous sweeps (z -> t -> z)
     for (int q = 0; q < lowpass_sweeps; ++q) {
       PIE_HIP(pie::launch_lowpass(z, s->wmc.p, hc, wc, t, s->stream));
@@ -945,29 +933,25 @@ int pie_warm_net_start(pie_solver* s, pie_unet* net, int factor,
     }
     // A_c-orthogonal to the earlier directions, one after the other
     for (int j = 0; j < k; ++j) {
-      PIE_CHECK(coarse_dots(s, n, {{z, AP(j)}}, d));
-      const double a[3] = {-(d[0][0] / pap[j][0]), -(d[0][1] / pap[j][1]),
-                           -(d[0][2] / pap[j][2])};
-      PIE_HIP(pie::launch_coarse_axpy(z, P(j), a, n, s->stream));
+      PIE_CHECK(coarse_dots(s, n, {{z, AP(j)}}));
+      PIE_HIP(pie::launch_warm_scalar(sc, pie::kWarmOrth, j, 0.0, s->stream));
+      PIE_HIP(pie::launch_coarse_axpy(z, P(j), sc + pie::kWsCoef, n,
+                                      s->stream));
     }
     PIE_HIP(pie::launch_coarse_apply(z, s->wmc.p, hc, wc, az, s->stream));
     // <z, A z>, <z, z>, <r, z>: the energy-optimal step along z
-    PIE_CHECK(coarse_dots(s, n, {{z, az}, {z, z}, {r, z}}, d));
-    double a[3], na[3];
-    for (int ch = 0; ch < 3; ++ch) {
-      const bool ok = d[0][ch] > 1e-30 * std::max(d[1][ch], 1e-300);
-      a[ch] = ok ? d[2][ch] / d[0][ch] : 0.0;
-      na[ch] = -a[ch];
-      pap[k][ch] = ok ? d[0][ch] : 1.0;
-    }
-    PIE_HIP(pie::launch_coarse_axpy(c, z, a, n, s->stream));
-    PIE_HIP(pie::launch_coarse_axpy(r, az, na, n, s->stream));
+    PIE_CHECK(coarse_dots(s, n, {{z, az}, {z, z}, {r, z}}));
+    PIE_HIP(pie::launch_warm_scalar(sc, pie::kWarmStep, k, 0.0, s->stream));
+    PIE_HIP(pie::launch_coarse_axpy(c, z, sc + pie::kWsStep, n, s->stream));
+    PIE_HIP(pie::launch_coarse_axpy(r, az, sc + pie::kWsNegStep, n,
+                                    s->stream));
   }
   PIE_HIP(pie::launch_coarse_to_float(c, (int)fs, s->coarse.p, s->stream));
-  if (corr_out)
+  if (corr_out) {
     PIE_HIP(hipMemcpyAsync(corr_out, s->coarse.p, fs * sizeof(float),
                            hipMemcpyDeviceToHost, s->stream));
-  PIE_HIP(hipStreamSynchronize(s->stream));
+    PIE_HIP(hipStreamSynchronize(s->stream));
+  }
   s->warm_factor = factor;
   return PIE_OK;
 }
@@ -984,22 +968,29 @@ int pie_warm_apply(pie_solver* s, int factor, const float* corr,
                             "pie_warm_net_start first or pass corr");
   }
   s->warm_factor = 0;
-  // line search: alpha_c = max(0, <r, d> / <d, A d>)
-  double dots[6];
-  PIE_CHECK(prolong_dots_device(s, factor, hc, wc, dots));
-  double alpha[3];
-  for (int ch = 0; ch < 3; ++ch) {
-    const double den = dots[3 + ch];
-    alpha[ch] = den > 0.0 ? dots[ch] / den : 0.0;
-    alpha[ch] = std::max(alpha[ch], 0.0);
-  }
+  // (the buffers are already large enough after pie_warm_net_start; with a
+  // host correction they may be allocated here, nothing queued uses them)
+  if (s->wsc.n < (size_t)pie::kWsCount) PIE_HIP(s->wsc.alloc(pie::kWsCount));
   if (!s->wmax.p) PIE_HIP(s->wmax.alloc(1));
-  PIE_HIP(hipMemsetAsync(s->wmax.p, 0, sizeof(unsigned), s->stream));
-  PIE_HIP(pie::launch_coarse_scale(s->coarse.p, alpha, hc * wc, s->wmax.p,
+  double* sc = s->wsc.p;
+  // line search on the device: alpha_c = max(0, <r, d> / <d, A d>)
+  const int nb = pie::prolong_dots_blocks(s->H, s->W);
+  if (s->wpart.n < (size_t)nb * 6) PIE_HIP(s->wpart.alloc((size_t)nb * 6));
+  PIE_HIP(pie::launch_prolong_dots(s->state(), s->g0(), s->m0(), s->plane,
+                                   s->H, s->W, s->P, factor, hc, wc,
+                                   s->coarse.p, s->wpart.p, nb, s->stream));
+  PIE_HIP(pie::launch_sum_partials(s->wpart.p, nb, 6, sc + pie::kWsDots,
                                    s->stream));
+  PIE_HIP(pie::launch_warm_scalar(sc, pie::kWarmAlpha, 0, 0.0, s->stream));
+  PIE_HIP(hipMemsetAsync(s->wmax.p, 0, sizeof(unsigned), s->stream));
+  PIE_HIP(pie::launch_coarse_scale(s->coarse.p, sc + pie::kWsAlpha, hc * wc,
+                                   s->wmax.p, s->stream));
   PIE_HIP(pie::launch_prolong(s->state(), s->m0(), s->plane, s->H, s->W, s->P,
                               factor, hc, wc, s->coarse.p, s->stream));
+  double alpha[3];
   unsigned mb = 0;
+  PIE_HIP(hipMemcpyAsync(alpha, sc + pie::kWsAlpha, sizeof(alpha),
+                         hipMemcpyDeviceToHost, s->stream));
   PIE_HIP(hipMemcpyAsync(&mb, s->wmax.p, sizeof(mb), hipMemcpyDeviceToHost,
                          s->stream));
   PIE_HIP(hipStreamSynchronize(s->stream));

@@ -56,6 +56,8 @@ struct pie_unet {
   long long P[4] = {0, 0, 0, 0};  // pixels per level
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // ordering against a caller's stream (pie::unet_forward_on_stream)
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
   long long nparam = 0;
   long long woff[kNumConv + 1] = {}, boff[kNumConv + 1] = {};
   DevBuf<float> param, grad, adam_m, adam_v;
@@ -285,6 +287,36 @@ int loss_and_grad(pie_unet* u, float* loss) {
 
 }  // namespace
 
+namespace pie {
+
+// The forward pass on device buffers, ordered after the work already
+// queued on `st` and before the work queued on it afterwards (events
+// between the caller's stream and the network's; no host wait).  st null:
+// ordered on the network's own stream only.
+int unet_forward_on_stream(pie_unet* u, int h, int w, const float* x,
+                           float* out, hipStream_t st) {
+  if (!u || !x || !out) return fail(PIE_EINVAL, "NULL argument");
+  if (u->B != 1 || u->H != h || u->W != w)
+    return fail(PIE_EINVAL, "network shape differs from (1, h, w)");
+  const long long P0 = u->P[0];
+  if (st) {
+    PIE_HIP(hipEventRecord(u->ev_in, st));
+    PIE_HIP(hipStreamWaitEvent(u->stream, u->ev_in, 0));
+  }
+  PIE_HIP(hipMemcpyAsync(u->x32.p, x, P0 * 8 * sizeof(float),
+                         hipMemcpyDeviceToDevice, u->stream));
+  PIE_CHECK(forward(u));
+  PIE_HIP(hipMemcpyAsync(out, u->out.p, P0 * 3 * sizeof(float),
+                         hipMemcpyDeviceToDevice, u->stream));
+  if (st) {
+    PIE_HIP(hipEventRecord(u->ev_out, u->stream));
+    PIE_HIP(hipStreamWaitEvent(st, u->ev_out, 0));
+  }
+  return PIE_OK;
+}
+
+}  // namespace pie
+
 extern "C" {
 
 int pie_unet_create(int batch, int h, int w, pie_unet** out) {
@@ -315,6 +347,8 @@ int pie_unet_create(int batch, int h, int w, pie_unet** out) {
   PIE_HIP(hipStreamCreateWithFlags(&u->stream, hipStreamNonBlocking));
   PIE_HIP(hipEventCreate(&u->ev0));
   PIE_HIP(hipEventCreate(&u->ev1));
+  PIE_HIP(hipEventCreateWithFlags(&u->ev_in, hipEventDisableTiming));
+  PIE_HIP(hipEventCreateWithFlags(&u->ev_out, hipEventDisableTiming));
   PIE_HIP(u->param.alloc(off));
   PIE_HIP(u->grad.alloc(off));
   PIE_HIP(u->adam_m.alloc(off));
@@ -337,6 +371,8 @@ int pie_unet_destroy(pie_unet* u) {
   if (u->stream) (void)hipStreamSynchronize(u->stream);
   if (u->ev0) (void)hipEventDestroy(u->ev0);
   if (u->ev1) (void)hipEventDestroy(u->ev1);
+  if (u->ev_in) (void)hipEventDestroy(u->ev_in);
+  if (u->ev_out) (void)hipEventDestroy(u->ev_out);
   hipStream_t st = u->stream;
   delete u;
   if (st) (void)hipStreamDestroy(st);
@@ -383,15 +419,7 @@ int pie_unet_forward(pie_unet* u, const float* x, float* out) {
 
 int pie_unet_forward_device(pie_unet* u, int h, int w, const float* x,
                             float* out) {
-  if (!u || !x || !out) return fail(PIE_EINVAL, "NULL argument");
-  if (u->B != 1 || u->H != h || u->W != w)
-    return fail(PIE_EINVAL, "network shape differs from (1, h, w)");
-  const long long P0 = u->P[0];
-  PIE_HIP(hipMemcpyAsync(u->x32.p, x, P0 * 8 * sizeof(float),
-                         hipMemcpyDeviceToDevice, u->stream));
-  PIE_CHECK(forward(u));
-  PIE_HIP(hipMemcpyAsync(out, u->out.p, P0 * 3 * sizeof(float),
-                         hipMemcpyDeviceToDevice, u->stream));
+  PIE_CHECK(pie::unet_forward_on_stream(u, h, w, x, out, nullptr));
   PIE_HIP(hipStreamSynchronize(u->stream));
   return PIE_OK;
 }
