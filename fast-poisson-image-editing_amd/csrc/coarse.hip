@@ -211,15 +211,33 @@ __global__ void __launch_bounds__(kBlock)
   if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicMax(maxbits, m);
 }
 
-// out[k] = sum over blocks b (in order) of partial[b * width + k]
-__global__ void __launch_bounds__(kWave)
+// out[k] = sum over blocks b of partial[b * width + k], k < width <= 12:
+// thread t adds blocks t, t + 256, ... in order, then a fixed tree over the
+// threads (deterministic; the grouping differs from a sequential sum only
+// in the last bits)
+constexpr int kSumWidth = kMaxDots * 3;
+__global__ void __launch_bounds__(kBlock)
     sum_partials_kernel(const double* __restrict__ partial, int nb, int width,
                         double* __restrict__ out) {
-  const int k = threadIdx.x;
-  if (k >= width) return;
-  double v = 0.0;
-  for (int b = 0; b < nb; ++b) v += partial[(long long)b * width + k];
-  out[k] = v;
+  __shared__ double red[kSumWidth][kBlock];
+  double acc[kSumWidth];
+#pragma unroll
+  for (int k = 0; k < kSumWidth; ++k) acc[k] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kBlock)
+#pragma unroll
+    for (int k = 0; k < kSumWidth; ++k)
+      if (k < width) acc[k] += partial[(long long)b * width + k];
+#pragma unroll
+  for (int k = 0; k < kSumWidth; ++k) red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h)
+#pragma unroll
+      for (int k = 0; k < kSumWidth; ++k)
+        red[k][threadIdx.x] = red[k][threadIdx.x] + red[k][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x < width) out[threadIdx.x] = red[threadIdx.x][0];
 }
 
 // The scalar steps of the flexible CG on the device (one thread), on the
@@ -266,8 +284,8 @@ __global__ void warm_scalar_kernel(double* __restrict__ sc, int op, int k,
 
 hipError_t launch_sum_partials(const double* partial, int nb, int width,
                                double* out, hipStream_t st) {
-  if (width > kWave) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kWave), 0, st,
+  if (width > kSumWidth) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kBlock), 0, st,
                      partial, nb, width, out);
   return hipGetLastError();
 }

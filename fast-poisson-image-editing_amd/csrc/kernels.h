@@ -218,7 +218,8 @@ constexpr int kWsDots = 0, kWsSig = kWsDots + kMaxDots * 3, kWsAny = kWsSig + 3,
               kWsNegStep = kWsStep + 3, kWsAlpha = kWsNegStep + 3,
               kWsPap = kWsAlpha + 3, kWsCount = kWsPap + 3 * kWarmMaxSteps;
 constexpr int kWarmSig = 0, kWarmOrth = 1, kWarmStep = 2, kWarmAlpha = 3;
-// out[k] = sum over b < nb, in order, of partial[b * width + k] (width <= 64)
+// out[k] = sum over b < nb of partial[b * width + k] (width <= 12; a fixed
+// summation tree: deterministic)
 hipError_t launch_sum_partials(const double* partial, int nb, int width,
                                double* out, hipStream_t st);
 // one scalar step on sc (op kWarmSig: sig/any from the dots of (r, r) and
