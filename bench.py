@@ -69,8 +69,9 @@ def bench_batch(a):
   t_in = time.perf_counter()
   items = synth.batch(256, seed=0)
   process.register_backend("grab", lambda **kw: Grab(), lambda **kw: Grab())
-  b = core_hip.BatchSolver(sweeps_per_launch=8, cols_per_lane=1,
-                           rows_per_task=64)
+  tile = ([int(v) for v in a.tile.split(",")] if a.tile else [8, 1, 64])
+  b = core_hip.BatchSolver(sweeps_per_launch=tile[0], cols_per_lane=tile[1],
+                           rows_per_task=tile[2])
   masked = []
   for w in items:
     p = process.GridProcessor(w.gradient, "grab")
@@ -105,7 +106,7 @@ def bench_batch(a):
       "config": {"workload": "batch256", "items": len(items),
                  "solved_pixels": size["masked"], "tiles": size["tasks"],
                  "max_sweeps": 5000, "tol_max_abs_update": 1e-3,
-                 "tile": [8, 1, 64], "rounds_per_graph": 16},
+                 "tile": tile, "rounds_per_graph": 16},
       "latency_p50_ms": float(np.percentile(lats, 50) * 1e3),
       "latency_p99_ms": float(np.percentile(lats, 99) * 1e3),
       "pixel_updates_per_s": upd / total,

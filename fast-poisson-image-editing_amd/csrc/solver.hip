@@ -905,8 +905,10 @@ int pie_warm_net_start(pie_solver* s, pie_unet* net, int factor,
   for (int k = 0; k < n; ++k) cells += mc[k] != 0;
 
   s->warm_factor = 0;
-  // mc is pageable host memory: the copy completes before this returns
-  PIE_HIP(hipMemcpy(s->wmc.p, mc, (size_t)n, hipMemcpyHostToDevice));
+  // mc is pageable host memory: copy it in stream order (after any work
+  // still queued that reads the previous mask) and wait for the copy
+  PIE_CHECK(upload(mc, (size_t)n, s->wmc.p, s->stream));
+  PIE_HIP(hipStreamSynchronize(s->stream));
   // r_c: the restricted residual on the solved cells
   PIE_HIP(pie::launch_restrict(s->state(), s->g0(), s->m0(), s->plane, s->P,
                                factor, hc, wc, s->coarse.p, s->stream));
