@@ -266,8 +266,8 @@ def main():
   else:
     core.autotune(8)
   tile = core.tile()
-  print(f"# tile S={tile[0]} C={tile[1]} rows={tile[2]}", file=sys.stderr,
-        flush=True)
+  print(f"# tile S={tile[0]} C={tile[1]} rows={tile[2]} channels/task "
+        f"{core.tile_channels()}", file=sys.stderr, flush=True)
 
   for _ in range(a.warmup):
     core.step(iters)

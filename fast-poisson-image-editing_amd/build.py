@@ -29,10 +29,13 @@ FLAGS = [
 ]
 
 
-def tiles() -> list[tuple[int, int]]:
+def tiles() -> list[tuple[int, int, int]]:
+  """(sweeps, columns per lane, channels per task) of every compiled
+  variant of the sweep kernel (PIE_TILES in csrc/kernels.h)."""
   text = open(os.path.join(CSRC, "kernels.h")).read()
   line = re.search(r"#define PIE_TILES\(X\)((?:.*\\\n)*.*)", text).group(1)
-  return [(int(a), int(b)) for a, b in re.findall(r"X\((\d+), (\d+)\)", line)]
+  return [(int(a), int(b), int(c))
+          for a, b, c in re.findall(r"X\((\d+), (\d+), (\d+)\)", line)]
 
 
 def _compile(src: str, obj: str, defines: list[str]) -> str:
@@ -52,10 +55,10 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
   units = [(os.path.join(CSRC, f), os.path.join(OBJ, f[:-4] + ".o"), [])
            for f in sorted(os.listdir(CSRC))
            if f.endswith(".hip") and f != "march_inst.hip"]
-  for s, c in tiles():
+  for s, c, n in tiles():
     units.append((os.path.join(CSRC, "march_inst.hip"),
-                  os.path.join(OBJ, f"march_{s}x{c}.o"),
-                  [f"-DPIE_S={s}", f"-DPIE_C={c}"]))
+                  os.path.join(OBJ, f"march_{s}x{c}x{n}.o"),
+                  [f"-DPIE_S={s}", f"-DPIE_C={c}", f"-DPIE_N={n}"]))
   jobs = jobs or min(len(units), max(1, (os.cpu_count() or 2)))
   with cf.ThreadPoolExecutor(jobs) as ex:
     objs = list(ex.map(lambda u: _compile(*u), units))

@@ -100,7 +100,7 @@ int prepare(pie_batch* b) {
     d.tol = it.tol;
     task_item.insert(task_item.end(),
                      (size_t)d.stripes * d.segs *
-                         pie::tile_groups(b->cfg.S, b->cfg.C),
+                         pie::tile_groups(b->cfg),
                      i);
   }
   b->ntask = (int)task_item.size();
@@ -182,7 +182,7 @@ int pie_batch_create(int order, int sweeps_per_launch, int cols_per_lane,
   TileCfg cfg{sweeps_per_launch > 0 ? sweeps_per_launch : 8,
               cols_per_lane > 0 ? cols_per_lane : 1,
               seg_rows > 0 ? seg_rows : 64};
-  if (!pie::tile_supported(cfg.S, cfg.C))
+  if (!pie::tile_supported(cfg))
     return fail(PIE_EUNSUPPORTED, "tile not compiled");
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count < 1)

@@ -74,29 +74,39 @@ struct BatchArgs {
 // rows written per wave task.
 struct TileCfg {
   int S, C, seg_rows;
+  int N = 0;  // channels per wave task: 3, 1, or 0 for the tile's default
 };
 
-// Compiled (sweeps, columns per lane) variants of the sweep kernel; each is
-// built from march_inst.hip with -DPIE_S=s -DPIE_C=c.
-// Tiles of two columns per lane keep twice as many values per level in
-// registers, so their wave tasks carry one colour channel each (three task
-// groups per tile; the channels are independent problems); one-column
-// tiles carry all three channels.
-#define PIE_TILES(X) X(4, 1) X(6, 1) X(8, 1) X(4, 2) X(6, 2) X(8, 2) X(12, 2)
-constexpr int tile_channels(int S, int C) { return C >= 2 ? 1 : 3; }
-constexpr int tile_groups(int S, int C) { return 3 / tile_channels(S, C); }
-#define PIE_DECL(s, c)                                             \
-  hipError_t launch_march_##s##_##c(int sched, int order, const MarchArgs& a, \
-                                    hipStream_t st);                         \
-  hipError_t launch_batch_##s##_##c(int sched, int order, const BatchArgs& b, \
-                                    int blocks, hipStream_t st);             \
-  int batch_blocks_per_cu_##s##_##c(int sched, int order);
+// Compiled (sweeps, columns per lane, channels per task) variants of the
+// sweep kernel; each is built from march_inst.hip with -DPIE_S=s -DPIE_C=c
+// -DPIE_N=n.  A wave task carries all three colour channels (n = 3) or one
+// (n = 1: three task groups per tile, the channels are independent
+// problems).  Tiles of two columns per lane keep twice as many values per
+// level in registers and take one channel; one-column tiles come in both
+// forms (one channel a task: a third of the registers and three times the
+// tasks, which fills the machine on small grids).
+#define PIE_TILES(X)                                                       \
+  X(4, 1, 3) X(6, 1, 3) X(8, 1, 3) X(8, 1, 1) X(4, 2, 1) X(6, 2, 1)        \
+  X(8, 2, 1) X(12, 2, 1)
+// the channels per task of cfg (its N, or the default of its column count)
+constexpr int tile_nch(const TileCfg& c) {
+  return c.N ? c.N : (c.C >= 2 ? 1 : 3);
+}
+constexpr int tile_groups(const TileCfg& c) { return 3 / tile_nch(c); }
+#define PIE_DECL(s, c, n)                                                  \
+  hipError_t launch_march_##s##_##c##_##n(int sched, int order,           \
+                                          const MarchArgs& a,             \
+                                          hipStream_t st);                \
+  hipError_t launch_batch_##s##_##c##_##n(int sched, int order,           \
+                                          const BatchArgs& b, int blocks, \
+                                          hipStream_t st);                \
+  int batch_blocks_per_cu_##s##_##c##_##n(int sched, int order);
 PIE_TILES(PIE_DECL)
 #undef PIE_DECL
 
-bool tile_supported(int S, int C);
-// The compiled (S, C) pairs, for the autotuner.
-int tile_variants(int (*out)[2], int max);
+bool tile_supported(const TileCfg& cfg);
+// The compiled (S, C, N) triples, for the autotuner.
+int tile_variants(int (*out)[3], int max);
 // Wave tasks (stripes x segments) of a tile on an H x W grid.
 void tile_tasks(const TileCfg& cfg, int H, int W, int* stripes, int* segs);
 

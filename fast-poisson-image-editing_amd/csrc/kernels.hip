@@ -562,21 +562,23 @@ int grid_blocks(long long n, int cap) {
 
 // ------------------------------------------------------------ launchers --
 
-bool tile_supported(int S, int C) {
-#define PIE_T(s, c) \
-  if (S == s && C == c) return true;
+bool tile_supported(const TileCfg& cfg) {
+  const int N = tile_nch(cfg);
+#define PIE_T(s, c, n) \
+  if (cfg.S == s && cfg.C == c && N == n) return true;
   PIE_TILES(PIE_T)
 #undef PIE_T
   return false;
 }
 
-int tile_variants(int (*out)[2], int max) {
+int tile_variants(int (*out)[3], int max) {
   int n = 0;
-#define PIE_T(s, c)   \
-  if (n < max) {      \
-    out[n][0] = s;    \
-    out[n][1] = c;    \
-  }                   \
+#define PIE_T(s, c, ch) \
+  if (n < max) {        \
+    out[n][0] = s;      \
+    out[n][1] = c;      \
+    out[n][2] = ch;     \
+  }                     \
   ++n;
   PIE_TILES(PIE_T)
 #undef PIE_T
@@ -593,9 +595,11 @@ hipError_t launch_march(const TileCfg& cfg, int sched, int order, MarchArgs a,
                         hipStream_t st) {
   tile_tasks(cfg, a.H, a.W, &a.stripes, &a.segs);
   a.seg_rows = cfg.seg_rows;
-  a.ntask = a.stripes * a.segs * tile_groups(cfg.S, cfg.C);
-#define PIE_T(s, c) \
-  if (cfg.S == s && cfg.C == c) return launch_march_##s##_##c(sched, order, a, st);
+  a.ntask = a.stripes * a.segs * tile_groups(cfg);
+  const int N = tile_nch(cfg);
+#define PIE_T(s, c, n)                              \
+  if (cfg.S == s && cfg.C == c && N == n)           \
+    return launch_march_##s##_##c##_##n(sched, order, a, st);
   PIE_TILES(PIE_T)
 #undef PIE_T
   return hipErrorInvalidValue;
@@ -603,17 +607,20 @@ hipError_t launch_march(const TileCfg& cfg, int sched, int order, MarchArgs a,
 
 hipError_t launch_batch(const TileCfg& cfg, int sched, int order,
                         const BatchArgs& b, int blocks, hipStream_t st) {
-#define PIE_T(s, c)                \
-  if (cfg.S == s && cfg.C == c)    \
-    return launch_batch_##s##_##c(sched, order, b, blocks, st);
+  const int N = tile_nch(cfg);
+#define PIE_T(s, c, n)                              \
+  if (cfg.S == s && cfg.C == c && N == n)           \
+    return launch_batch_##s##_##c##_##n(sched, order, b, blocks, st);
   PIE_TILES(PIE_T)
 #undef PIE_T
   return hipErrorInvalidValue;
 }
 
 int batch_blocks_per_cu(const TileCfg& cfg, int sched, int order) {
-#define PIE_T(s, c) \
-  if (cfg.S == s && cfg.C == c) return batch_blocks_per_cu_##s##_##c(sched, order);
+  const int N = tile_nch(cfg);
+#define PIE_T(s, c, n)                              \
+  if (cfg.S == s && cfg.C == c && N == n)           \
+    return batch_blocks_per_cu_##s##_##c##_##n(sched, order);
   PIE_TILES(PIE_T)
 #undef PIE_T
   return 1;

@@ -166,14 +166,14 @@ __device__ __forceinline__ T* pick(T* x, T* y) {
 // One wave task: column stripe `stripe`, row range `seg`, S steps.  When
 // dmax is not null the largest |change| of the last step over the pixels
 // the task writes is merged into *dmax (float bits, non-negative).
-template <int S, int C, int SCHED, int ORDER>
+template <int S, int C, int NCH, int SCHED, int ORDER>
 __device__ __forceinline__ void march_task(const MarchArgs& a, int stripe,
                                            int seg, int group,
                                            unsigned* dmax) {
   static_assert(S % 2 == 0 && S % C == 0, "S even and a multiple of C");
   static_assert(S * C <= 32, "mask queue is one 32-bit word");
-  // channels of this task: all three, or (deep tiles) channel `group` only
-  constexpr int NCH = tile_channels(S, C);
+  static_assert(NCH == 1 || NCH == 3, "one or three channels a task");
+  // channels of this task: all three, or channel `group` only
   const long long choff = (long long)group * NCH * a.plane;
   const float* __restrict__ xin = a.xin + choff;
   float* __restrict__ xout = a.xout + choff;
@@ -390,7 +390,7 @@ __device__ __forceinline__ void march_task(const MarchArgs& a, int stripe,
 #ifndef PIE_WAVES_PER_EU
 #define PIE_WAVES_PER_EU 1
 #endif
-template <int S, int C, int SCHED, int ORDER>
+template <int S, int C, int NCH, int SCHED, int ORDER>
 __global__ void __launch_bounds__(kBlock)
     __attribute__((amdgpu_waves_per_eu(PIE_WAVES_PER_EU)))
     march_kernel(MarchArgs a) {
@@ -399,15 +399,15 @@ __global__ void __launch_bounds__(kBlock)
   // task = (channel group, segment, stripe), stripe fastest
   const int per_group = a.stripes * a.segs;
   const int group = task / per_group, rem = task % per_group;
-  march_task<S, C, SCHED, ORDER>(a, rem % a.stripes, rem / a.stripes, group,
-                                 a.dmax);
+  march_task<S, C, NCH, SCHED, ORDER>(a, rem % a.stripes, rem / a.stripes,
+                                      group, a.dmax);
 }
 
 // Persistent sweep kernel of a batch: every wave takes tasks (tiles of any
 // item) from one atomic work counter until the counter passes the total,
 // so uneven items share the machine.  Items already stopped are skipped.
 // Each item's launch count selects its input buffer and first step.
-template <int S, int C, int SCHED, int ORDER>
+template <int S, int C, int NCH, int SCHED, int ORDER>
 __global__ void __launch_bounds__(kBlock) march_batch_kernel(BatchArgs b) {
   const int lane = threadIdx.x & (kWave - 1);
   for (;;) {
@@ -437,8 +437,8 @@ __global__ void __launch_bounds__(kBlock) march_batch_kernel(BatchArgs b) {
     const int local = t - it.task0;
     const int per_group = it.stripes * it.segs;
     const int group = local / per_group, rem = local % per_group;
-    march_task<S, C, SCHED, ORDER>(a, rem % it.stripes, rem / it.stripes,
-                                   group, &b.items[id].dmax_bits);
+    march_task<S, C, NCH, SCHED, ORDER>(a, rem % it.stripes, rem / it.stripes,
+                                        group, &b.items[id].dmax_bits);
   }
 }
 

@@ -193,12 +193,13 @@ int default_tile(pie_solver* s) {
   if (s->tile_fixed) return PIE_OK;
   // Below 2^22 pixels (where pie_iterate does not autotune) short row
   // ranges keep enough wave tasks in flight: measured on MI355X, Jacobi
-  // sweeps of a 1024^2 composite take 3.5 us with (8, 1, 16) against 4.9 us
-  // with (8, 2, 32) and 6.5 us with (8, 1, 64), a 2048^2 one 6.3 us with
+  // sweeps of a 1024^2 composite take 3.1 us with (8, 1, 16) one channel a
+  // task, 3.5 us with all three, 4.9 us with (8, 2, 32) and 6.5 us with
+  // (8, 1, 64); a 512^2 one 1.8 us against 2.4 us; a 2048^2 one 6.3 us with
   // (8, 2, 32) against 8.4 us with (8, 1, 16) and 8.3 us with (8, 1, 64).
   const long long px = (long long)s->H * s->W;
   if (px <= (1LL << 21))
-    s->cfg = TileCfg{8, 1, 16};
+    s->cfg = TileCfg{8, 1, 16, 1};
   else if (px < (1LL << 22))
     s->cfg = TileCfg{8, 2, 32};
   else
@@ -210,7 +211,7 @@ int run_sweeps(pie_solver* s, int iters);
 
 int autotune(pie_solver* s, int launches) {
   if (!s->grid_layout) return PIE_OK;
-  int var[32][2];
+  int var[32][3];
   const int nv = std::min(pie::tile_variants(var, 32), 32);
   const int rows[] = {16, 32, 64, 128, 256};
   float best = 1e30f;
@@ -220,7 +221,7 @@ int autotune(pie_solver* s, int launches) {
   for (int v = 0; v < nv; ++v)
     for (int rr : rows) {
       if (rr > 2 * s->H && rr != rows[0]) continue;
-      TileCfg c{var[v][0], var[v][1], rr};
+      TileCfg c{var[v][0], var[v][1], rr, var[v][2]};
       // the state buffer is only read; the spare buffer takes the output.
       // t0 = H + W puts every pixel inside the wavefront band of gs_lex.
       MarchArgs a = march_args(s, s->state(), s->other(), s->H + s->W,
@@ -1029,12 +1030,35 @@ int pie_set_tile(pie_solver* s, int sweeps, int cols_per_lane, int seg_rows) {
   if (sweeps > 0) c.S = sweeps;
   if (cols_per_lane > 0) c.C = cols_per_lane;
   if (seg_rows > 0) c.seg_rows = seg_rows;
-  if (!pie::tile_supported(c.S, c.C))
+  c.N = 0;  // the column count's default; pie_set_tile_channels overrides
+  if (!pie::tile_supported(c))
     return fail(PIE_EUNSUPPORTED, "tile (sweeps " + std::to_string(c.S) +
                                       ", columns/lane " + std::to_string(c.C) +
                                       ") not compiled");
   s->cfg = c;
   s->tile_fixed = true;
+  return PIE_OK;
+}
+
+int pie_set_tile_channels(pie_solver* s, int channels) {
+  if (!s) return fail(PIE_EINVAL, "NULL solver");
+  if (channels != 0 && channels != 1 && channels != 3)
+    return fail(PIE_EINVAL, "channels per task must be 0, 1 or 3");
+  TileCfg c = s->cfg;
+  c.N = channels;
+  if (!pie::tile_supported(c))
+    return fail(PIE_EUNSUPPORTED,
+                "tile (sweeps " + std::to_string(c.S) + ", columns/lane " +
+                    std::to_string(c.C) + ", channels " +
+                    std::to_string(pie::tile_nch(c)) + ") not compiled");
+  s->cfg = c;
+  s->tile_fixed = true;
+  return PIE_OK;
+}
+
+int pie_get_tile_channels(pie_solver* s, int* channels) {
+  if (!s || !channels) return fail(PIE_EINVAL, "NULL argument");
+  *channels = pie::tile_nch(s->cfg);
   return PIE_OK;
 }
 

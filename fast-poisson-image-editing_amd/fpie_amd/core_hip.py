@@ -67,9 +67,19 @@ class _Core:
     native.check(self._lib.pie_iterate(self._h, int(iteration)))
 
   def set_tile(self, sweeps: int = 0, cols_per_lane: int = 0,
-               seg_rows: int = 0) -> None:
+               seg_rows: int = 0, channels: int = 0) -> None:
+    """Fix the sweep tile (0 keeps a value); channels per wave task 3 or
+    1 (0: the default of the column count)."""
     native.check(self._lib.pie_set_tile(self._h, sweeps, cols_per_lane,
                                         seg_rows))
+    if channels:
+      native.check(self._lib.pie_set_tile_channels(self._h, int(channels)))
+
+  def tile_channels(self) -> int:
+    """Colour channels per wave task of the current tile (3 or 1)."""
+    v = ctypes.c_int()
+    native.check(self._lib.pie_get_tile_channels(self._h, ctypes.byref(v)))
+    return v.value
 
   def tile(self) -> tuple[int, int, int]:
     v = [ctypes.c_int() for _ in range(3)]

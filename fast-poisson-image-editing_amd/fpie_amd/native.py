@@ -38,6 +38,8 @@ _SIGNATURES = {
     "pie_read": ([_vp, _vp, _vp], _ip),
     "pie_get_state": ([_vp, _vp], _ip),
     "pie_set_tile": ([_vp, _ip, _ip, _ip], _ip),
+    "pie_set_tile_channels": ([_vp, _ip], _ip),
+    "pie_get_tile_channels": ([_vp, ctypes.POINTER(_ip)], _ip),
     "pie_get_tile": ([_vp, ctypes.POINTER(_ip), ctypes.POINTER(_ip),
                       ctypes.POINTER(_ip)], _ip),
     "pie_autotune": ([_vp, _ip], _ip),

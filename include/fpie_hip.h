@@ -158,6 +158,12 @@ int pie_get_state(pie_solver* s, float* out);
 int pie_set_tile(pie_solver* s, int sweeps, int cols_per_lane, int seg_rows);
 int pie_get_tile(pie_solver* s, int* sweeps, int* cols_per_lane,
                  int* seg_rows);
+/* Colour channels per wave task of the tile: 3 (all), 1 (one channel a
+ * task, three times the tasks), or 0 for the default of the tile's column
+ * count (3 for one column, 1 for two).  The get call reports the channels
+ * the tile uses. */
+int pie_set_tile_channels(pie_solver* s, int channels);
+int pie_get_tile_channels(pie_solver* s, int* channels);
 /* Time every candidate tile on the loaded problem (`launches` launches
  * each, state untouched) and keep the fastest. */
 int pie_autotune(pie_solver* s, int launches);

@@ -163,7 +163,8 @@ def _tiles():
   from conftest import PKG
   text = open(os.path.join(PKG, "csrc", "kernels.h")).read()
   line = re.search(r"#define PIE_TILES\(X\)((?:.*\\\n)*.*)", text).group(1)
-  return [(int(a), int(b)) for a, b in re.findall(r"X\((\d+), (\d+)\)", line)]
+  return [(int(a), int(b), int(c))
+          for a, b, c in re.findall(r"X\((\d+), (\d+), (\d+)\)", line)]
 
 
 @pytest.mark.parametrize("sched,order", [("jacobi", "ulrd"),
@@ -179,15 +180,16 @@ def test_every_tile_matches_oracle(hip, oracle, sched, order):
     want = oracle.grid_gs(m, t, gr, it)
   else:
     want = oracle.grid_colour(m, t, gr, it, order)
-  for S, C in _tiles():
+  for S, C, N in _tiles():
     for rows in (16, 48, 256):
       core = hip.GridSolver(schedule=sched, order=order)
-      core.set_tile(S, C, rows)
+      core.set_tile(S, C, rows, channels=N)
+      assert core.tile() == (S, C, rows) and core.tile_channels() == N
       core.reset(0, m, t, gr)
       core.step(it)
       got = core.state()
       assert np.array_equal(bits(got), bits(want)), (
-          f"tile S={S} C={C} rows={rows}: "
+          f"tile S={S} C={C} N={N} rows={rows}: "
           f"{int((bits(got) != bits(want)).sum())} values differ")
 
 
