@@ -114,13 +114,15 @@ class WarmStart:
   `sweeps` plain Jacobi sweeps per smoothing (default 24 f^2); networks
   are kept per coarse shape, so repeated problems of one size reuse them.
   precond "none" drops the network, for comparison.  `coarse_solver` makes
-  the coarse GridSolver of the optional polish."""
+  the coarse GridSolver of the optional polish.  `tol` (optional) stops the
+  cycles as soon as the mean residual is at or below it."""
 
   def __init__(self, params: np.ndarray, coarse: int = 256, cycles: int = 12,
                sweeps: int | None = None, omega: float = 0.5,
                stop: float = 0.005, net_steps: int = 2, precond: str = "net",
                lowpass_sweeps: int = 4, coarse_rtol: float | None = None,
-               coarse_max_sweeps: int = 1_000_000, coarse_solver=None):
+               coarse_max_sweeps: int = 1_000_000, coarse_solver=None,
+               tol: float | None = None):
     if precond not in ("net", "none"):
       raise ValueError("precond must be 'net' or 'none'")
     self.params = np.asarray(params, np.float32)
@@ -130,6 +132,8 @@ class WarmStart:
     self.coarse_rtol, self.coarse_max_sweeps = coarse_rtol, coarse_max_sweeps
     self.coarse_solver = coarse_solver or (
         lambda: GridSolver(schedule="jacobi"))
+    # optional target: stop cycling once the mean residual is at or below it
+    self.tol = tol
     self._nets: dict = {}
 
   def factor(self, h: int, w: int) -> int:
@@ -203,6 +207,8 @@ class WarmStart:
     steps = []
     r_prev = mean_residual(solver)
     for k in range(self.cycles):
+      if self.tol is not None and r_prev <= self.tol:
+        break
       cc, info = self.coarse_solve(solver, f, mc, cs)
       alpha, size = solver.warm_apply(f, cc)
       smooth()
@@ -435,6 +441,12 @@ def evaluate(ws: WarmStart, size: int, seed: int, tol: float = 1e-3,
                  ws.stop, ws.net_steps, "none", ws.lowpass_sweeps, 1e-2,
                  ws.coarse_max_sweeps)
   t_cl, n_cl, r_cl, info_cl, x_cl = warm(cl)
+  # the learned warm start stopped as soon as it reaches the tolerance
+  wt = WarmStart(ws.params, ws.coarse, ws.cycles, ws.sweeps, ws.omega,
+                 ws.stop, ws.net_steps, ws.precond, ws.lowpass_sweeps,
+                 ws.coarse_rtol, ws.coarse_max_sweeps, tol=tol_eff)
+  wt._nets = ws._nets  # the same networks
+  t_wt, n_wt, r_wt, info_wt, x_wt = warm(wt)
 
   x_star = exact_solution(mask, tgt, grad, x_ref, log=log)
 
@@ -465,12 +477,22 @@ def evaluate(ws: WarmStart, size: int, seed: int, tol: float = 1e-3,
                              "warm_start_seconds": info_cl["seconds"],
                              "jacobi_sweeps": n_cl, "residual": r_cl,
                              **err(x_cl), "warm_start": info_cl},
+      "warm_stop_at_tol": {"seconds": t_wt,
+                           "warm_start_seconds": info_wt["seconds"],
+                           "jacobi_sweeps": n_wt, "residual": r_wt,
+                           **err(x_wt), "cycles": info_wt["cycles"],
+                           "sweeps": info_wt["sweeps"],
+                           "used": info_wt["used"]},
       "speedup": t_cold / max(t_warm, 1e-9),
       "speedup_without_network": t_cold / max(t_nn, 1e-9),
       "speedup_classical_two_grid": t_cold / max(t_cl, 1e-9),
+      "speedup_stop_at_tol": t_cold / max(t_wt, 1e-9),
   }
   rec["same_accuracy"] = bool(
       rec["warm"]["max_abs_err_vs_exact"] <=
+      rec["cold"]["max_abs_err_vs_exact"])
+  rec["same_accuracy_stop_at_tol"] = bool(
+      rec["warm_stop_at_tol"]["max_abs_err_vs_exact"] <=
       rec["cold"]["max_abs_err_vs_exact"])
   log(json.dumps(rec))
   return rec
@@ -551,6 +573,12 @@ def main(argv=None) -> None:
           sz: float(np.mean([e["speedup"] for e in evals
                              if e["size"] == int(sz)]))
           for sz in a.sizes.split(",")},
+      "speedup_stop_at_tol_by_size": {
+          sz: float(np.mean([e["speedup_stop_at_tol"] for e in evals
+                             if e["size"] == int(sz)]))
+          for sz in a.sizes.split(",")},
+      "same_accuracy_stop_at_tol": all(e["same_accuracy_stop_at_tol"]
+                                       for e in evals),
       "speedup_without_network_by_size": {
           sz: float(np.mean([e["speedup_without_network"] for e in evals
                              if e["size"] == int(sz)]))
