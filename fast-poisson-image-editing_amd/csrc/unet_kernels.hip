@@ -448,18 +448,17 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Weight gradient over spatial tiles (input channels a multiple of 8, output
-// channels of 32): a block owns 32 input x 32 output channels (input
-// channels past cin read as zero) for all nine taps and
-// sums over 8 x 16-pixel tiles of the images. For each tile it stages,
-// channel-major with the pixels of a row contiguous, three copies of the
-// input window (tile plus one-pixel halo) shifted by one column each, so
-// that every tap reads the 8 pixels of a lane group as one aligned 16-byte
-// word, and the output gradient of the tile. Wave w owns the 16 x 16 (ci,
-// co) block (w / 2, w % 2) of all nine taps; one MFMA step covers two rows
-// of the tile (lane group g: row g / 2, columns 8 (g % 2) .. +7).
+// channels of 32): a block owns 32 input x CO (32 or 64) output channels
+// (input channels past cin read as zero) for all nine taps and sums over
+// 8 x 16-pixel tiles of the images. For each tile it stages, channel-major
+// with the pixels of a row contiguous, three copies of the input window
+// (tile plus one-pixel halo) shifted by one column each, so that every tap
+// reads the 8 pixels of a lane group as one aligned 16-byte word, and the
+// output gradient of the tile. One MFMA step covers two rows of the tile
+// (lane group g: row g / 2, columns 8 (g % 2) .. +7).
 //
-// Staging jobs: threads 0..63 load 8 pixels x 8 channels of the gradient
-// (and keep the bias sums of their channels); threads 64..143 load 10
+// Staging jobs: the first threads each load 8 pixels x 8 channels of the
+// gradient (and keep the bias sums of their channels); the next 80 load 10
 // pixels x 8 channels of an input window row, from which the three shifted
 // copies are cut. A job's global loads are unconditional (addresses clamped
 // into the image, values outside selected to zero) and the loads of the
@@ -470,7 +469,7 @@ constexpr int kWtRowsX = kWtH + 2;
 // 16 lanes of a group hit distinct banks
 constexpr int kWtXStride = kWtRowsX * 2 + 1;
 constexpr int kWtDStride = kWtH * 2 + 1;
-constexpr int kWtDJobs = kWtH * 2 * 4, kWtXJobs = kWtRowsX * 2 * 4;
+constexpr int kWtXJobs = kWtRowsX * 2 * 4;
 
 __device__ inline uint32_t pack_lo(uint32_t lo, uint32_t hi) {
   return (lo & 0xffffu) | (hi << 16);
@@ -494,35 +493,46 @@ __device__ __forceinline__ void transpose8(const u32x4 (&v)[N], int p0,
   }
 }
 
+// CO = 32 or 64 output channels per block: wave (ta, tb) owns input
+// channels 16 ta .. +15 and the CO / 32 output blocks of 16 starting at
+// (CO / 32) tb, for all nine taps (64: twice the MFMAs per staged input
+// window, 0.6 instead of 1.1 LDS operand reads per MFMA).
+template <int CO>
 __global__ void __launch_bounds__(kBlock)
     wgrad_lds_kernel(const bf16_t* __restrict__ x, int x_stride, int x_off,
                      int cin, int n, int h, int w,
                      const bf16_t* __restrict__ dy, int cout,
                      float* __restrict__ dw, float* __restrict__ db,
                      int tiles_per_block) {
+  constexpr int NBW = CO / 32;           // output blocks of 16 per wave
+  constexpr int DG = CO / 8;             // gradient channel groups
+  constexpr int DJOBS = kWtH * 2 * DG;   // gradient staging jobs
+  static_assert(DJOBS + kWtXJobs <= kBlock, "one staging job per thread");
   __shared__ u32x4 xw[3 * 32 * kWtXStride];
-  __shared__ u32x4 dt[32 * kWtDStride];
-  __shared__ float bsum[32];
+  __shared__ u32x4 dt[CO * kWtDStride];
+  __shared__ float bsum[CO];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   const int g = lane >> 4, i = lane & 15;
   const int ta = wave >> 1, tb = wave & 1;
   const int nci = (cin + 31) / 32;
-  const int ci0 = (blockIdx.y % nci) * 32, co0 = (blockIdx.y / nci) * 32;
+  const int ci0 = (blockIdx.y % nci) * 32, co0 = (blockIdx.y / nci) * CO;
   const int tiles_x = (w + kWtW - 1) / kWtW, tiles_y = (h + kWtH - 1) / kWtH;
   const long long ntiles = (long long)n * tiles_x * tiles_y;
   const bool bias = db && ci0 == 0;
   const int tid = threadIdx.x;
-  const bool djob = tid < kWtDJobs;
-  const bool xjob = tid >= kWtDJobs && tid < kWtDJobs + kWtXJobs;
+  const bool djob = tid < DJOBS;
+  const bool xjob = tid >= DJOBS && tid < DJOBS + kWtXJobs;
   // job coordinates: channel group, half row, row
-  const int jid = djob ? tid : tid - kWtDJobs;
-  const int cg = jid & 3, hx = (jid >> 2) & 1, jr = jid >> 3;
+  const int jid = djob ? tid : tid - DJOBS;
+  const int cg = djob ? jid % DG : jid & 3;
+  const int hx = djob ? (jid / DG) & 1 : (jid >> 2) & 1;
+  const int jr = djob ? jid / DG / 2 : jid >> 3;
   // input channel group inside cin (the loads of the others are clamped to
   // the last group and zeroed)
   const bool cok = ci0 + cg * 8 < cin;
   const int xch = min(ci0 + cg * 8, cin - 8);
-  if (tid < 32) bsum[tid] = 0.0f;
+  if (tid < CO) bsum[tid] = 0.0f;
   float bpart[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) bpart[k] = 0.0f;
@@ -588,9 +598,11 @@ __global__ void __launch_bounds__(kBlock)
     }
   };
 
-  f32x4 acc[9];
+  f32x4 acc[NBW][9];
 #pragma unroll
-  for (int t = 0; t < 9; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int q = 0; q < NBW; ++q)
+#pragma unroll
+    for (int t = 0; t < 9; ++t) acc[q][t] = f32x4{0.f, 0.f, 0.f, 0.f};
 
   const long long t_begin = (long long)blockIdx.x * tiles_per_block;
   const long long t_end = min(ntiles, t_begin + tiles_per_block);
@@ -602,16 +614,22 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int s = 0; s < kWtH / 2; ++s) {
       const int row = 2 * s + (g >> 1), half = g & 1;
-      const bf16x8 B = __builtin_bit_cast(
-          bf16x8, dt[(tb * 16 + i) * kWtDStride + row * 2 + half]);
+      bf16x8 B[NBW];
+#pragma unroll
+      for (int q = 0; q < NBW; ++q)
+        B[q] = __builtin_bit_cast(
+            bf16x8,
+            dt[((tb * NBW + q) * 16 + i) * kWtDStride + row * 2 + half]);
 #pragma unroll
       for (int tap = 0; tap < 9; ++tap) {
         const int ddy = tap / 3 - 1, d = tap % 3;
         const bf16x8 A = __builtin_bit_cast(
             bf16x8, xw[(d * 32 + ta * 16 + i) * kWtXStride +
                        (row + 1 + ddy) * 2 + half]);
-        acc[tap] =
-            __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[tap], 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < NBW; ++q)
+          acc[q][tap] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+              A, B[q], acc[q][tap], 0, 0, 0);
       }
     }
     __syncthreads();
@@ -622,18 +640,22 @@ __global__ void __launch_bounds__(kBlock)
     for (int k = 0; k < 8; ++k) atomicAdd(&bsum[cg * 8 + k], bpart[k]);
   }
   __syncthreads();
-  if (bias && tid < 32) atomicAdd(&db[co0 + tid], bsum[tid]);
-  // D[ci][co]: lane holds col co = co0 + 16 tb + i, rows ci = ci0 + 16 ta +
-  // 4 g + r
-  const int co = co0 + tb * 16 + i;
+  if (bias && tid < CO) atomicAdd(&db[co0 + tid], bsum[tid]);
+  // D[ci][co]: lane holds col co = co0 + 16 (NBW tb + q) + i, rows
+  // ci = ci0 + 16 ta + 4 g + r
 #pragma unroll
-  for (int tap = 0; tap < 9; ++tap)
+  for (int q = 0; q < NBW; ++q) {
+    const int co = co0 + (tb * NBW + q) * 16 + i;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int ci = ci0 + ta * 16 + 4 * g + r;
-      if (ci < cin)
-        atomicAdd(&dw[((long long)tap * cin + ci) * cout + co], acc[tap][r]);
-    }
+    for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int ci = ci0 + ta * 16 + 4 * g + r;
+        if (ci < cin)
+          atomicAdd(&dw[((long long)tap * cin + ci) * cout + co],
+                    acc[q][tap][r]);
+      }
+  }
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -1101,14 +1123,20 @@ hipError_t conv3x3_wgrad(const Slice& x, int n, int h, int w,
   if (cout % 32 == 0) {
     const long long ntiles = (long long)n * ((h + kWtH - 1) / kWtH) *
                              ((w + kWtW - 1) / kWtW);
-    const int tiles = ((x.ch + 31) / 32) * (cout / 32);
+    const int co = cout % 64 == 0 ? 64 : 32;
+    const int tiles = ((x.ch + 31) / 32) * (cout / co);
     // aim at about 1024 blocks in all
     long long per = (ntiles * tiles + 1023) / 1024;
     if (per < 1) per = 1;
     const int xb = (int)((ntiles + per - 1) / per);
-    hipLaunchKernelGGL(wgrad_lds_kernel, dim3(xb, tiles), dim3(kBlock), 0, st,
-                       static_cast<const bf16_t*>(x.base), x.stride, x.off,
-                       x.ch, n, h, w, dy, cout, dw, db, (int)per);
+    if (co == 64)
+      hipLaunchKernelGGL(wgrad_lds_kernel<64>, dim3(xb, tiles), dim3(kBlock),
+                         0, st, static_cast<const bf16_t*>(x.base), x.stride,
+                         x.off, x.ch, n, h, w, dy, cout, dw, db, (int)per);
+    else
+      hipLaunchKernelGGL(wgrad_lds_kernel<32>, dim3(xb, tiles), dim3(kBlock),
+                         0, st, static_cast<const bf16_t*>(x.base), x.stride,
+                         x.off, x.ch, n, h, w, dy, cout, dw, db, (int)per);
     return hipGetLastError();
   }
   const long long P = (long long)n * h * w;
