@@ -450,53 +450,36 @@ __global__ void __launch_bounds__(kBlock)
 // Weight gradient over spatial tiles (input channels a multiple of 8, output
 // channels of 32): a block owns 32 input x CO (32 or 64) output channels
 // (input channels past cin read as zero) for all nine taps and sums over
-// 8 x 16-pixel tiles of the images. For each tile it stages, channel-major
-// with the pixels of a row contiguous, three copies of the input window
-// (tile plus one-pixel halo) shifted by one column each, so that every tap
-// reads the 8 pixels of a lane group as one aligned 16-byte word, and the
-// output gradient of the tile. One MFMA step covers two rows of the tile
-// (lane group g: row g / 2, columns 8 (g % 2) .. +7).
-//
-// Staging jobs: the first threads each load 8 pixels x 8 channels of the
-// gradient (and keep the bias sums of their channels); the next 80 load 10
-// pixels x 8 channels of an input window row, from which the three shifted
-// copies are cut. A job's global loads are unconditional (addresses clamped
-// into the image, values outside selected to zero) and the loads of the
-// next tile are in flight while the MFMAs of the current one run.
-constexpr int kWtH = 8, kWtW = 16;
-constexpr int kWtRowsX = kWtH + 2;
-// 16-byte words per channel row block, padded to an odd count so that the
-// 16 lanes of a group hit distinct banks
-constexpr int kWtXStride = kWtRowsX * 2 + 1;
-constexpr int kWtDStride = kWtH * 2 + 1;
-constexpr int kWtXJobs = kWtRowsX * 2 * 4;
-
-__device__ inline uint32_t pack_lo(uint32_t lo, uint32_t hi) {
-  return (lo & 0xffffu) | (hi << 16);
-}
-__device__ inline uint32_t pack_hi(uint32_t lo, uint32_t hi) {
-  return (lo >> 16) | (hi & 0xffff0000u);
-}
-
-// 8 pixels x 8 channels (v[p0 + p] holds channels 0..7 of pixel p) to 8
-// channels x 8 pixels (o[c] holds pixels 0..7 of channel c)
-template <int N>
-__device__ __forceinline__ void transpose8(const u32x4 (&v)[N], int p0,
-                                           u32x4 (&o)[8]) {
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const uint32_t lo = v[p0 + 2 * j][c >> 1], hi = v[p0 + 2 * j + 1][c >> 1];
-      o[c][j] = (c & 1) ? pack_hi(lo, hi) : pack_lo(lo, hi);
-    }
-  }
-}
-
-// CO = 32 or 64 output channels per block: wave (ta, tb) owns input
+// 8 x 16-pixel tiles of the images.  Per tile it stages the input window
+// (tile plus one-pixel halo) and the output gradient in LDS in their
+// natural [pixel][channel] layout, straight from 16-byte global loads, and
+// reads the MFMA operands (K = pixels) with gfx950's transposing LDS read:
+// ds_read_b64_tr_b16 gives each lane of a 16-lane group one channel of four
+// pixels, lanes 4j .. 4j + 3 naming pixel j's 16 channels, so a tap's
+// shifted window is just other per-lane addresses.  One MFMA step covers
+// two rows of the tile (lane group g: row g / 2, columns 8 (g % 2) .. +7,
+// two transposing reads of four pixels each).  Wave (ta, tb) owns input
 // channels 16 ta .. +15 and the CO / 32 output blocks of 16 starting at
-// (CO / 32) tb, for all nine taps (64: twice the MFMAs per staged input
-// window, 0.6 instead of 1.1 LDS operand reads per MFMA).
+// (CO / 32) tb.  The staging loads are unconditional (addresses clamped
+// into the image, values outside selected to zero) and the next tile's
+// loads are in flight while the MFMAs of the current one run.
+constexpr int kWtH = 8, kWtW = 16;
+constexpr int kWtWH = kWtH + 2, kWtWW = kWtW + 2;  // input window
+
+typedef short i16x4 __attribute__((ext_vector_type(4)));
+typedef short i16x8 __attribute__((ext_vector_type(8)));
+
+// Two transposing reads (pixels p and p + 4 of a lane group's eight) as one
+// 8-element MFMA operand.
+__device__ __forceinline__ bf16x8 tr_pair(const bf16_t* lo, const bf16_t* hi) {
+  const i16x4 a = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+      (__attribute__((address_space(3))) i16x4*)(lo));
+  const i16x4 b = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+      (__attribute__((address_space(3))) i16x4*)(hi));
+  const i16x8 c = __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_bit_cast(bf16x8, c);
+}
+
 template <int CO>
 __global__ void __launch_bounds__(kBlock)
     wgrad_lds_kernel(const bf16_t* __restrict__ x, int x_stride, int x_off,
@@ -504,16 +487,20 @@ __global__ void __launch_bounds__(kBlock)
                      const bf16_t* __restrict__ dy, int cout,
                      float* __restrict__ dw, float* __restrict__ db,
                      int tiles_per_block) {
-  constexpr int NBW = CO / 32;           // output blocks of 16 per wave
-  constexpr int DG = CO / 8;             // gradient channel groups
-  constexpr int DJOBS = kWtH * 2 * DG;   // gradient staging jobs
-  static_assert(DJOBS + kWtXJobs <= kBlock, "one staging job per thread");
-  __shared__ u32x4 xw[3 * 32 * kWtXStride];
-  __shared__ u32x4 dt[CO * kWtDStride];
+  constexpr int NBW = CO / 32;                      // output blocks per wave
+  constexpr int DG = CO / 8;                        // gradient channel groups
+  constexpr int XJ = kWtWH * kWtWW * 4;             // window loads (16 B)
+  constexpr int DJ = kWtH * kWtW * DG;              // gradient loads (16 B)
+  constexpr int XIT = (XJ + kBlock - 1) / kBlock;
+  static_assert(DJ % kBlock == 0, "whole rounds of gradient loads");
+  constexpr int DIT = DJ / kBlock;
+  __shared__ __attribute__((aligned(16))) bf16_t xs[kWtWH][kWtWW][32];
+  __shared__ __attribute__((aligned(16))) bf16_t ds[kWtH][kWtW][CO];
   __shared__ float bsum[CO];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   const int g = lane >> 4, i = lane & 15;
+  const int pj = i >> 2, pq = i & 3;  // transposing read: pixel, 4-channel chunk
   const int ta = wave >> 1, tb = wave & 1;
   const int nci = (cin + 31) / 32;
   const int ci0 = (blockIdx.y % nci) * 32, co0 = (blockIdx.y / nci) * CO;
@@ -521,80 +508,67 @@ __global__ void __launch_bounds__(kBlock)
   const long long ntiles = (long long)n * tiles_x * tiles_y;
   const bool bias = db && ci0 == 0;
   const int tid = threadIdx.x;
-  const bool djob = tid < DJOBS;
-  const bool xjob = tid >= DJOBS && tid < DJOBS + kWtXJobs;
-  // job coordinates: channel group, half row, row
-  const int jid = djob ? tid : tid - DJOBS;
-  const int cg = djob ? jid % DG : jid & 3;
-  const int hx = djob ? (jid / DG) & 1 : (jid >> 2) & 1;
-  const int jr = djob ? jid / DG / 2 : jid >> 3;
-  // input channel group inside cin (the loads of the others are clamped to
-  // the last group and zeroed)
-  const bool cok = ci0 + cg * 8 < cin;
-  const int xch = min(ci0 + cg * 8, cin - 8);
+  const int dcg = tid % DG;  // the gradient channel group of all the
+                             // thread's gradient loads (kBlock % DG == 0)
   if (tid < CO) bsum[tid] = 0.0f;
   float bpart[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) bpart[k] = 0.0f;
 
-  // staging registers of the thread's job and the in-image bit of each
-  u32x4 v[10];
-  uint32_t ok = 0;
+  u32x4 xv[XIT], dv[DIT];
+  uint32_t okx = 0, okd = 0;
   auto load = [&](long long t) {
     const int img = (int)(t / (tiles_x * tiles_y));
     const int trem = (int)(t % (tiles_x * tiles_y));
     const int row0 = (trem / tiles_x) * kWtH, col0 = (trem % tiles_x) * kWtW;
-    ok = 0;
-    if (djob) {
-      const int y = row0 + jr;
-      const int yc = min(y, h - 1);
-      const bf16_t* row = dy + ((long long)img * h + yc) * w * cout + co0 + cg * 8;
+    const long long ib = (long long)img * h;
+    okx = okd = 0;
 #pragma unroll
-      for (int p = 0; p < 8; ++p) {
-        const int xx = col0 + 8 * hx + p;
-        if (y < h && xx < w) ok |= 1u << p;
-        v[p] = *reinterpret_cast<const u32x4*>(row + min(xx, w - 1) * cout);
-      }
-    } else if (xjob) {
-      const int y = row0 + jr - 1;
-      const int yc = min(max(y, 0), h - 1);
-      const bf16_t* row = x + ((long long)img * h + yc) * w * x_stride + x_off +
-                          xch;
+    for (int k = 0; k < XIT; ++k) {
+      const int job = min(tid + k * kBlock, XJ - 1);
+      const int pix = job >> 2, cg = job & 3;
+      const int y = row0 - 1 + pix / kWtWW, xx = col0 - 1 + pix % kWtWW;
+      const int ch = ci0 + cg * 8;
+      if (tid + k * kBlock < XJ && ch < cin && y >= 0 && y < h && xx >= 0 &&
+          xx < w)
+        okx |= 1u << k;
+      const int yc = min(max(y, 0), h - 1), xc = min(max(xx, 0), w - 1);
+      xv[k] = *reinterpret_cast<const u32x4*>(
+          x + ((ib + yc) * w + xc) * x_stride + x_off + min(ch, cin - 8));
+    }
 #pragma unroll
-      for (int p = 0; p < 10; ++p) {
-        const int xx = col0 + 8 * hx - 1 + p;
-        const int xc = min(max(xx, 0), w - 1);
-        if (cok && y >= 0 && y < h && xx >= 0 && xx < w) ok |= 1u << p;
-        v[p] = *reinterpret_cast<const u32x4*>(row + xc * x_stride);
-      }
+    for (int k = 0; k < DIT; ++k) {
+      const int job = tid + k * kBlock;
+      const int pix = job / DG;
+      const int y = row0 + pix / kWtW, xx = col0 + pix % kWtW;
+      if (y < h && xx < w) okd |= 1u << k;
+      const int yc = min(y, h - 1), xc = min(xx, w - 1);
+      dv[k] = *reinterpret_cast<const u32x4*>(
+          dy + ((ib + yc) * w + xc) * cout + co0 + dcg * 8);
     }
   };
   auto store = [&]() {
-    u32x4 o[8];
+    const u32x4 zero = u32x4{0u, 0u, 0u, 0u};
 #pragma unroll
-    for (int p = 0; p < 10; ++p)
-      if (!((ok >> p) & 1u)) v[p] = u32x4{0u, 0u, 0u, 0u};
-    if (djob) {
+    for (int k = 0; k < XIT; ++k) {
+      const int job = tid + k * kBlock;
+      if (job < XJ) {
+        const int pix = job >> 2, cg = job & 3;
+        *reinterpret_cast<u32x4*>(&xs[pix / kWtWW][pix % kWtWW][cg * 8]) =
+            ((okx >> k) & 1u) ? xv[k] : zero;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < DIT; ++k) {
+      const int pix = (tid + k * kBlock) / DG;
+      const u32x4 v = ((okd >> k) & 1u) ? dv[k] : zero;
       if (bias) {
 #pragma unroll
-        for (int p = 0; p < 8; ++p)
-#pragma unroll
-          for (int k = 0; k < 8; ++k)
-            bpart[k] += __uint_as_float((k & 1) ? (v[p][k >> 1] & 0xffff0000u)
-                                                : (v[p][k >> 1] << 16));
+        for (int e = 0; e < 8; ++e)
+          bpart[e] += __uint_as_float((e & 1) ? (v[e >> 1] & 0xffff0000u)
+                                              : (v[e >> 1] << 16));
       }
-      transpose8(v, 0, o);
-#pragma unroll
-      for (int c = 0; c < 8; ++c)
-        dt[(cg * 8 + c) * kWtDStride + jr * 2 + hx] = o[c];
-    } else if (xjob) {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        transpose8(v, d, o);
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-          xw[(d * 32 + cg * 8 + c) * kWtXStride + jr * 2 + hx] = o[c];
-      }
+      *reinterpret_cast<u32x4*>(&ds[pix / kWtW][pix % kWtW][dcg * 8]) = v;
     }
   };
 
@@ -613,19 +587,19 @@ __global__ void __launch_bounds__(kBlock)
     if (t + 1 < t_end) load(t + 1);
 #pragma unroll
     for (int s = 0; s < kWtH / 2; ++s) {
-      const int row = 2 * s + (g >> 1), half = g & 1;
+      const int row = 2 * s + (g >> 1), c0 = 8 * (g & 1);
       bf16x8 B[NBW];
 #pragma unroll
-      for (int q = 0; q < NBW; ++q)
-        B[q] = __builtin_bit_cast(
-            bf16x8,
-            dt[((tb * NBW + q) * 16 + i) * kWtDStride + row * 2 + half]);
+      for (int q = 0; q < NBW; ++q) {
+        const int cb = (tb * NBW + q) * 16 + 4 * pq;
+        B[q] = tr_pair(&ds[row][c0 + pj][cb], &ds[row][c0 + 4 + pj][cb]);
+      }
 #pragma unroll
       for (int tap = 0; tap < 9; ++tap) {
-        const int ddy = tap / 3 - 1, d = tap % 3;
-        const bf16x8 A = __builtin_bit_cast(
-            bf16x8, xw[(d * 32 + ta * 16 + i) * kWtXStride +
-                       (row + 1 + ddy) * 2 + half]);
+        const int wy = row + 1 + tap / 3 - 1, wx = c0 + 1 + tap % 3 - 1;
+        const int cb = ta * 16 + 4 * pq;
+        const bf16x8 A =
+            tr_pair(&xs[wy][wx + pj][cb], &xs[wy][wx + 4 + pj][cb]);
 #pragma unroll
         for (int q = 0; q < NBW; ++q)
           acc[q][tap] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
@@ -634,10 +608,10 @@ __global__ void __launch_bounds__(kBlock)
     }
     __syncthreads();
   }
-  // bias: the gradient jobs hold per-channel partial sums
-  if (bias && djob) {
+  // bias: per-channel partial sums of each thread's gradient loads
+  if (bias) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) atomicAdd(&bsum[cg * 8 + k], bpart[k]);
+    for (int e = 0; e < 8; ++e) atomicAdd(&bsum[dcg * 8 + e], bpart[e]);
   }
   __syncthreads();
   if (bias && tid < CO) atomicAdd(&db[co0 + tid], bsum[tid]);
