@@ -69,8 +69,8 @@ struct pie_unet {
   // fp32 host-facing buffers and gradients
   DevBuf<float> x32, out, dout, target;
   DevBuf<uint8_t> mask;
-  DevBuf<float> gA, gB, gC0, gC1, gC2, gP;
-  DevBuf<bf16_t> dyb;
+  DevBuf<float> gA, gC0, gC1, gC2, gP;
+  DevBuf<bf16_t> dyb, dyb2;
   DevBuf<double> acc;
   double ms_total = 0.0, flops_total = 0.0;
 };
@@ -109,12 +109,12 @@ int alloc_all(pie_unet* u) {
   // gradient scratch: the largest per-level working set
   long long ga = std::max({P0 * 32, P1 * 64, P2 * 128, P3 * 256});
   PIE_HIP(u->gA.alloc(ga));
-  PIE_HIP(u->gB.alloc(ga));
   PIE_HIP(u->gC0.alloc(P0 * 96));
   PIE_HIP(u->gC1.alloc(P1 * 192));
   PIE_HIP(u->gC2.alloc(P2 * 384));
   PIE_HIP(u->gP.alloc(std::max({P1 * 32, P2 * 64, P3 * 128})));
   PIE_HIP(u->dyb.alloc(ga));
+  PIE_HIP(u->dyb2.alloc(ga));
   PIE_HIP(u->acc.alloc(2));
   return PIE_OK;
 }
@@ -186,25 +186,33 @@ int forward(pie_unet* u) {
   return PIE_OK;
 }
 
-// Backward of conv l: upstream gradient g (fp32 slice), output y (bf16
-// slice), input x (bf16 slice); writes dX (fp32, Cin channels) into dx
-// unless dx is null.
-int conv_bwd(pie_unet* u, int l, const float* g, int g_stride, int g_off,
-             const Slice& y, const Slice& x, float* dx) {
-  const ConvSpec& c = kConvs[l];
-  const int s = 1 << c.level;
+// Backward of a pair of convolutions la -> lb (lb's input is la's ReLU
+// output xb, and nothing else reads it): lb's data gradient is written by
+// the convolution's epilogue as la's output gradient, masked by xb > 0 and
+// rounded to bf16, the value relu_grad would make from the fp32 gradient.
+int conv_pair_bwd(pie_unet* u, int la, int lb, const float* g, int g_stride,
+                  int g_off, const Slice& yb, const Slice& xb, const Slice& xa,
+                  float* dx) {
+  const ConvSpec& ca = kConvs[la];
+  const ConvSpec& cb = kConvs[lb];
+  const int s = 1 << cb.level;
   const int h = u->H / s, w = u->W / s;
-  const long long P = u->P[c.level];
+  const long long P = u->P[cb.level];
   hipStream_t st = u->stream;
-  PIE_HIP(pie::unet::relu_grad(g, g_stride, g_off, y, (int)P, u->dyb.p, st));
-  PIE_HIP(pie::unet::conv3x3_wgrad(x, u->B, h, w, u->dyb.p, c.cout,
-                                   u->grad.p + u->woff[l],
-                                   u->grad.p + u->boff[l], st));
-  if (dx) {
-    Slice dys{u->dyb.p, c.cout, 0, c.cout};
-    PIE_HIP(pie::unet::conv3x3(dys, u->B, h, w, u->wkt[l].p, c.cin, nullptr,
-                               false, nullptr, dx, c.cin, false, st));
-  }
+  PIE_HIP(pie::unet::relu_grad(g, g_stride, g_off, yb, (int)P, u->dyb.p, st));
+  PIE_HIP(pie::unet::conv3x3_wgrad(xb, u->B, h, w, u->dyb.p, cb.cout,
+                                   u->grad.p + u->woff[lb],
+                                   u->grad.p + u->boff[lb], st));
+  Slice dys{u->dyb.p, cb.cout, 0, cb.cout};
+  Slice dya{u->dyb2.p, cb.cin, 0, cb.cin};
+  PIE_HIP(pie::unet::conv3x3(dys, u->B, h, w, u->wkt[lb].p, cb.cin, nullptr,
+                             false, &dya, nullptr, 0, false, st, &xb));
+  PIE_HIP(pie::unet::conv3x3_wgrad(xa, u->B, h, w, u->dyb2.p, ca.cout,
+                                   u->grad.p + u->woff[la],
+                                   u->grad.p + u->boff[la], st));
+  if (dx)
+    PIE_HIP(pie::unet::conv3x3(dya, u->B, h, w, u->wkt[la].p, ca.cin, nullptr,
+                               false, nullptr, dx, ca.cin, false, st));
   return PIE_OK;
 }
 
@@ -217,43 +225,36 @@ int backward(pie_unet* u) {
   PIE_HIP(pie::unet::head_bwd(sl(u->d0b, 32, 0, 32), (int)u->P[0],
                               u->param.p + hw, u->dout.p, u->grad.p + hw,
                               u->grad.p + hw + 96, u->gA.p, st));
-  PIE_CHECK(conv_bwd(u, D0B, u->gA.p, 32, 0, sl(u->d0b, 32, 0, 32),
-                     sl(u->d0a, 32, 0, 32), u->gB.p));
-  PIE_CHECK(conv_bwd(u, D0A, u->gB.p, 32, 0, sl(u->d0a, 32, 0, 32),
-                     sl(u->c0, 96, 0, 96), u->gC0.p));
+  PIE_CHECK(conv_pair_bwd(u, D0A, D0B, u->gA.p, 32, 0, sl(u->d0b, 32, 0, 32),
+                          sl(u->d0a, 32, 0, 32), sl(u->c0, 96, 0, 96),
+                          u->gC0.p));
   PIE_HIP(pie::unet::up_bwd(u->gC0.p, 96, B, H / 2, W / 2, 64, u->gA.p, st));
-  PIE_CHECK(conv_bwd(u, D1B, u->gA.p, 64, 0, sl(u->d1b, 64, 0, 64),
-                     sl(u->d1a, 64, 0, 64), u->gB.p));
-  PIE_CHECK(conv_bwd(u, D1A, u->gB.p, 64, 0, sl(u->d1a, 64, 0, 64),
-                     sl(u->c1, 192, 0, 192), u->gC1.p));
+  PIE_CHECK(conv_pair_bwd(u, D1A, D1B, u->gA.p, 64, 0, sl(u->d1b, 64, 0, 64),
+                          sl(u->d1a, 64, 0, 64), sl(u->c1, 192, 0, 192),
+                          u->gC1.p));
   PIE_HIP(pie::unet::up_bwd(u->gC1.p, 192, B, H / 4, W / 4, 128, u->gA.p, st));
-  PIE_CHECK(conv_bwd(u, D2B, u->gA.p, 128, 0, sl(u->d2b, 128, 0, 128),
-                     sl(u->d2a, 128, 0, 128), u->gB.p));
-  PIE_CHECK(conv_bwd(u, D2A, u->gB.p, 128, 0, sl(u->d2a, 128, 0, 128),
-                     sl(u->c2, 384, 0, 384), u->gC2.p));
+  PIE_CHECK(conv_pair_bwd(u, D2A, D2B, u->gA.p, 128, 0,
+                          sl(u->d2b, 128, 0, 128), sl(u->d2a, 128, 0, 128),
+                          sl(u->c2, 384, 0, 384), u->gC2.p));
   PIE_HIP(pie::unet::up_bwd(u->gC2.p, 384, B, H / 8, W / 8, 256, u->gA.p, st));
-  PIE_CHECK(conv_bwd(u, BNB, u->gA.p, 256, 0, sl(u->bnb, 256, 0, 256),
-                     sl(u->bna, 256, 0, 256), u->gB.p));
-  PIE_CHECK(conv_bwd(u, BNA, u->gB.p, 256, 0, sl(u->bna, 256, 0, 256),
-                     sl(u->p2, 128, 0, 128), u->gP.p));
+  PIE_CHECK(conv_pair_bwd(u, BNA, BNB, u->gA.p, 256, 0,
+                          sl(u->bnb, 256, 0, 256), sl(u->bna, 256, 0, 256),
+                          sl(u->p2, 128, 0, 128), u->gP.p));
   PIE_HIP(pie::unet::pool_bwd(u->gP.p, B, H / 8, W / 8, 128, u->gC2.p + 256,
                               384, st));
-  PIE_CHECK(conv_bwd(u, E2B, u->gC2.p, 384, 256, sl(u->c2, 384, 256, 128),
-                     sl(u->e2a, 128, 0, 128), u->gB.p));
-  PIE_CHECK(conv_bwd(u, E2A, u->gB.p, 128, 0, sl(u->e2a, 128, 0, 128),
-                     sl(u->p1, 64, 0, 64), u->gP.p));
+  PIE_CHECK(conv_pair_bwd(u, E2A, E2B, u->gC2.p, 384, 256,
+                          sl(u->c2, 384, 256, 128), sl(u->e2a, 128, 0, 128),
+                          sl(u->p1, 64, 0, 64), u->gP.p));
   PIE_HIP(pie::unet::pool_bwd(u->gP.p, B, H / 4, W / 4, 64, u->gC1.p + 128,
                               192, st));
-  PIE_CHECK(conv_bwd(u, E1B, u->gC1.p, 192, 128, sl(u->c1, 192, 128, 64),
-                     sl(u->e1a, 64, 0, 64), u->gB.p));
-  PIE_CHECK(conv_bwd(u, E1A, u->gB.p, 64, 0, sl(u->e1a, 64, 0, 64),
-                     sl(u->p0, 32, 0, 32), u->gP.p));
+  PIE_CHECK(conv_pair_bwd(u, E1A, E1B, u->gC1.p, 192, 128,
+                          sl(u->c1, 192, 128, 64), sl(u->e1a, 64, 0, 64),
+                          sl(u->p0, 32, 0, 32), u->gP.p));
   PIE_HIP(pie::unet::pool_bwd(u->gP.p, B, H / 2, W / 2, 32, u->gC0.p + 64, 96,
                               st));
-  PIE_CHECK(conv_bwd(u, E0B, u->gC0.p, 96, 64, sl(u->c0, 96, 64, 32),
-                     sl(u->e0a, 32, 0, 32), u->gB.p));
-  PIE_CHECK(conv_bwd(u, E0A, u->gB.p, 32, 0, sl(u->e0a, 32, 0, 32),
-                     sl(u->in, 8, 0, 8), nullptr));
+  PIE_CHECK(conv_pair_bwd(u, E0A, E0B, u->gC0.p, 96, 64,
+                          sl(u->c0, 96, 64, 32), sl(u->e0a, 32, 0, 32),
+                          sl(u->in, 8, 0, 8), nullptr));
   return PIE_OK;
 }
 

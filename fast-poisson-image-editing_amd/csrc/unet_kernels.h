@@ -23,11 +23,15 @@ struct Slice {
 // v_mfma_f32_16x16x32_bf16: out[p][co] = sum_k A[p][k] B[k][co] with
 // k = 8 * kb + e, kb = tap * (Cin / 8) + cb, A = in[p + offset(tap)][8cb+e].
 // wk holds B as [kb][Cout][8] bf16.  Output: bf16 (+bias, optional ReLU)
-// into out_bf, and/or fp32 into out_f32 (optionally accumulated).
+// into out_bf, and/or fp32 into out_f32 (optionally accumulated).  With a
+// mask slice (Cout bf16 channels), the bf16 output is masked by it: element
+// (p, co) is written as mask[p][co] > 0 ? v : 0, the backward of a ReLU whose
+// output is the mask, rounded once from the fp32 value (the LDS-tiled path
+// only: Cin and Cout multiples of 32; otherwise hipErrorInvalidValue).
 hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
                    int cout, const float* bias, bool relu, const Slice* out_bf,
                    float* out_f32, int out_f32_stride, bool accumulate,
-                   hipStream_t st);
+                   hipStream_t st, const Slice* mask = nullptr);
 
 // dW[tap][ci][co] += sum_p x[p + offset(tap)][ci] * dy[p][co] (fp32
 // atomics) and db[co] += sum_p dy[p][co]; x is a bf16 slice, dy bf16 with

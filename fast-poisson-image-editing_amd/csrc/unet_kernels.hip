@@ -54,6 +54,8 @@ struct ConvArgs {
   int out_stride, out_off;
   float* out_f32;
   int out_f32_stride, accumulate;
+  const bf16_t* mask;  // bf16 output masked by mask > 0 (LDS-tiled kernel)
+  int mask_stride, mask_off;
 };
 
 // Wave tile: MT x 16 pixels by NT x 16 output channels; a block is four
@@ -293,15 +295,26 @@ __global__ void __launch_bounds__(kBlock) conv3x3_lds_kernel(ConvArgs a) {
       const int pix = job / (CT / 8), c8 = job % (CT / 8);
       const int y = row0 + pix / TW, x = col0 + pix % TW;
       if (job < kJobs && y < a.h && x < a.w) {
-        const f32x4 v0 = *reinterpret_cast<const f32x4*>(&T[pix * TS + c8 * 8]);
-        const f32x4 v1 =
-            *reinterpret_cast<const f32x4*>(&T[pix * TS + c8 * 8 + 4]);
+        f32x4 v0 = *reinterpret_cast<const f32x4*>(&T[pix * TS + c8 * 8]);
+        f32x4 v1 = *reinterpret_cast<const f32x4*>(&T[pix * TS + c8 * 8 + 4]);
+        const long long p = img_base + (long long)y * a.w + x;
+        if (a.mask) {
+          const u32x4 m = *reinterpret_cast<const u32x4*>(
+              a.mask + p * a.mask_stride + a.mask_off + co0 + c8 * 8);
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            if (!(__uint_as_float(m[j] << 16) > 0.0f)) v0[2 * j] = 0.0f;
+            if (!(__uint_as_float(m[j] & 0xffff0000u) > 0.0f)) v0[2 * j + 1] = 0.0f;
+            if (!(__uint_as_float(m[j + 2] << 16) > 0.0f)) v1[2 * j] = 0.0f;
+            if (!(__uint_as_float(m[j + 2] & 0xffff0000u) > 0.0f))
+              v1[2 * j + 1] = 0.0f;
+          }
+        }
         u32x4 o;
         o[0] = (uint32_t)f2bf(v0[0]) | ((uint32_t)f2bf(v0[1]) << 16);
         o[1] = (uint32_t)f2bf(v0[2]) | ((uint32_t)f2bf(v0[3]) << 16);
         o[2] = (uint32_t)f2bf(v1[0]) | ((uint32_t)f2bf(v1[1]) << 16);
         o[3] = (uint32_t)f2bf(v1[2]) | ((uint32_t)f2bf(v1[3]) << 16);
-        const long long p = img_base + (long long)y * a.w + x;
         *reinterpret_cast<u32x4*>(a.out_bf + p * a.out_stride + a.out_off +
                                   co0 + c8 * 8) = o;
       }
@@ -1039,8 +1052,11 @@ int blocks_for(long long n, int cap = 8192) {
 hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
                    int cout, const float* bias, bool relu, const Slice* out_bf,
                    float* out_f32, int out_f32_stride, bool accumulate,
-                   hipStream_t st) {
+                   hipStream_t st, const Slice* mask) {
   if (in.ch % 8 || cout % 16 || in.stride % 8 || in.off % 8)
+    return hipErrorInvalidValue;
+  if (mask && (!out_bf || mask->ch != cout || mask->stride % 8 ||
+               mask->off % 8 || in.ch % 32 || cout % 32))
     return hipErrorInvalidValue;
   ConvArgs a{};
   a.in = static_cast<const bf16_t*>(in.base);
@@ -1060,6 +1076,9 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
   a.out_f32 = out_f32;
   a.out_f32_stride = out_f32_stride;
   a.accumulate = accumulate;
+  a.mask = mask ? static_cast<const bf16_t*>(mask->base) : nullptr;
+  a.mask_stride = mask ? mask->stride : 0;
+  a.mask_off = mask ? mask->off : 0;
   const long long P = (long long)n * h * w;
   const bool vec_out = (!out_bf || (out_bf->stride % 8 == 0 &&
                                     out_bf->off % 8 == 0)) &&
@@ -1077,6 +1096,7 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
                          dim3(kBlock), 0, st, a);
     return hipGetLastError();
   }
+  if (mask) return hipErrorInvalidValue;  // unaligned output slice
   constexpr int MT = 2;
   if (cout % 32 == 0) {
     constexpr int NT = 2;
