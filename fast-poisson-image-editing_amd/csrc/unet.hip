@@ -69,7 +69,7 @@ struct pie_unet {
   // fp32 host-facing buffers and gradients
   DevBuf<float> x32, out, dout, target;
   DevBuf<uint8_t> mask;
-  DevBuf<float> gA, gC0, gC1, gC2, gP;
+  DevBuf<float> gC0, gC1, gC2, gP;
   DevBuf<bf16_t> dyb, dyb2;
   DevBuf<double> acc;
   double ms_total = 0.0, flops_total = 0.0;
@@ -108,7 +108,6 @@ int alloc_all(pie_unet* u) {
   PIE_HIP(u->mask.alloc(P0));
   // gradient scratch: the largest per-level working set
   long long ga = std::max({P0 * 32, P1 * 64, P2 * 128, P3 * 256});
-  PIE_HIP(u->gA.alloc(ga));
   PIE_HIP(u->gC0.alloc(P0 * 96));
   PIE_HIP(u->gC1.alloc(P1 * 192));
   PIE_HIP(u->gC2.alloc(P2 * 384));
@@ -186,20 +185,19 @@ int forward(pie_unet* u) {
   return PIE_OK;
 }
 
-// Backward of a pair of convolutions la -> lb (lb's input is la's ReLU
-// output xb, and nothing else reads it): lb's data gradient is written by
-// the convolution's epilogue as la's output gradient, masked by xb > 0 and
-// rounded to bf16, the value relu_grad would make from the fp32 gradient.
-int conv_pair_bwd(pie_unet* u, int la, int lb, const float* g, int g_stride,
-                  int g_off, const Slice& yb, const Slice& xb, const Slice& xa,
-                  float* dx) {
+// Backward of a pair of convolutions la -> lb, lb's output gradient
+// already in dyb (bf16, masked by lb's ReLU): lb's input is la's ReLU output
+// xb, and nothing else reads it, so lb's data gradient is written by the
+// convolution's epilogue as la's output gradient, masked by xb > 0 and
+// rounded to bf16 once from the fp32 sum.  la's data gradient (fp32) goes to
+// dx unless dx is null.
+int conv_pair_bwd(pie_unet* u, int la, int lb, const Slice& xb,
+                  const Slice& xa, float* dx) {
   const ConvSpec& ca = kConvs[la];
   const ConvSpec& cb = kConvs[lb];
   const int s = 1 << cb.level;
   const int h = u->H / s, w = u->W / s;
-  const long long P = u->P[cb.level];
   hipStream_t st = u->stream;
-  PIE_HIP(pie::unet::relu_grad(g, g_stride, g_off, yb, (int)P, u->dyb.p, st));
   PIE_HIP(pie::unet::conv3x3_wgrad(xb, u->B, h, w, u->dyb.p, cb.cout,
                                    u->grad.p + u->woff[lb],
                                    u->grad.p + u->boff[lb], st));
@@ -220,40 +218,40 @@ int backward(pie_unet* u) {
   hipStream_t st = u->stream;
   const int B = u->B, H = u->H, W = u->W;
   const long long hw = u->woff[kNumConv];
+  bf16_t* dy = u->dyb.p;
   PIE_HIP(hipMemsetAsync(u->grad.p, 0, u->nparam * sizeof(float), st));
-  // head: dout -> gA [P0][32]
+  // decoder: each level's output gradient comes from the head or from the
+  // upsampling backward, with that level's ReLU folded in
   PIE_HIP(pie::unet::head_bwd(sl(u->d0b, 32, 0, 32), (int)u->P[0],
                               u->param.p + hw, u->dout.p, u->grad.p + hw,
-                              u->grad.p + hw + 96, u->gA.p, st));
-  PIE_CHECK(conv_pair_bwd(u, D0A, D0B, u->gA.p, 32, 0, sl(u->d0b, 32, 0, 32),
-                          sl(u->d0a, 32, 0, 32), sl(u->c0, 96, 0, 96),
-                          u->gC0.p));
-  PIE_HIP(pie::unet::up_bwd(u->gC0.p, 96, B, H / 2, W / 2, 64, u->gA.p, st));
-  PIE_CHECK(conv_pair_bwd(u, D1A, D1B, u->gA.p, 64, 0, sl(u->d1b, 64, 0, 64),
-                          sl(u->d1a, 64, 0, 64), sl(u->c1, 192, 0, 192),
-                          u->gC1.p));
-  PIE_HIP(pie::unet::up_bwd(u->gC1.p, 192, B, H / 4, W / 4, 128, u->gA.p, st));
-  PIE_CHECK(conv_pair_bwd(u, D2A, D2B, u->gA.p, 128, 0,
-                          sl(u->d2b, 128, 0, 128), sl(u->d2a, 128, 0, 128),
+                              u->grad.p + hw + 96, dy, st));
+  PIE_CHECK(conv_pair_bwd(u, D0A, D0B, sl(u->d0a, 32, 0, 32),
+                          sl(u->c0, 96, 0, 96), u->gC0.p));
+  PIE_HIP(pie::unet::up_bwd_relu(u->gC0.p, 96, B, H / 2, W / 2,
+                                 sl(u->d1b, 64, 0, 64), dy, st));
+  PIE_CHECK(conv_pair_bwd(u, D1A, D1B, sl(u->d1a, 64, 0, 64),
+                          sl(u->c1, 192, 0, 192), u->gC1.p));
+  PIE_HIP(pie::unet::up_bwd_relu(u->gC1.p, 192, B, H / 4, W / 4,
+                                 sl(u->d2b, 128, 0, 128), dy, st));
+  PIE_CHECK(conv_pair_bwd(u, D2A, D2B, sl(u->d2a, 128, 0, 128),
                           sl(u->c2, 384, 0, 384), u->gC2.p));
-  PIE_HIP(pie::unet::up_bwd(u->gC2.p, 384, B, H / 8, W / 8, 256, u->gA.p, st));
-  PIE_CHECK(conv_pair_bwd(u, BNA, BNB, u->gA.p, 256, 0,
-                          sl(u->bnb, 256, 0, 256), sl(u->bna, 256, 0, 256),
+  PIE_HIP(pie::unet::up_bwd_relu(u->gC2.p, 384, B, H / 8, W / 8,
+                                 sl(u->bnb, 256, 0, 256), dy, st));
+  PIE_CHECK(conv_pair_bwd(u, BNA, BNB, sl(u->bna, 256, 0, 256),
                           sl(u->p2, 128, 0, 128), u->gP.p));
-  PIE_HIP(pie::unet::pool_bwd(u->gP.p, B, H / 8, W / 8, 128, u->gC2.p + 256,
-                              384, st));
-  PIE_CHECK(conv_pair_bwd(u, E2A, E2B, u->gC2.p, 384, 256,
-                          sl(u->c2, 384, 256, 128), sl(u->e2a, 128, 0, 128),
+  // encoder: the skip channels' gradient from the decoder plus the pooling
+  // backward, with the ReLU of the encoder level's output folded in
+  PIE_HIP(pie::unet::pool_bwd_relu(u->gP.p, B, H / 8, W / 8, u->gC2.p, 384,
+                                   256, sl(u->c2, 384, 256, 128), dy, st));
+  PIE_CHECK(conv_pair_bwd(u, E2A, E2B, sl(u->e2a, 128, 0, 128),
                           sl(u->p1, 64, 0, 64), u->gP.p));
-  PIE_HIP(pie::unet::pool_bwd(u->gP.p, B, H / 4, W / 4, 64, u->gC1.p + 128,
-                              192, st));
-  PIE_CHECK(conv_pair_bwd(u, E1A, E1B, u->gC1.p, 192, 128,
-                          sl(u->c1, 192, 128, 64), sl(u->e1a, 64, 0, 64),
+  PIE_HIP(pie::unet::pool_bwd_relu(u->gP.p, B, H / 4, W / 4, u->gC1.p, 192,
+                                   128, sl(u->c1, 192, 128, 64), dy, st));
+  PIE_CHECK(conv_pair_bwd(u, E1A, E1B, sl(u->e1a, 64, 0, 64),
                           sl(u->p0, 32, 0, 32), u->gP.p));
-  PIE_HIP(pie::unet::pool_bwd(u->gP.p, B, H / 2, W / 2, 32, u->gC0.p + 64, 96,
-                              st));
-  PIE_CHECK(conv_pair_bwd(u, E0A, E0B, u->gC0.p, 96, 64,
-                          sl(u->c0, 96, 64, 32), sl(u->e0a, 32, 0, 32),
+  PIE_HIP(pie::unet::pool_bwd_relu(u->gP.p, B, H / 2, W / 2, u->gC0.p, 96, 64,
+                                   sl(u->c0, 96, 64, 32), dy, st));
+  PIE_CHECK(conv_pair_bwd(u, E0A, E0B, sl(u->e0a, 32, 0, 32),
                           sl(u->in, 8, 0, 8), nullptr));
   return PIE_OK;
 }

@@ -52,24 +52,29 @@ hipError_t to_bf16(const float* x, int p, int ch, const Slice& out,
 // 2x2 average pool bf16 -> bf16 (h, w: input size, even)
 hipError_t pool_fwd(const Slice& in, int n, int h, int w, const Slice& out,
                     hipStream_t st);
-// dx[n][2i+a][2j+b][c] += dy[n][i][j][c] / 4   (fp32, dx stride cs)
-hipError_t pool_bwd(const float* dy, int n, int h2, int w2, int ch,
-                    float* dx, int dx_stride, hipStream_t st);
+// Backward of a 2x2 average pool feeding a skip connection, with the ReLU
+// of the skip output y folded in: out[n][2i+a][2j+b][c] =
+// bf16(y > 0 ? gs[..][gs_off + c] + dy[n][i][j][c] / 4 : 0), out [P][y.ch]
+// (gs: fp32, stride gs_stride; channels, strides, offsets multiples of 8)
+hipError_t pool_bwd_relu(const float* dy, int n, int h2, int w2,
+                         const float* gs, int gs_stride, int gs_off,
+                         const Slice& y, bf16_t* out, hipStream_t st);
 // 2x nearest upsample bf16 -> bf16 slice (h, w: input size)
 hipError_t up_fwd(const Slice& in, int n, int h, int w, const Slice& out,
                   hipStream_t st);
-// dx[n][i][j][c] = sum_ab dy[n][2i+a][2j+b][c] (fp32 slice of dy)
-hipError_t up_bwd(const float* dy, int dy_stride, int n, int h, int w, int ch,
-                  float* dx, hipStream_t st);
-// g[p][c] * (y[p][c] > 0) -> bf16 [P][ch] (ReLU gradient, ready for MFMA)
-hipError_t relu_grad(const float* g, int g_stride, int g_off,
-                     const Slice& y, int p, bf16_t* out, hipStream_t st);
+// Backward of the upsample with the ReLU of its input y folded in:
+// out[n][i][j][c] = bf16(y > 0 ? sum_ab dy[n][2i+a][2j+b][c] : 0) (dy fp32,
+// first y.ch channels of stride dy_stride), out [P][y.ch]
+hipError_t up_bwd_relu(const float* dy, int dy_stride, int n, int h, int w,
+                       const Slice& y, bf16_t* out, hipStream_t st);
 // head: out[p][0..3) = sum_c x[p][c] * bf16(w[c][o]) + b[o]  (x bf16 32 ch)
 hipError_t head_fwd(const Slice& x, int p, const float* w, const float* b,
                     float* out, hipStream_t st);
-// head backward: dw, db (atomics), dx[p][c] = sum_o dout[p][o] bf16(w[c][o])
+// head backward: dw, db (atomics), dx[p][c] = bf16(x[p][c] > 0 ?
+// sum_o dout[p][o] bf16(w[c][o]) : 0) (with the backward of the ReLU that
+// made x), dx [P][32]
 hipError_t head_bwd(const Slice& x, int p, const float* w, const float* dout,
-                    float* dw, float* db, float* dx, hipStream_t st);
+                    float* dw, float* db, bf16_t* dx, hipStream_t st);
 // masked mean squared error over solved pixels: loss_sum[0] += sum d^2,
 // count[0] += solved * 3, dout = 2 d (to be scaled by 1/count)
 hipError_t mse(const float* out, const float* target, const uint8_t* mask,

@@ -704,22 +704,6 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 __global__ void __launch_bounds__(kBlock)
-    pool_bwd_kernel(const float* __restrict__ dy, int n, int h2, int w2,
-                    int ch, float* __restrict__ dx, int dx_stride) {
-  const int h = 2 * h2, w = 2 * w2;
-  const long long total = (long long)n * h * w * ch;
-  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < total;
-       id += (long long)gridDim.x * kBlock) {
-    const int c = (int)(id % ch);
-    const long long q = id / ch;
-    const int x = (int)(q % w), y = (int)((q / w) % h);
-    const long long img = q / ((long long)w * h);
-    const long long s = (img * h2 + y / 2) * w2 + x / 2;
-    dx[q * dx_stride + c] += dy[s * ch + c] * 0.25f;
-  }
-}
-
-__global__ void __launch_bounds__(kBlock)
     up_fwd_kernel(const bf16_t* __restrict__ in, int in_stride, int in_off,
                   int n, int h, int w, int ch, bf16_t* __restrict__ out,
                   int out_stride, int out_off) {
@@ -753,24 +737,6 @@ __global__ void __launch_bounds__(kBlock)
     const long long sp = (img * h + y / 2) * w + x / 2;
     *reinterpret_cast<uint4*>(out + q * out_stride + out_off + c) =
         *reinterpret_cast<const uint4*>(in + sp * in_stride + in_off + c);
-  }
-}
-
-__global__ void __launch_bounds__(kBlock)
-    up_bwd_kernel(const float* __restrict__ dy, int dy_stride, int n, int h,
-                  int w, int ch, float* __restrict__ dx) {
-  const int W = 2 * w;
-  const long long total = (long long)n * h * w * ch;
-  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < total;
-       id += (long long)gridDim.x * kBlock) {
-    const int c = (int)(id % ch);
-    const long long q = id / ch;
-    const int x = (int)(q % w), y = (int)((q / w) % h);
-    const long long img = q / ((long long)w * h);
-    const long long b = (img * 2 * h + 2 * y) * W + 2 * x;
-    dx[q * ch + c] = dy[b * dy_stride + c] + dy[(b + 1) * dy_stride + c] +
-                     dy[(b + W) * dy_stride + c] +
-                     dy[(b + W + 1) * dy_stride + c];
   }
 }
 
@@ -812,85 +778,84 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// The backward of the ReLU on a level's output folded into the producer of
+// its gradient: out[p][c] = bf16(y[p][c] > 0 ? g : 0), with g the fp32
+// gradient the producer sums (eight channels per thread; channel counts,
+// strides and offsets multiples of 8).
+__device__ __forceinline__ u32x4 relu_mask_bf16(const u32x4 y, f32x4 g0,
+                                                f32x4 g1) {
+  u32x4 o;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const float ga = j < 2 ? g0[2 * j] : g1[2 * j - 4];
+    const float gb = j < 2 ? g0[2 * j + 1] : g1[2 * j - 3];
+    const float ya = __uint_as_float(y[j] << 16);
+    const float yb = __uint_as_float(y[j] & 0xffff0000u);
+    o[j] = (uint32_t)f2bf(ya > 0.0f ? ga : 0.0f) |
+           ((uint32_t)f2bf(yb > 0.0f ? gb : 0.0f) << 16);
+  }
+  return o;
+}
+
+// Average-pooling backward into a skip connection: for the four pixels p of
+// source pixel s, g = gs[p] + dy[s] / 4 (gs: the skip channels' gradient
+// from the decoder), masked by the skip output y, into out [P][ch].
 __global__ void __launch_bounds__(kBlock)
-    pool_bwd4_kernel(const float* __restrict__ dy, int h2, int w2, int cg,
-                     int total, float* __restrict__ dx, int dx_stride) {
+    pool_bwd_relu8_kernel(const float* __restrict__ dy, int h2, int w2,
+                          int cg, int total, const float* __restrict__ gs,
+                          int gs_stride, int gs_off,
+                          const bf16_t* __restrict__ y, int y_stride,
+                          int y_off, bf16_t* __restrict__ out) {
   const int w = 2 * w2;
   for (int id = blockIdx.x * kBlock + threadIdx.x; id < total;
        id += gridDim.x * kBlock) {
-    const int c = (id % cg) * 4, s = id / cg;  // source pixel [n][h2][w2]
+    const int c = (id % cg) * 8, s = id / cg;  // source pixel [n][h2][w2]
     const int x2 = s % w2, t = s / w2;
     const int y2 = t % h2, img = t / h2;
-    const f32x4 v = *reinterpret_cast<const f32x4*>(dy + (long long)s * cg * 4 + c) *
-                    0.25f;
+    const f32x4* dp = reinterpret_cast<const f32x4*>(dy + (long long)s * cg * 8 + c);
+    const f32x4 v0 = dp[0] * 0.25f, v1 = dp[1] * 0.25f;
     const long long b = ((long long)img * 2 * h2 + 2 * y2) * w + 2 * x2;
     const long long q[4] = {b, b + 1, b + w, b + w + 1};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      f32x4* o = reinterpret_cast<f32x4*>(dx + q[k] * dx_stride + c);
-      *o = *o + v;
+      const f32x4* gp =
+          reinterpret_cast<const f32x4*>(gs + q[k] * gs_stride + gs_off + c);
+      const u32x4 yv =
+          *reinterpret_cast<const u32x4*>(y + q[k] * y_stride + y_off + c);
+      *reinterpret_cast<u32x4*>(out + q[k] * cg * 8 + c) =
+          relu_mask_bf16(yv, gp[0] + v0, gp[1] + v1);
     }
   }
 }
 
+// Nearest-upsampling backward: g = the sum of the four fine pixels'
+// gradients (first ch channels of dy), masked by the coarse output y, into
+// out [P][ch].
 __global__ void __launch_bounds__(kBlock)
-    up_bwd4_kernel(const float* __restrict__ dy, int dy_stride, int h, int w,
-                   int cg, int total, float* __restrict__ dx) {
+    up_bwd_relu8_kernel(const float* __restrict__ dy, int dy_stride, int h,
+                        int w, int cg, int total,
+                        const bf16_t* __restrict__ y, int y_stride, int y_off,
+                        bf16_t* __restrict__ out) {
   const int W = 2 * w;
   for (int id = blockIdx.x * kBlock + threadIdx.x; id < total;
        id += gridDim.x * kBlock) {
-    const int c = (id % cg) * 4, q = id / cg;
+    const int c = (id % cg) * 8, q = id / cg;
     const int x = q % w, t = q / w;
-    const int y = t % h, img = t / h;
-    const long long b = ((long long)img * 2 * h + 2 * y) * W + 2 * x;
-    const f32x4 d0 = *reinterpret_cast<const f32x4*>(dy + b * dy_stride + c);
-    const f32x4 d1 = *reinterpret_cast<const f32x4*>(dy + (b + 1) * dy_stride + c);
-    const f32x4 d2 = *reinterpret_cast<const f32x4*>(dy + (b + W) * dy_stride + c);
-    const f32x4 d3 =
-        *reinterpret_cast<const f32x4*>(dy + (b + W + 1) * dy_stride + c);
-    *reinterpret_cast<f32x4*>(dx + (long long)q * cg * 4 + c) = d0 + d1 + d2 + d3;
-  }
-}
-
-__global__ void __launch_bounds__(kBlock)
-    relu_grad_kernel(const float* __restrict__ g, int g_stride, int g_off,
-                     const bf16_t* __restrict__ y, int y_stride, int y_off,
-                     long long p, int ch, bf16_t* __restrict__ out) {
-  const long long total = p * ch;
-  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < total;
-       id += (long long)gridDim.x * kBlock) {
-    const int c = (int)(id % ch);
-    const long long q = id / ch;
-    const float yv = bf2f(y[q * y_stride + y_off + c]);
-    out[id] = f2bf(yv > 0.0f ? g[q * g_stride + g_off + c] : 0.0f);
-  }
-}
-
-// relu_grad_kernel on 8 channels per thread (channels, strides and offsets
-// multiples of 8 for y, of 4 for g; p * ch / 8 below 2^31)
-__global__ void __launch_bounds__(kBlock)
-    relu_grad8_kernel(const float* __restrict__ g, int g_stride, int g_off,
-                      const bf16_t* __restrict__ y, int y_stride, int y_off,
-                      int total8, int ch8, bf16_t* __restrict__ out) {
-  for (int id = blockIdx.x * kBlock + threadIdx.x; id < total8;
-       id += gridDim.x * kBlock) {
-    const int q = id / ch8, c = (id - q * ch8) * 8;
-    const u32x4 yv = *reinterpret_cast<const u32x4*>(
-        y + (long long)q * y_stride + y_off + c);
-    const f32x4* gp = reinterpret_cast<const f32x4*>(
-        g + (long long)q * g_stride + g_off + c);
-    const f32x4 g0 = gp[0], g1 = gp[1];
-    u32x4 o;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const float ga = j < 2 ? g0[2 * j] : g1[2 * j - 4];
-      const float gb = j < 2 ? g0[2 * j + 1] : g1[2 * j - 3];
-      const float ya = __uint_as_float(yv[j] << 16);
-      const float yb = __uint_as_float(yv[j] & 0xffff0000u);
-      o[j] = (uint32_t)f2bf(ya > 0.0f ? ga : 0.0f) |
-             ((uint32_t)f2bf(yb > 0.0f ? gb : 0.0f) << 16);
-    }
-    *reinterpret_cast<u32x4*>(out + (long long)id * 8) = o;
+    const int yy = t % h, img = t / h;
+    const long long b = ((long long)img * 2 * h + 2 * yy) * W + 2 * x;
+    const f32x4* d0 = reinterpret_cast<const f32x4*>(dy + b * dy_stride + c);
+    const f32x4* d1 =
+        reinterpret_cast<const f32x4*>(dy + (b + 1) * dy_stride + c);
+    const f32x4* d2 =
+        reinterpret_cast<const f32x4*>(dy + (b + W) * dy_stride + c);
+    const f32x4* d3 =
+        reinterpret_cast<const f32x4*>(dy + (b + W + 1) * dy_stride + c);
+    const f32x4 s0 = d0[0] + d1[0] + d2[0] + d3[0];
+    const f32x4 s1 = d0[1] + d1[1] + d2[1] + d3[1];
+    const u32x4 yv =
+        *reinterpret_cast<const u32x4*>(y + (long long)q * y_stride + y_off + c);
+    *reinterpret_cast<u32x4*>(out + (long long)q * cg * 8 + c) =
+        relu_mask_bf16(yv, s0, s1);
   }
 }
 
@@ -920,9 +885,10 @@ __global__ void __launch_bounds__(kBlock)
     head_bwd_kernel(const bf16_t* __restrict__ x, int x_stride, int x_off,
                     long long p, const float* __restrict__ w,
                     const float* __restrict__ dout, float* dw, float* db,
-                    float* __restrict__ dx) {
+                    bf16_t* __restrict__ dx) {
   // four threads per pixel, eight channels each: one 16-byte load of the
-  // input, two 16-byte stores of its gradient
+  // input, one 16-byte store of its gradient masked by input > 0 (the
+  // backward of the ReLU that made the input), in bf16
   __shared__ float ws[32 * 3];
   __shared__ float wsum[kBlock / kWave][99];
   for (int k = threadIdx.x; k < 96; k += kBlock) ws[k] = bf2f(f2bf(w[k]));
@@ -956,9 +922,8 @@ __global__ void __launch_bounds__(kBlock)
       o[k >> 2][k & 3] =
           d0 * ws[c * 3 + 0] + d1 * ws[c * 3 + 1] + d2 * ws[c * 3 + 2];
     }
-    f32x4* dp = reinterpret_cast<f32x4*>(dx + q * 32 + c8 * 8);
-    dp[0] = o[0];
-    dp[1] = o[1];
+    *reinterpret_cast<u32x4*>(dx + q * 32 + c8 * 8) =
+        relu_mask_bf16(xv, o[0], o[1]);
   }
   // block sums of the 99 parameter gradients: butterfly sums over the lanes
   // of a wave that hold the same channels (fixed indices, so the
@@ -1180,19 +1145,18 @@ hipError_t pool_fwd(const Slice& in, int n, int h, int w, const Slice& out,
   return hipGetLastError();
 }
 
-hipError_t pool_bwd(const float* dy, int n, int h2, int w2, int ch, float* dx,
-                    int dx_stride, hipStream_t st) {
-  const long long total4 = (long long)n * h2 * w2 * (ch / 4);
-  if (ch % 4 == 0 && dx_stride % 4 == 0 &&
-      reinterpret_cast<uintptr_t>(dx) % 16 == 0 &&
-      reinterpret_cast<uintptr_t>(dy) % 16 == 0 && total4 < (1LL << 31)) {
-    hipLaunchKernelGGL(pool_bwd4_kernel, dim3(blocks_for(total4)), dim3(kBlock),
-                       0, st, dy, h2, w2, ch / 4, (int)total4, dx, dx_stride);
-    return hipGetLastError();
-  }
-  hipLaunchKernelGGL(pool_bwd_kernel,
-                     dim3(blocks_for((long long)n * h2 * w2 * 4 * ch)),
-                     dim3(kBlock), 0, st, dy, n, h2, w2, ch, dx, dx_stride);
+hipError_t pool_bwd_relu(const float* dy, int n, int h2, int w2,
+                         const float* gs, int gs_stride, int gs_off,
+                         const Slice& y, bf16_t* out, hipStream_t st) {
+  const int ch = y.ch;
+  const long long total8 = (long long)n * h2 * w2 * (ch / 8);
+  if (ch % 8 || gs_stride % 8 || gs_off % 8 || y.stride % 8 || y.off % 8 ||
+      total8 >= (1LL << 31))
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(pool_bwd_relu8_kernel, dim3(blocks_for(total8)),
+                     dim3(kBlock), 0, st, dy, h2, w2, ch / 8, (int)total8, gs,
+                     gs_stride, gs_off, static_cast<const bf16_t*>(y.base),
+                     y.stride, y.off, out);
   return hipGetLastError();
 }
 
@@ -1216,37 +1180,17 @@ hipError_t up_fwd(const Slice& in, int n, int h, int w, const Slice& out,
   return hipGetLastError();
 }
 
-hipError_t up_bwd(const float* dy, int dy_stride, int n, int h, int w, int ch,
-                  float* dx, hipStream_t st) {
-  const long long total4 = (long long)n * h * w * (ch / 4);
-  if (ch % 4 == 0 && dy_stride % 4 == 0 &&
-      reinterpret_cast<uintptr_t>(dx) % 16 == 0 &&
-      reinterpret_cast<uintptr_t>(dy) % 16 == 0 && total4 < (1LL << 31)) {
-    hipLaunchKernelGGL(up_bwd4_kernel, dim3(blocks_for(total4)), dim3(kBlock),
-                       0, st, dy, dy_stride, h, w, ch / 4, (int)total4, dx);
-    return hipGetLastError();
-  }
-  hipLaunchKernelGGL(up_bwd_kernel,
-                     dim3(blocks_for((long long)n * h * w * ch)),
-                     dim3(kBlock), 0, st, dy, dy_stride, n, h, w, ch, dx);
-  return hipGetLastError();
-}
-
-hipError_t relu_grad(const float* g, int g_stride, int g_off, const Slice& y,
-                     int p, bf16_t* out, hipStream_t st) {
-  const long long total8 = (long long)p * y.ch / 8;
-  if (y.ch % 8 == 0 && y.stride % 8 == 0 && y.off % 8 == 0 &&
-      g_stride % 4 == 0 && g_off % 4 == 0 && total8 < (1LL << 31)) {
-    hipLaunchKernelGGL(relu_grad8_kernel, dim3(blocks_for(total8)),
-                       dim3(kBlock), 0, st, g, g_stride, g_off,
-                       static_cast<const bf16_t*>(y.base), y.stride, y.off,
-                       (int)total8, y.ch / 8, out);
-    return hipGetLastError();
-  }
-  hipLaunchKernelGGL(relu_grad_kernel, dim3(blocks_for((long long)p * y.ch)),
-                     dim3(kBlock), 0, st, g, g_stride, g_off,
-                     static_cast<const bf16_t*>(y.base), y.stride, y.off,
-                     (long long)p, y.ch, out);
+hipError_t up_bwd_relu(const float* dy, int dy_stride, int n, int h, int w,
+                       const Slice& y, bf16_t* out, hipStream_t st) {
+  const int ch = y.ch;
+  const long long total8 = (long long)n * h * w * (ch / 8);
+  if (ch % 8 || dy_stride % 8 || y.stride % 8 || y.off % 8 ||
+      total8 >= (1LL << 31))
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(up_bwd_relu8_kernel, dim3(blocks_for(total8)),
+                     dim3(kBlock), 0, st, dy, dy_stride, h, w, ch / 8,
+                     (int)total8, static_cast<const bf16_t*>(y.base), y.stride,
+                     y.off, out);
   return hipGetLastError();
 }
 
@@ -1259,7 +1203,7 @@ hipError_t head_fwd(const Slice& x, int p, const float* w, const float* b,
 }
 
 hipError_t head_bwd(const Slice& x, int p, const float* w, const float* dout,
-                    float* dw, float* db, float* dx, hipStream_t st) {
+                    float* dw, float* db, bf16_t* dx, hipStream_t st) {
   if (x.ch != 32 || x.stride % 8 || x.off % 8) return hipErrorInvalidValue;
   hipLaunchKernelGGL(head_bwd_kernel, dim3(blocks_for(4LL * p, 1024)),
                      dim3(kBlock),
