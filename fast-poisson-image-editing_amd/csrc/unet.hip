@@ -190,9 +190,11 @@ int forward(pie_unet* u) {
 // xb, and nothing else reads it, so lb's data gradient is written by the
 // convolution's epilogue as la's output gradient, masked by xb > 0 and
 // rounded to bf16 once from the fp32 sum.  la's data gradient (fp32) goes to
-// dx unless dx is null.
+// dx unless dx is null; with up, its first up->ch channels (an upsampled
+// input) go through the upsampling backward into up->out instead.
 int conv_pair_bwd(pie_unet* u, int la, int lb, const Slice& xb,
-                  const Slice& xa, float* dx) {
+                  const Slice& xa, float* dx,
+                  const pie::unet::UpSum* up = nullptr) {
   const ConvSpec& ca = kConvs[la];
   const ConvSpec& cb = kConvs[lb];
   const int s = 1 << cb.level;
@@ -210,7 +212,8 @@ int conv_pair_bwd(pie_unet* u, int la, int lb, const Slice& xb,
                                    u->grad.p + u->boff[la], st));
   if (dx)
     PIE_HIP(pie::unet::conv3x3(dya, u->B, h, w, u->wkt[la].p, ca.cin, nullptr,
-                               false, nullptr, dx, ca.cin, false, st));
+                               false, nullptr, dx, ca.cin, false, st, nullptr,
+                               up));
   return PIE_OK;
 }
 
@@ -225,18 +228,17 @@ int backward(pie_unet* u) {
   PIE_HIP(pie::unet::head_bwd(sl(u->d0b, 32, 0, 32), (int)u->P[0],
                               u->param.p + hw, u->dout.p, u->grad.p + hw,
                               u->grad.p + hw + 96, dy, st));
+  // the decoder's first convolutions take an upsampled input: their data
+  // gradient's epilogue makes the coarser level's output gradient
+  const pie::unet::UpSum up0{dy, 64, sl(u->d1b, 64, 0, 64)};
   PIE_CHECK(conv_pair_bwd(u, D0A, D0B, sl(u->d0a, 32, 0, 32),
-                          sl(u->c0, 96, 0, 96), u->gC0.p));
-  PIE_HIP(pie::unet::up_bwd_relu(u->gC0.p, 96, B, H / 2, W / 2,
-                                 sl(u->d1b, 64, 0, 64), dy, st));
+                          sl(u->c0, 96, 0, 96), u->gC0.p, &up0));
+  const pie::unet::UpSum up1{dy, 128, sl(u->d2b, 128, 0, 128)};
   PIE_CHECK(conv_pair_bwd(u, D1A, D1B, sl(u->d1a, 64, 0, 64),
-                          sl(u->c1, 192, 0, 192), u->gC1.p));
-  PIE_HIP(pie::unet::up_bwd_relu(u->gC1.p, 192, B, H / 4, W / 4,
-                                 sl(u->d2b, 128, 0, 128), dy, st));
+                          sl(u->c1, 192, 0, 192), u->gC1.p, &up1));
+  const pie::unet::UpSum up2{dy, 256, sl(u->bnb, 256, 0, 256)};
   PIE_CHECK(conv_pair_bwd(u, D2A, D2B, sl(u->d2a, 128, 0, 128),
-                          sl(u->c2, 384, 0, 384), u->gC2.p));
-  PIE_HIP(pie::unet::up_bwd_relu(u->gC2.p, 384, B, H / 8, W / 8,
-                                 sl(u->bnb, 256, 0, 256), dy, st));
+                          sl(u->c2, 384, 0, 384), u->gC2.p, &up2));
   PIE_CHECK(conv_pair_bwd(u, BNA, BNB, sl(u->bna, 256, 0, 256),
                           sl(u->p2, 128, 0, 128), u->gP.p));
   // encoder: the skip channels' gradient from the decoder plus the pooling

@@ -28,10 +28,24 @@ struct Slice {
 // (p, co) is written as mask[p][co] > 0 ? v : 0, the backward of a ReLU whose
 // output is the mask, rounded once from the fp32 value (the LDS-tiled path
 // only: Cin and Cout multiples of 32; otherwise hipErrorInvalidValue).
+// With up (data gradient of a convolution whose input is a nearest 2x
+// upsample in its first up->ch channels, concatenated with a skip
+// connection): the first up->ch output channels are not written to out_f32
+// but summed over each 2 x 2 pixel block (fine pixels (0,0), (0,1), (1,0),
+// (1,1) in that order), masked by up->mask > 0 of the half-resolution level
+// and written as bf16 [n][h/2][w/2][up->ch] to up->out (the upsampling
+// backward with the backward of the ReLU that made its input); up->ch a
+// multiple of 64, Cin and Cout of 32, h and w even, no bf16 output.
+struct UpSum {
+  bf16_t* out;
+  int ch;
+  Slice mask;
+};
 hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
                    int cout, const float* bias, bool relu, const Slice* out_bf,
                    float* out_f32, int out_f32_stride, bool accumulate,
-                   hipStream_t st, const Slice* mask = nullptr);
+                   hipStream_t st, const Slice* mask = nullptr,
+                   const UpSum* up = nullptr);
 
 // dW[tap][ci][co] += sum_p x[p + offset(tap)][ci] * dy[p][co] (fp32
 // atomics) and db[co] += sum_p dy[p][co]; x is a bf16 slice, dy bf16 with
@@ -62,11 +76,6 @@ hipError_t pool_bwd_relu(const float* dy, int n, int h2, int w2,
 // 2x nearest upsample bf16 -> bf16 slice (h, w: input size)
 hipError_t up_fwd(const Slice& in, int n, int h, int w, const Slice& out,
                   hipStream_t st);
-// Backward of the upsample with the ReLU of its input y folded in:
-// out[n][i][j][c] = bf16(y > 0 ? sum_ab dy[n][2i+a][2j+b][c] : 0) (dy fp32,
-// first y.ch channels of stride dy_stride), out [P][y.ch]
-hipError_t up_bwd_relu(const float* dy, int dy_stride, int n, int h, int w,
-                       const Slice& y, bf16_t* out, hipStream_t st);
 // head: out[p][0..3) = sum_c x[p][c] * bf16(w[c][o]) + b[o]  (x bf16 32 ch)
 hipError_t head_fwd(const Slice& x, int p, const float* w, const float* b,
                     float* out, hipStream_t st);

@@ -42,6 +42,25 @@ __device__ __forceinline__ bf16x8 zero8() {
   return __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0));
 }
 
+// The backward of the ReLU on a level's output folded into the producer of
+// its gradient: out[p][c] = bf16(y[p][c] > 0 ? g : 0), with g the fp32
+// gradient the producer sums (eight channels per thread; channel counts,
+// strides and offsets multiples of 8).
+__device__ __forceinline__ u32x4 relu_mask_bf16(const u32x4 y, f32x4 g0,
+                                                f32x4 g1) {
+  u32x4 o;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const float ga = j < 2 ? g0[2 * j] : g1[2 * j - 4];
+    const float gb = j < 2 ? g0[2 * j + 1] : g1[2 * j - 3];
+    const float ya = __uint_as_float(y[j] << 16);
+    const float yb = __uint_as_float(y[j] & 0xffff0000u);
+    o[j] = (uint32_t)f2bf(ya > 0.0f ? ga : 0.0f) |
+           ((uint32_t)f2bf(yb > 0.0f ? gb : 0.0f) << 16);
+  }
+  return o;
+}
+
 struct ConvArgs {
   const bf16_t* in;
   int in_stride, in_off, cin;
@@ -56,6 +75,14 @@ struct ConvArgs {
   int out_f32_stride, accumulate;
   const bf16_t* mask;  // bf16 output masked by mask > 0 (LDS-tiled kernel)
   int mask_stride, mask_off;
+  // upsampling backward folded in (LDS-tiled kernel): output channels below
+  // up_ch are summed over 2 x 2 pixel blocks, masked by up_mask > 0 of the
+  // half-resolution level and written as bf16 [n][h/2][w/2][up_ch] to up_out;
+  // the other channels go to out_f32 as usual
+  bf16_t* up_out;
+  int up_ch;
+  const bf16_t* up_mask;
+  int up_mask_stride, up_mask_off;
 };
 
 // Wave tile: MT x 16 pixels by NT x 16 output channels; a block is four
@@ -218,8 +245,17 @@ __global__ void __launch_bounds__(kBlock) conv3x3_lds_kernel(ConvArgs a) {
   constexpr int CT = 16 * NT;
   constexpr int kWinJobs = WH * WW * 4, kWinIt = (kWinJobs + kBlock - 1) / kBlock;
   constexpr int kWtJobs = 9 * 4 * CT, kWtIt = (kWtJobs + kBlock - 1) / kBlock;
-  __shared__ u32x4 win[WH][WW][4];
-  __shared__ u32x4 ws[9][4][CT];
+  // staging arrays, reused for the output tile after the last barrier
+  struct Stage {
+    u32x4 win[WH][WW][4];
+    u32x4 ws[9][4][CT];
+  };
+  constexpr int TS = CT + 4;  // padded output row: lane groups on distinct banks
+  constexpr int kOutBytes = TH * TW * TS * 4;
+  __shared__ __attribute__((aligned(16)))
+  unsigned char smem[sizeof(Stage) > kOutBytes ? sizeof(Stage) : kOutBytes];
+  auto& win = reinterpret_cast<Stage*>(smem)->win;
+  auto& ws = reinterpret_cast<Stage*>(smem)->ws;
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   const int g = lane >> 4, i = lane & 15;
@@ -270,9 +306,8 @@ __global__ void __launch_bounds__(kBlock) conv3x3_lds_kernel(ConvArgs a) {
   // epilogue through LDS (the staging arrays are free after the last
   // barrier): bias and ReLU applied per lane into a [pixel][channel] fp32
   // tile, then 16-byte stores of 8 bf16 or 4 fp32 channels per thread
-  constexpr int TS = CT + 4;  // padded row: lane groups on distinct banks
-  static_assert(TH * TW * TS * 4 <= (int)sizeof(ws), "output tile fits");
-  float* T = reinterpret_cast<float*>(&ws[0][0][0]);
+  float* T = reinterpret_cast<float*>(smem);
+  const bool up = a.up_out && co0 < a.up_ch;
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) {
     const int co = nt * 16 + i;
@@ -320,7 +355,40 @@ __global__ void __launch_bounds__(kBlock) conv3x3_lds_kernel(ConvArgs a) {
       }
     }
   }
-  if (a.out_f32) {
+  if (up) {
+    // coarse pixel (row0 / 2 + cy, col0 / 2 + cx): the fine pixels in the
+    // order (0, 0), (0, 1), (1, 0), (1, 1)
+    constexpr int kJobs = (TH / 2) * (TW / 2) * (CT / 8);
+    const int h2 = a.h / 2, w2 = a.w / 2;
+#pragma unroll
+    for (int k = 0; k < (kJobs + kBlock - 1) / kBlock; ++k) {
+      const int job = threadIdx.x + k * kBlock;
+      const int cp = job / (CT / 8), c8 = job % (CT / 8);
+      const int cy = cp / (TW / 2), cx = cp % (TW / 2);
+      const int y2 = row0 / 2 + cy, x2 = col0 / 2 + cx;
+      if (job < kJobs && y2 < h2 && x2 < w2) {
+        const int f = (2 * cy) * TW + 2 * cx;
+        const float* t0 = &T[f * TS + c8 * 8];
+        const float* t1 = t0 + TS;
+        const float* t2 = t0 + TW * TS;
+        const float* t3 = t2 + TS;
+        f32x4 s0 = *reinterpret_cast<const f32x4*>(t0);
+        f32x4 s1 = *reinterpret_cast<const f32x4*>(t0 + 4);
+        s0 = s0 + *reinterpret_cast<const f32x4*>(t1);
+        s1 = s1 + *reinterpret_cast<const f32x4*>(t1 + 4);
+        s0 = s0 + *reinterpret_cast<const f32x4*>(t2);
+        s1 = s1 + *reinterpret_cast<const f32x4*>(t2 + 4);
+        s0 = s0 + *reinterpret_cast<const f32x4*>(t3);
+        s1 = s1 + *reinterpret_cast<const f32x4*>(t3 + 4);
+        const long long q = ((long long)img * h2 + y2) * w2 + x2;
+        const u32x4 m = *reinterpret_cast<const u32x4*>(
+            a.up_mask + q * a.up_mask_stride + a.up_mask_off + co0 + c8 * 8);
+        *reinterpret_cast<u32x4*>(a.up_out + q * a.up_ch + co0 + c8 * 8) =
+            relu_mask_bf16(m, s0, s1);
+      }
+    }
+  }
+  if (a.out_f32 && !up) {
     constexpr int kJobs = TH * TW * (CT / 4);
 #pragma unroll
     for (int k = 0; k < (kJobs + kBlock - 1) / kBlock; ++k) {
@@ -778,25 +846,6 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// The backward of the ReLU on a level's output folded into the producer of
-// its gradient: out[p][c] = bf16(y[p][c] > 0 ? g : 0), with g the fp32
-// gradient the producer sums (eight channels per thread; channel counts,
-// strides and offsets multiples of 8).
-__device__ __forceinline__ u32x4 relu_mask_bf16(const u32x4 y, f32x4 g0,
-                                                f32x4 g1) {
-  u32x4 o;
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const float ga = j < 2 ? g0[2 * j] : g1[2 * j - 4];
-    const float gb = j < 2 ? g0[2 * j + 1] : g1[2 * j - 3];
-    const float ya = __uint_as_float(y[j] << 16);
-    const float yb = __uint_as_float(y[j] & 0xffff0000u);
-    o[j] = (uint32_t)f2bf(ya > 0.0f ? ga : 0.0f) |
-           ((uint32_t)f2bf(yb > 0.0f ? gb : 0.0f) << 16);
-  }
-  return o;
-}
-
 // Average-pooling backward into a skip connection: for the four pixels p of
 // source pixel s, g = gs[p] + dy[s] / 4 (gs: the skip channels' gradient
 // from the decoder), masked by the skip output y, into out [P][ch].
@@ -825,37 +874,6 @@ __global__ void __launch_bounds__(kBlock)
       *reinterpret_cast<u32x4*>(out + q[k] * cg * 8 + c) =
           relu_mask_bf16(yv, gp[0] + v0, gp[1] + v1);
     }
-  }
-}
-
-// Nearest-upsampling backward: g = the sum of the four fine pixels'
-// gradients (first ch channels of dy), masked by the coarse output y, into
-// out [P][ch].
-__global__ void __launch_bounds__(kBlock)
-    up_bwd_relu8_kernel(const float* __restrict__ dy, int dy_stride, int h,
-                        int w, int cg, int total,
-                        const bf16_t* __restrict__ y, int y_stride, int y_off,
-                        bf16_t* __restrict__ out) {
-  const int W = 2 * w;
-  for (int id = blockIdx.x * kBlock + threadIdx.x; id < total;
-       id += gridDim.x * kBlock) {
-    const int c = (id % cg) * 8, q = id / cg;
-    const int x = q % w, t = q / w;
-    const int yy = t % h, img = t / h;
-    const long long b = ((long long)img * 2 * h + 2 * yy) * W + 2 * x;
-    const f32x4* d0 = reinterpret_cast<const f32x4*>(dy + b * dy_stride + c);
-    const f32x4* d1 =
-        reinterpret_cast<const f32x4*>(dy + (b + 1) * dy_stride + c);
-    const f32x4* d2 =
-        reinterpret_cast<const f32x4*>(dy + (b + W) * dy_stride + c);
-    const f32x4* d3 =
-        reinterpret_cast<const f32x4*>(dy + (b + W + 1) * dy_stride + c);
-    const f32x4 s0 = d0[0] + d1[0] + d2[0] + d3[0];
-    const f32x4 s1 = d0[1] + d1[1] + d2[1] + d3[1];
-    const u32x4 yv =
-        *reinterpret_cast<const u32x4*>(y + (long long)q * y_stride + y_off + c);
-    *reinterpret_cast<u32x4*>(out + (long long)q * cg * 8 + c) =
-        relu_mask_bf16(yv, s0, s1);
   }
 }
 
@@ -1017,11 +1035,17 @@ int blocks_for(long long n, int cap = 8192) {
 hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
                    int cout, const float* bias, bool relu, const Slice* out_bf,
                    float* out_f32, int out_f32_stride, bool accumulate,
-                   hipStream_t st, const Slice* mask) {
+                   hipStream_t st, const Slice* mask, const UpSum* up) {
   if (in.ch % 8 || cout % 16 || in.stride % 8 || in.off % 8)
     return hipErrorInvalidValue;
   if (mask && (!out_bf || mask->ch != cout || mask->stride % 8 ||
                mask->off % 8 || in.ch % 32 || cout % 32))
+    return hipErrorInvalidValue;
+  if (up && (!up->out || up->ch <= 0 || up->ch % 64 || up->ch > cout ||
+             up->mask.ch != up->ch || up->mask.stride % 8 ||
+             up->mask.off % 8 || h % 2 || w % 2 || in.ch % 32 ||
+             cout % 32 || (up->ch < cout && !out_f32) || out_bf ||
+             accumulate))
     return hipErrorInvalidValue;
   ConvArgs a{};
   a.in = static_cast<const bf16_t*>(in.base);
@@ -1044,6 +1068,11 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
   a.mask = mask ? static_cast<const bf16_t*>(mask->base) : nullptr;
   a.mask_stride = mask ? mask->stride : 0;
   a.mask_off = mask ? mask->off : 0;
+  a.up_out = up ? up->out : nullptr;
+  a.up_ch = up ? up->ch : 0;
+  a.up_mask = up ? static_cast<const bf16_t*>(up->mask.base) : nullptr;
+  a.up_mask_stride = up ? up->mask.stride : 0;
+  a.up_mask_off = up ? up->mask.off : 0;
   const long long P = (long long)n * h * w;
   const bool vec_out = (!out_bf || (out_bf->stride % 8 == 0 &&
                                     out_bf->off % 8 == 0)) &&
@@ -1053,7 +1082,12 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
     constexpr int LMT = 2;
     const unsigned tiles = (unsigned)n * ((h + 4 * LMT - 1) / (4 * LMT)) *
                            ((w + 15) / 16);
-    if (cout % 64 == 0)
+    if (cout % 64 == 0 && in.ch >= 128) {
+      // deep inputs: 16 x 16-pixel tiles, twice the MFMAs per staged weight
+      const unsigned tiles4 = (unsigned)n * ((h + 15) / 16) * ((w + 15) / 16);
+      hipLaunchKernelGGL((conv3x3_lds_kernel<4, 4>), dim3(tiles4, cout / 64),
+                         dim3(kBlock), 0, st, a);
+    } else if (cout % 64 == 0)
       hipLaunchKernelGGL((conv3x3_lds_kernel<LMT, 4>), dim3(tiles, cout / 64),
                          dim3(kBlock), 0, st, a);
     else
@@ -1061,7 +1095,7 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
                          dim3(kBlock), 0, st, a);
     return hipGetLastError();
   }
-  if (mask) return hipErrorInvalidValue;  // unaligned output slice
+  if (mask || up) return hipErrorInvalidValue;  // unaligned output slice
   constexpr int MT = 2;
   if (cout % 32 == 0) {
     constexpr int NT = 2;
@@ -1177,20 +1211,6 @@ hipError_t up_fwd(const Slice& in, int n, int h, int w, const Slice& out,
                      dim3(kBlock), 0, st, static_cast<const bf16_t*>(in.base),
                      in.stride, in.off, n, h, w, in.ch,
                      static_cast<bf16_t*>(out.base), out.stride, out.off);
-  return hipGetLastError();
-}
-
-hipError_t up_bwd_relu(const float* dy, int dy_stride, int n, int h, int w,
-                       const Slice& y, bf16_t* out, hipStream_t st) {
-  const int ch = y.ch;
-  const long long total8 = (long long)n * h * w * (ch / 8);
-  if (ch % 8 || dy_stride % 8 || y.stride % 8 || y.off % 8 ||
-      total8 >= (1LL << 31))
-    return hipErrorInvalidValue;
-  hipLaunchKernelGGL(up_bwd_relu8_kernel, dim3(blocks_for(total8)),
-                     dim3(kBlock), 0, st, dy, dy_stride, h, w, ch / 8,
-                     (int)total8, static_cast<const bf16_t*>(y.base), y.stride,
-                     y.off, out);
   return hipGetLastError();
 }
 
