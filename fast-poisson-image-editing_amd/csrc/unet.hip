@@ -137,45 +137,47 @@ double conv_flops(const pie_unet* u, bool backward) {
   return f;
 }
 
-int conv_fwd(pie_unet* u, int l, const Slice& x, const Slice& y) {
+// Forward of conv l (bias, ReLU) into the bf16 slice y; with pool / upf,
+// also y's 2 x 2 average pool / 2x nearest upsample.
+int conv_fwd(pie_unet* u, int l, const Slice& x, const Slice& y,
+             const Slice* pool = nullptr, const Slice* upf = nullptr) {
   const ConvSpec& c = kConvs[l];
   const int s = 1 << c.level;
   PIE_HIP(pie::unet::conv3x3(x, u->B, u->H / s, u->W / s, u->wk[l].p, c.cout,
                              u->param.p + u->boff[l], true, &y, nullptr, 0,
-                             false, u->stream));
+                             false, u->stream, nullptr, nullptr, pool, upf));
   return PIE_OK;
 }
 
 int forward(pie_unet* u) {
   hipStream_t st = u->stream;
-  const int B = u->B, H = u->H, W = u->W;
   PIE_HIP(pie::unet::to_bf16(u->x32.p, (int)u->P[0], 8, sl(u->in, 8, 0, 8),
                              st));
   PIE_CHECK(conv_fwd(u, E0A, sl(u->in, 8, 0, 8), sl(u->e0a, 32, 0, 32)));
-  PIE_CHECK(conv_fwd(u, E0B, sl(u->e0a, 32, 0, 32), sl(u->c0, 96, 64, 32)));
-  PIE_HIP(pie::unet::pool_fwd(sl(u->c0, 96, 64, 32), B, H, W,
-                              sl(u->p0, 32, 0, 32), st));
-  PIE_CHECK(conv_fwd(u, E1A, sl(u->p0, 32, 0, 32), sl(u->e1a, 64, 0, 64)));
-  PIE_CHECK(conv_fwd(u, E1B, sl(u->e1a, 64, 0, 64), sl(u->c1, 192, 128, 64)));
-  PIE_HIP(pie::unet::pool_fwd(sl(u->c1, 192, 128, 64), B, H / 2, W / 2,
-                              sl(u->p1, 64, 0, 64), st));
-  PIE_CHECK(conv_fwd(u, E2A, sl(u->p1, 64, 0, 64), sl(u->e2a, 128, 0, 128)));
+  // encoder outputs go to the skip half of the concatenation buffers, with
+  // their average pool for the next level; decoder level outputs to their
+  // own buffers, with their upsample into the finer level's concatenation
+  const Slice p0 = sl(u->p0, 32, 0, 32), p1 = sl(u->p1, 64, 0, 64),
+              p2 = sl(u->p2, 128, 0, 128);
+  PIE_CHECK(conv_fwd(u, E0B, sl(u->e0a, 32, 0, 32), sl(u->c0, 96, 64, 32),
+                     &p0));
+  PIE_CHECK(conv_fwd(u, E1A, p0, sl(u->e1a, 64, 0, 64)));
+  PIE_CHECK(conv_fwd(u, E1B, sl(u->e1a, 64, 0, 64), sl(u->c1, 192, 128, 64),
+                     &p1));
+  PIE_CHECK(conv_fwd(u, E2A, p1, sl(u->e2a, 128, 0, 128)));
   PIE_CHECK(conv_fwd(u, E2B, sl(u->e2a, 128, 0, 128),
-                     sl(u->c2, 384, 256, 128)));
-  PIE_HIP(pie::unet::pool_fwd(sl(u->c2, 384, 256, 128), B, H / 4, W / 4,
-                              sl(u->p2, 128, 0, 128), st));
-  PIE_CHECK(conv_fwd(u, BNA, sl(u->p2, 128, 0, 128), sl(u->bna, 256, 0, 256)));
-  PIE_CHECK(conv_fwd(u, BNB, sl(u->bna, 256, 0, 256), sl(u->bnb, 256, 0, 256)));
-  PIE_HIP(pie::unet::up_fwd(sl(u->bnb, 256, 0, 256), B, H / 8, W / 8,
-                            sl(u->c2, 384, 0, 256), st));
+                     sl(u->c2, 384, 256, 128), &p2));
+  PIE_CHECK(conv_fwd(u, BNA, p2, sl(u->bna, 256, 0, 256)));
+  const Slice u2 = sl(u->c2, 384, 0, 256), u1 = sl(u->c1, 192, 0, 128),
+              u0 = sl(u->c0, 96, 0, 64);
+  PIE_CHECK(conv_fwd(u, BNB, sl(u->bna, 256, 0, 256), sl(u->bnb, 256, 0, 256),
+                     nullptr, &u2));
   PIE_CHECK(conv_fwd(u, D2A, sl(u->c2, 384, 0, 384), sl(u->d2a, 128, 0, 128)));
-  PIE_CHECK(conv_fwd(u, D2B, sl(u->d2a, 128, 0, 128), sl(u->d2b, 128, 0, 128)));
-  PIE_HIP(pie::unet::up_fwd(sl(u->d2b, 128, 0, 128), B, H / 4, W / 4,
-                            sl(u->c1, 192, 0, 128), st));
+  PIE_CHECK(conv_fwd(u, D2B, sl(u->d2a, 128, 0, 128), sl(u->d2b, 128, 0, 128),
+                     nullptr, &u1));
   PIE_CHECK(conv_fwd(u, D1A, sl(u->c1, 192, 0, 192), sl(u->d1a, 64, 0, 64)));
-  PIE_CHECK(conv_fwd(u, D1B, sl(u->d1a, 64, 0, 64), sl(u->d1b, 64, 0, 64)));
-  PIE_HIP(pie::unet::up_fwd(sl(u->d1b, 64, 0, 64), B, H / 2, W / 2,
-                            sl(u->c0, 96, 0, 64), st));
+  PIE_CHECK(conv_fwd(u, D1B, sl(u->d1a, 64, 0, 64), sl(u->d1b, 64, 0, 64),
+                     nullptr, &u0));
   PIE_CHECK(conv_fwd(u, D0A, sl(u->c0, 96, 0, 96), sl(u->d0a, 32, 0, 32)));
   PIE_CHECK(conv_fwd(u, D0B, sl(u->d0a, 32, 0, 32), sl(u->d0b, 32, 0, 32)));
   const long long hw = u->woff[kNumConv];  // head weights

@@ -36,6 +36,10 @@ struct Slice {
 // and written as bf16 [n][h/2][w/2][up->ch] to up->out (the upsampling
 // backward with the backward of the ReLU that made its input); up->ch a
 // multiple of 64, Cin and Cout of 32, h and w even, no bf16 output.
+// With pool_out / upf_out (Cout channels each; needs out_bf): the bf16
+// output's 2 x 2 average pool (the four bf16 values summed in the order
+// (0,0), (0,1), (1,0), (1,1), times 1/4, rounded; h and w even) and its 2x
+// nearest upsample are written there as well.
 struct UpSum {
   bf16_t* out;
   int ch;
@@ -45,7 +49,8 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
                    int cout, const float* bias, bool relu, const Slice* out_bf,
                    float* out_f32, int out_f32_stride, bool accumulate,
                    hipStream_t st, const Slice* mask = nullptr,
-                   const UpSum* up = nullptr);
+                   const UpSum* up = nullptr, const Slice* pool_out = nullptr,
+                   const Slice* upf_out = nullptr);
 
 // dW[tap][ci][co] += sum_p x[p + offset(tap)][ci] * dy[p][co] (fp32
 // atomics) and db[co] += sum_p dy[p][co]; x is a bf16 slice, dy bf16 with
@@ -63,9 +68,6 @@ hipError_t pack_weights(const float* w, int cin, int cout, bf16_t* wk,
 // input fp32 [P][8] -> bf16 slice
 hipError_t to_bf16(const float* x, int p, int ch, const Slice& out,
                    hipStream_t st);
-// 2x2 average pool bf16 -> bf16 (h, w: input size, even)
-hipError_t pool_fwd(const Slice& in, int n, int h, int w, const Slice& out,
-                    hipStream_t st);
 // Backward of a 2x2 average pool feeding a skip connection, with the ReLU
 // of the skip output y folded in: out[n][2i+a][2j+b][c] =
 // bf16(y > 0 ? gs[..][gs_off + c] + dy[n][i][j][c] / 4 : 0), out [P][y.ch]
@@ -73,9 +75,6 @@ hipError_t pool_fwd(const Slice& in, int n, int h, int w, const Slice& out,
 hipError_t pool_bwd_relu(const float* dy, int n, int h2, int w2,
                          const float* gs, int gs_stride, int gs_off,
                          const Slice& y, bf16_t* out, hipStream_t st);
-// 2x nearest upsample bf16 -> bf16 slice (h, w: input size)
-hipError_t up_fwd(const Slice& in, int n, int h, int w, const Slice& out,
-                  hipStream_t st);
 // head: out[p][0..3) = sum_c x[p][c] * bf16(w[c][o]) + b[o]  (x bf16 32 ch)
 hipError_t head_fwd(const Slice& x, int p, const float* w, const float* b,
                     float* out, hipStream_t st);

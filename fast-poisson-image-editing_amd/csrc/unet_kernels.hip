@@ -83,6 +83,12 @@ struct ConvArgs {
   int up_ch;
   const bf16_t* up_mask;
   int up_mask_stride, up_mask_off;
+  // forward folds (LDS-tiled kernel, with out_bf): the bf16 output's 2 x 2
+  // average pool into pool_out, its 2x nearest upsample into upf_out
+  bf16_t* pool_out;
+  int pool_stride, pool_off;
+  bf16_t* upf_out;
+  int upf_stride, upf_off;
 };
 
 // Wave tile: MT x 16 pixels by NT x 16 output channels; a block is four
@@ -352,6 +358,48 @@ __global__ void __launch_bounds__(kBlock) conv3x3_lds_kernel(ConvArgs a) {
         o[3] = (uint32_t)f2bf(v1[2]) | ((uint32_t)f2bf(v1[3]) << 16);
         *reinterpret_cast<u32x4*>(a.out_bf + p * a.out_stride + a.out_off +
                                   co0 + c8 * 8) = o;
+        if (a.upf_out) {
+          const int W2 = 2 * a.w;
+          const long long f =
+              ((long long)img * 2 * a.h + 2 * y) * W2 + 2 * x;
+          bf16_t* q = a.upf_out + a.upf_off + co0 + c8 * 8;
+          *reinterpret_cast<u32x4*>(q + f * a.upf_stride) = o;
+          *reinterpret_cast<u32x4*>(q + (f + 1) * a.upf_stride) = o;
+          *reinterpret_cast<u32x4*>(q + (f + W2) * a.upf_stride) = o;
+          *reinterpret_cast<u32x4*>(q + (f + W2 + 1) * a.upf_stride) = o;
+        }
+      }
+    }
+  }
+  if (a.pool_out) {
+    // average of the four bf16 outputs of each 2 x 2 block, in the order
+    // (0, 0), (0, 1), (1, 0), (1, 1)
+    constexpr int kJobs = (TH / 2) * (TW / 2) * (CT / 8);
+    const int h2 = a.h / 2, w2 = a.w / 2;
+#pragma unroll
+    for (int k = 0; k < (kJobs + kBlock - 1) / kBlock; ++k) {
+      const int job = threadIdx.x + k * kBlock;
+      const int cp = job / (CT / 8), c8 = job % (CT / 8);
+      const int cy = cp / (TW / 2), cx = cp % (TW / 2);
+      const int y2 = row0 / 2 + cy, x2 = col0 / 2 + cx;
+      if (job < kJobs && y2 < h2 && x2 < w2) {
+        const float* t0 = &T[((2 * cy) * TW + 2 * cx) * TS + c8 * 8];
+        const float* tk[4] = {t0, t0 + TS, t0 + TW * TS, t0 + TW * TS + TS};
+        float sum[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) sum[e] = bf2f(f2bf(tk[0][e]));
+#pragma unroll
+        for (int k2 = 1; k2 < 4; ++k2)
+#pragma unroll
+          for (int e = 0; e < 8; ++e) sum[e] += bf2f(f2bf(tk[k2][e]));
+        u32x4 o;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          o[j] = (uint32_t)f2bf(sum[2 * j] * 0.25f) |
+                 ((uint32_t)f2bf(sum[2 * j + 1] * 0.25f) << 16);
+        const long long q = ((long long)img * h2 + y2) * w2 + x2;
+        *reinterpret_cast<u32x4*>(a.pool_out + q * a.pool_stride +
+                                  a.pool_off + co0 + c8 * 8) = o;
       }
     }
   }
@@ -750,102 +798,6 @@ __global__ void __launch_bounds__(kBlock)
     out[(id / ch) * stride + off + id % ch] = f2bf(x[id]);
 }
 
-__global__ void __launch_bounds__(kBlock)
-    pool_fwd_kernel(const bf16_t* __restrict__ in, int in_stride, int in_off,
-                    int n, int h, int w, int ch, bf16_t* __restrict__ out,
-                    int out_stride, int out_off) {
-  const int h2 = h / 2, w2 = w / 2;
-  const long long total = (long long)n * h2 * w2 * ch;
-  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < total;
-       id += (long long)gridDim.x * kBlock) {
-    const int c = (int)(id % ch);
-    const long long q = id / ch;
-    const int x2 = (int)(q % w2), y2 = (int)((q / w2) % h2);
-    const long long img = q / ((long long)w2 * h2);
-    const long long b = (img * h + 2 * y2) * w + 2 * x2;
-    const float s = bf2f(in[b * in_stride + in_off + c]) +
-                    bf2f(in[(b + 1) * in_stride + in_off + c]) +
-                    bf2f(in[(b + w) * in_stride + in_off + c]) +
-                    bf2f(in[(b + w + 1) * in_stride + in_off + c]);
-    out[q * out_stride + out_off + c] = f2bf(s * 0.25f);
-  }
-}
-
-__global__ void __launch_bounds__(kBlock)
-    up_fwd_kernel(const bf16_t* __restrict__ in, int in_stride, int in_off,
-                  int n, int h, int w, int ch, bf16_t* __restrict__ out,
-                  int out_stride, int out_off) {
-  const int H = 2 * h, W = 2 * w;
-  const long long total = (long long)n * H * W * ch;
-  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < total;
-       id += (long long)gridDim.x * kBlock) {
-    const int c = (int)(id % ch);
-    const long long q = id / ch;
-    const int x = (int)(q % W), y = (int)((q / W) % H);
-    const long long img = q / ((long long)W * H);
-    const long long s = (img * h + y / 2) * w + x / 2;
-    out[q * out_stride + out_off + c] = in[s * in_stride + in_off + c];
-  }
-}
-
-// up_fwd_kernel with eight channels (16 bytes) per thread: channel counts,
-// strides and offsets are multiples of 8.
-__global__ void __launch_bounds__(kBlock)
-    up_fwd8_kernel(const bf16_t* __restrict__ in, int in_stride, int in_off,
-                   int n, int h, int w, int ch, bf16_t* __restrict__ out,
-                   int out_stride, int out_off) {
-  const int H = 2 * h, W = 2 * w, cg = ch / 8;
-  const long long total = (long long)n * H * W * cg;
-  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < total;
-       id += (long long)gridDim.x * kBlock) {
-    const int c = (int)(id % cg) * 8;
-    const long long q = id / cg;
-    const int x = (int)(q % W), y = (int)((q / W) % H);
-    const long long img = q / ((long long)W * H);
-    const long long sp = (img * h + y / 2) * w + x / 2;
-    *reinterpret_cast<uint4*>(out + q * out_stride + out_off + c) =
-        *reinterpret_cast<const uint4*>(in + sp * in_stride + in_off + c);
-  }
-}
-
-// Vector forms of the pooling and upsampling kernels: 8 bf16 or 4 fp32
-// channels per thread with 16-byte accesses and 32-bit index arithmetic
-// (element counts below 2^31; same sums in the same order as above).
-__global__ void __launch_bounds__(kBlock)
-    pool_fwd8_kernel(const bf16_t* __restrict__ in, int in_stride, int in_off,
-                     int h, int w, int cg, int total,
-                     bf16_t* __restrict__ out, int out_stride, int out_off) {
-  const int h2 = h / 2, w2 = w / 2;
-  for (int id = blockIdx.x * kBlock + threadIdx.x; id < total;
-       id += gridDim.x * kBlock) {
-    const int c = (id % cg) * 8, q = id / cg;
-    const int x2 = q % w2, t = q / w2;
-    const int y2 = t % h2, img = t / h2;
-    const long long b = ((long long)img * h + 2 * y2) * w + 2 * x2;
-    const u32x4 v0 = *reinterpret_cast<const u32x4*>(in + b * in_stride + in_off + c);
-    const u32x4 v1 =
-        *reinterpret_cast<const u32x4*>(in + (b + 1) * in_stride + in_off + c);
-    const u32x4 v2 =
-        *reinterpret_cast<const u32x4*>(in + (b + w) * in_stride + in_off + c);
-    const u32x4 v3 = *reinterpret_cast<const u32x4*>(
-        in + (b + w + 1) * in_stride + in_off + c);
-    u32x4 o;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const float lo = __uint_as_float(v0[j] << 16) +
-                       __uint_as_float(v1[j] << 16) +
-                       __uint_as_float(v2[j] << 16) +
-                       __uint_as_float(v3[j] << 16);
-      const float hi = __uint_as_float(v0[j] & 0xffff0000u) +
-                       __uint_as_float(v1[j] & 0xffff0000u) +
-                       __uint_as_float(v2[j] & 0xffff0000u) +
-                       __uint_as_float(v3[j] & 0xffff0000u);
-      o[j] = (uint32_t)f2bf(lo * 0.25f) | ((uint32_t)f2bf(hi * 0.25f) << 16);
-    }
-    *reinterpret_cast<u32x4*>(out + (long long)q * out_stride + out_off + c) = o;
-  }
-}
-
 // Average-pooling backward into a skip connection: for the four pixels p of
 // source pixel s, g = gs[p] + dy[s] / 4 (gs: the skip channels' gradient
 // from the decoder), masked by the skip output y, into out [P][ch].
@@ -1035,7 +987,8 @@ int blocks_for(long long n, int cap = 8192) {
 hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
                    int cout, const float* bias, bool relu, const Slice* out_bf,
                    float* out_f32, int out_f32_stride, bool accumulate,
-                   hipStream_t st, const Slice* mask, const UpSum* up) {
+                   hipStream_t st, const Slice* mask, const UpSum* up,
+                   const Slice* pool_out, const Slice* upf_out) {
   if (in.ch % 8 || cout % 16 || in.stride % 8 || in.off % 8)
     return hipErrorInvalidValue;
   if (mask && (!out_bf || mask->ch != cout || mask->stride % 8 ||
@@ -1047,6 +1000,10 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
              cout % 32 || (up->ch < cout && !out_f32) || out_bf ||
              accumulate))
     return hipErrorInvalidValue;
+  for (const Slice* f : {pool_out, upf_out})
+    if (f && (!out_bf || f->ch != cout || f->stride % 8 || f->off % 8 ||
+              in.ch % 32 || cout % 32 || (f == pool_out && (h % 2 || w % 2))))
+      return hipErrorInvalidValue;
   ConvArgs a{};
   a.in = static_cast<const bf16_t*>(in.base);
   a.in_stride = in.stride;
@@ -1073,6 +1030,12 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
   a.up_mask = up ? static_cast<const bf16_t*>(up->mask.base) : nullptr;
   a.up_mask_stride = up ? up->mask.stride : 0;
   a.up_mask_off = up ? up->mask.off : 0;
+  a.pool_out = pool_out ? static_cast<bf16_t*>(pool_out->base) : nullptr;
+  a.pool_stride = pool_out ? pool_out->stride : 0;
+  a.pool_off = pool_out ? pool_out->off : 0;
+  a.upf_out = upf_out ? static_cast<bf16_t*>(upf_out->base) : nullptr;
+  a.upf_stride = upf_out ? upf_out->stride : 0;
+  a.upf_off = upf_out ? upf_out->off : 0;
   const long long P = (long long)n * h * w;
   const bool vec_out = (!out_bf || (out_bf->stride % 8 == 0 &&
                                     out_bf->off % 8 == 0)) &&
@@ -1095,7 +1058,8 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
                          dim3(kBlock), 0, st, a);
     return hipGetLastError();
   }
-  if (mask || up) return hipErrorInvalidValue;  // unaligned output slice
+  if (mask || up || pool_out || upf_out)
+    return hipErrorInvalidValue;  // unaligned output slice
   constexpr int MT = 2;
   if (cout % 32 == 0) {
     constexpr int NT = 2;
@@ -1160,25 +1124,6 @@ hipError_t to_bf16(const float* x, int p, int ch, const Slice& out,
   return hipGetLastError();
 }
 
-hipError_t pool_fwd(const Slice& in, int n, int h, int w, const Slice& out,
-                    hipStream_t st) {
-  const long long total8 = (long long)n * (h / 2) * (w / 2) * (in.ch / 8);
-  if (in.ch % 8 == 0 && in.stride % 8 == 0 && in.off % 8 == 0 &&
-      out.stride % 8 == 0 && out.off % 8 == 0 && total8 < (1LL << 31)) {
-    hipLaunchKernelGGL(pool_fwd8_kernel, dim3(blocks_for(total8)), dim3(kBlock),
-                       0, st, static_cast<const bf16_t*>(in.base), in.stride,
-                       in.off, h, w, in.ch / 8, (int)total8,
-                       static_cast<bf16_t*>(out.base), out.stride, out.off);
-    return hipGetLastError();
-  }
-  hipLaunchKernelGGL(pool_fwd_kernel,
-                     dim3(blocks_for((long long)n * h * w / 4 * in.ch)),
-                     dim3(kBlock), 0, st, static_cast<const bf16_t*>(in.base),
-                     in.stride, in.off, n, h, w, in.ch,
-                     static_cast<bf16_t*>(out.base), out.stride, out.off);
-  return hipGetLastError();
-}
-
 hipError_t pool_bwd_relu(const float* dy, int n, int h2, int w2,
                          const float* gs, int gs_stride, int gs_off,
                          const Slice& y, bf16_t* out, hipStream_t st) {
@@ -1191,26 +1136,6 @@ hipError_t pool_bwd_relu(const float* dy, int n, int h2, int w2,
                      dim3(kBlock), 0, st, dy, h2, w2, ch / 8, (int)total8, gs,
                      gs_stride, gs_off, static_cast<const bf16_t*>(y.base),
                      y.stride, y.off, out);
-  return hipGetLastError();
-}
-
-hipError_t up_fwd(const Slice& in, int n, int h, int w, const Slice& out,
-                  hipStream_t st) {
-  if (in.ch % 8 == 0 && in.stride % 8 == 0 && in.off % 8 == 0 &&
-      out.stride % 8 == 0 && out.off % 8 == 0) {
-    hipLaunchKernelGGL(up_fwd8_kernel,
-                       dim3(blocks_for((long long)n * h * w * 4 * in.ch / 8)),
-                       dim3(kBlock), 0, st,
-                       static_cast<const bf16_t*>(in.base), in.stride, in.off,
-                       n, h, w, in.ch, static_cast<bf16_t*>(out.base),
-                       out.stride, out.off);
-    return hipGetLastError();
-  }
-  hipLaunchKernelGGL(up_fwd_kernel,
-                     dim3(blocks_for((long long)n * h * w * 4 * in.ch)),
-                     dim3(kBlock), 0, st, static_cast<const bf16_t*>(in.base),
-                     in.stride, in.off, n, h, w, in.ch,
-                     static_cast<bf16_t*>(out.base), out.stride, out.off);
   return hipGetLastError();
 }
 
