@@ -42,6 +42,22 @@ __device__ __forceinline__ bf16x8 zero8() {
   return __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0));
 }
 
+// Blocks that read the same data (the output-channel blocks of one pixel
+// tile, the channel blocks of one tile range) are numbered so that they run
+// on one XCD, one after another, and share its L2: the dispatcher hands
+// block L to XCD L % 8, so group g's nm members are blocks
+// 8 (nm (g / 8) + m) + g % 8, m = 0 .. nm - 1.  The grid holds
+// ceil(groups / 8) * 8 * nm blocks; groups past the last are empty.
+__device__ __forceinline__ void xcd_group(int L, int nm, int& grp, int& mem) {
+  const int k = L >> 3;
+  mem = k % nm;
+  grp = (k / nm) * 8 + (L & 7);
+}
+
+inline unsigned xcd_grid(long long groups, int nm) {
+  return (unsigned)(((groups + 7) / 8) * 8 * nm);
+}
+
 // The backward of the ReLU on a level's output folded into the producer of
 // its gradient: out[p][c] = bf16(y[p][c] > 0 ? g : 0), with g the fp32
 // gradient the producer sums (eight channels per thread; channel counts,
@@ -266,11 +282,13 @@ __global__ void __launch_bounds__(kBlock) conv3x3_lds_kernel(ConvArgs a) {
   const int wave = threadIdx.x / kWave;
   const int g = lane >> 4, i = lane & 15;
   const int tiles_x = (a.w + TW - 1) / TW, tiles_y = (a.h + TH - 1) / TH;
-  const int tile = blockIdx.x;
+  int tile, cob;
+  xcd_group(blockIdx.x, a.cout / CT, tile, cob);
+  if (tile >= a.n * tiles_x * tiles_y) return;  // whole block, before any barrier
   const int img = tile / (tiles_x * tiles_y);
   const int trem = tile % (tiles_x * tiles_y);
   const int row0 = (trem / tiles_x) * TH, col0 = (trem % tiles_x) * TW;
-  const int co0 = blockIdx.y * CT;
+  const int co0 = cob * CT;
   const long long img_base = (long long)img * a.h * a.w;
   u32x4 wv[kWinIt], wt[kWtIt];
   f32x4 acc[MT][NT];
@@ -579,7 +597,7 @@ __global__ void __launch_bounds__(kBlock)
 // Weight gradient over spatial tiles (input channels a multiple of 8, output
 // channels of 32): a block owns 32 input x CO (32 or 64) output channels
 // (input channels past cin read as zero) for all nine taps and sums over
-// 8 x 16-pixel tiles of the images.  Per tile it stages the input window
+// kWtH x 16-pixel tiles of the images.  Per tile it stages the input window
 // (tile plus one-pixel halo) and the output gradient in LDS in their
 // natural [pixel][channel] layout, straight from 16-byte global loads, and
 // reads the MFMA operands (K = pixels) with gfx950's transposing LDS read:
@@ -592,8 +610,7 @@ __global__ void __launch_bounds__(kBlock)
 // (CO / 32) tb.  The staging loads are unconditional (addresses clamped
 // into the image, values outside selected to zero) and the next tile's
 // loads are in flight while the MFMAs of the current one run.
-constexpr int kWtH = 8, kWtW = 16;
-constexpr int kWtWH = kWtH + 2, kWtWW = kWtW + 2;  // input window
+constexpr int kWtW = 16;  // tile width; the height is a template argument
 
 typedef short i16x4 __attribute__((ext_vector_type(4)));
 typedef short i16x8 __attribute__((ext_vector_type(8)));
@@ -609,13 +626,14 @@ __device__ __forceinline__ bf16x8 tr_pair(const bf16_t* lo, const bf16_t* hi) {
   return __builtin_bit_cast(bf16x8, c);
 }
 
-template <int CO>
+template <int CO, int kWtH>
 __global__ void __launch_bounds__(kBlock)
     wgrad_lds_kernel(const bf16_t* __restrict__ x, int x_stride, int x_off,
                      int cin, int n, int h, int w,
                      const bf16_t* __restrict__ dy, int cout,
                      float* __restrict__ dw, float* __restrict__ db,
                      int tiles_per_block) {
+  constexpr int kWtWH = kWtH + 2, kWtWW = kWtW + 2;  // input window
   constexpr int NBW = CO / 32;                      // output blocks per wave
   constexpr int DG = CO / 8;                        // gradient channel groups
   constexpr int XJ = kWtWH * kWtWW * 4;             // window loads (16 B)
@@ -632,9 +650,13 @@ __global__ void __launch_bounds__(kBlock)
   const int pj = i >> 2, pq = i & 3;  // transposing read: pixel, 4-channel chunk
   const int ta = wave >> 1, tb = wave & 1;
   const int nci = (cin + 31) / 32;
-  const int ci0 = (blockIdx.y % nci) * 32, co0 = (blockIdx.y / nci) * CO;
+  int grp, mem;
+  xcd_group(blockIdx.x, nci * (cout / CO), grp, mem);
+  const int ci0 = (mem % nci) * 32, co0 = (mem / nci) * CO;
   const int tiles_x = (w + kWtW - 1) / kWtW, tiles_y = (h + kWtH - 1) / kWtH;
   const long long ntiles = (long long)n * tiles_x * tiles_y;
+  const long long t_begin = (long long)grp * tiles_per_block;
+  if (t_begin >= ntiles) return;  // whole block, before any barrier
   const bool bias = db && ci0 == 0;
   const int tid = threadIdx.x;
   const int dcg = tid % DG;  // the gradient channel group of all the
@@ -707,9 +729,8 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int t = 0; t < 9; ++t) acc[q][t] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  const long long t_begin = (long long)blockIdx.x * tiles_per_block;
   const long long t_end = min(ntiles, t_begin + tiles_per_block);
-  if (t_begin < t_end) load(t_begin);
+  load(t_begin);
   for (long long t = t_begin; t < t_end; ++t) {
     store();
     __syncthreads();
@@ -1048,14 +1069,17 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
     if (cout % 64 == 0 && in.ch >= 128) {
       // deep inputs: 16 x 16-pixel tiles, twice the MFMAs per staged weight
       const unsigned tiles4 = (unsigned)n * ((h + 15) / 16) * ((w + 15) / 16);
-      hipLaunchKernelGGL((conv3x3_lds_kernel<4, 4>), dim3(tiles4, cout / 64),
-                         dim3(kBlock), 0, st, a);
+      hipLaunchKernelGGL((conv3x3_lds_kernel<4, 4>),
+                         dim3(xcd_grid(tiles4, cout / 64)), dim3(kBlock), 0,
+                         st, a);
     } else if (cout % 64 == 0)
-      hipLaunchKernelGGL((conv3x3_lds_kernel<LMT, 4>), dim3(tiles, cout / 64),
-                         dim3(kBlock), 0, st, a);
+      hipLaunchKernelGGL((conv3x3_lds_kernel<LMT, 4>),
+                         dim3(xcd_grid(tiles, cout / 64)), dim3(kBlock), 0, st,
+                         a);
     else
-      hipLaunchKernelGGL((conv3x3_lds_kernel<LMT, 2>), dim3(tiles, cout / 32),
-                         dim3(kBlock), 0, st, a);
+      hipLaunchKernelGGL((conv3x3_lds_kernel<LMT, 2>),
+                         dim3(xcd_grid(tiles, cout / 32)), dim3(kBlock), 0, st,
+                         a);
     return hipGetLastError();
   }
   if (mask || up || pool_out || upf_out)
@@ -1078,6 +1102,7 @@ hipError_t conv3x3_wgrad(const Slice& x, int n, int h, int w,
   if (x.ch % 8 || cout % 8 || x.stride % 8 || x.off % 8)
     return hipErrorInvalidValue;
   if (cout % 32 == 0) {
+    constexpr int kWtH = 16;  // 16 x 16 tiles: 1.27x window halo, not 1.41x
     const long long ntiles = (long long)n * ((h + kWtH - 1) / kWtH) *
                              ((w + kWtW - 1) / kWtW);
     const int co = cout % 64 == 0 ? 64 : 32;
@@ -1086,14 +1111,15 @@ hipError_t conv3x3_wgrad(const Slice& x, int n, int h, int w,
     long long per = (ntiles * tiles + 1023) / 1024;
     if (per < 1) per = 1;
     const int xb = (int)((ntiles + per - 1) / per);
+    const dim3 grid(xcd_grid(xb, tiles));
     if (co == 64)
-      hipLaunchKernelGGL(wgrad_lds_kernel<64>, dim3(xb, tiles), dim3(kBlock),
-                         0, st, static_cast<const bf16_t*>(x.base), x.stride,
-                         x.off, x.ch, n, h, w, dy, cout, dw, db, (int)per);
+      hipLaunchKernelGGL((wgrad_lds_kernel<64, kWtH>), grid, dim3(kBlock), 0, st,
+                         static_cast<const bf16_t*>(x.base), x.stride, x.off,
+                         x.ch, n, h, w, dy, cout, dw, db, (int)per);
     else
-      hipLaunchKernelGGL(wgrad_lds_kernel<32>, dim3(xb, tiles), dim3(kBlock),
-                         0, st, static_cast<const bf16_t*>(x.base), x.stride,
-                         x.off, x.ch, n, h, w, dy, cout, dw, db, (int)per);
+      hipLaunchKernelGGL((wgrad_lds_kernel<32, kWtH>), grid, dim3(kBlock), 0, st,
+                         static_cast<const bf16_t*>(x.base), x.stride, x.off,
+                         x.ch, n, h, w, dy, cout, dw, db, (int)per);
     return hipGetLastError();
   }
   const long long P = (long long)n * h * w;
