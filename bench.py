@@ -12,7 +12,9 @@ Printed: one JSON line with the metric (solved-pixel updates per second),
 the roofline of the sweep kernel (algorithmic bytes per launch over its
 event-timed average launch duration, against the 8 TB/s HBM peak), the
 time to a mean residual of 1e-3 on an unseen 1024 x 1024 composite with and
-without the learned warm start (``time_to_residual``) and the
+without the learned warm start (``time_to_residual``), the MFMA
+utilisation of the warm start's U-Net in a training step (``unet_mfma``:
+batch 64 x 256 x 256, bf16, device time) and the
 reference's CPU path timed on this host (its openmp GridSolver compiled from
 the reference sources, or the oracle's restatement when that is absent) on a
 bounded sample of the same workload.
@@ -232,6 +234,8 @@ def main():
   ap.add_argument("--no-cpu-baseline", action="store_true")
   ap.add_argument("--no-warm-start", action="store_true",
                   help="skip the time-to-residual comparison")
+  ap.add_argument("--no-unet", action="store_true",
+                  help="skip the U-Net training-step MFMA measurement")
   a = ap.parse_args()
   if a.gpus != 1:
     print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -323,6 +327,11 @@ def main():
   if not a.no_warm_start and w.method == "grid":
     del core
     line["time_to_residual"] = bench_warm_start()
+  if not a.no_unet:
+    # the warm start's network: one Adam step of UNet(64, 256, 256) in bf16
+    # on the MFMA units (dense bf16 peak 2.5 PFLOP/s)
+    from fpie_amd import unet
+    line["unet_mfma"] = unet.measure(64, 256, 5)
   if not a.no_cpu_baseline:
     line["cpu_baseline"] = cpu_baseline(w, a.cpu_budget)
   print(json.dumps(line), flush=True)

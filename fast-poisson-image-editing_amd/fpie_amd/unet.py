@@ -206,6 +206,38 @@ def train(M, X, Y, epochs: int = 4, batch: int = 64, lr: float = 1e-3,
                "steps": len(losses), **st}
 
 
+def measure(batch: int = 64, size: int = 256, steps: int = 5) -> dict:
+  """Time Adam steps of UNet(batch, size, size) on random inputs (one
+  untimed step first).  device_ms_per_step: forward + backward between HIP
+  events on the network's stream, the batch already uploaded; ms_per_step:
+  whole train_step calls including the host-to-device upload of the batch,
+  Adam and the weight packing.  conv_tflops counts 2 H W 9 Cin Cout per
+  convolution and pass."""
+  rng = np.random.default_rng(0)
+  b, s = batch, size
+  x = np.zeros((b, s, s, 8), np.float32)
+  x[..., :3] = rng.normal(0, 1, (b, s, s, 3))
+  x[..., 6] = 1.0
+  y = rng.normal(0, 1, (b, s, s, 3)).astype(np.float32)
+  m = np.ones((b, s, s), np.uint8)
+  net = UNet(b, s, s, seed=0)
+  net.train_step(x, y, m, 1e-3)  # untimed: kernel loading
+  before = net.stats()
+  t0 = time.perf_counter()
+  for _ in range(steps):
+    net.train_step(x, y, m, 1e-3)
+  dt = time.perf_counter() - t0
+  after = net.stats()
+  flops = after["conv_flops"] - before["conv_flops"]
+  dev = (after["ms"] - before["ms"]) / 1e3
+  return {"batch": b, "size": s, "steps": steps,
+          "ms_per_step": dt * 1e3 / steps,
+          "device_ms_per_step": dev * 1e3 / steps,
+          "conv_tflops": flops / dev / 1e12,
+          "conv_tflops_wall": flops / dt / 1e12,
+          "mfma_utilisation_vs_2500_tflops": flops / dev / 2.5e15}
+
+
 def main(argv=None) -> None:
   """Time Adam steps of the network on random inputs:
   python -m fpie_amd.unet [--batch 64] [--size 256] [--steps 5]"""
@@ -216,33 +248,7 @@ def main(argv=None) -> None:
   ap.add_argument("--size", type=int, default=256)
   ap.add_argument("--steps", type=int, default=5)
   a = ap.parse_args(argv)
-  rng = np.random.default_rng(0)
-  b, s = a.batch, a.size
-  x = np.zeros((b, s, s, 8), np.float32)
-  x[..., :3] = rng.normal(0, 1, (b, s, s, 3))
-  x[..., 6] = 1.0
-  y = rng.normal(0, 1, (b, s, s, 3)).astype(np.float32)
-  m = np.ones((b, s, s), np.uint8)
-  net = UNet(b, s, s, seed=0)
-  net.train_step(x, y, m, 1e-3)  # untimed: kernel loading
-  before = net.stats()
-  t0 = time.perf_counter()
-  for _ in range(a.steps):
-    net.train_step(x, y, m, 1e-3)
-  dt = time.perf_counter() - t0
-  after = net.stats()
-  flops = after["conv_flops"] - before["conv_flops"]
-  # device: forward + backward between HIP events on the network's stream
-  # (the batch already uploaded); wall: whole train_step calls including
-  # the host-to-device upload of the batch, Adam and the weight packing
-  dev = (after["ms"] - before["ms"]) / 1e3
-  print(json.dumps({"batch": b, "size": s, "steps": a.steps,
-                    "ms_per_step": dt * 1e3 / a.steps,
-                    "device_ms_per_step": dev * 1e3 / a.steps,
-                    "conv_tflops": flops / dev / 1e12,
-                    "conv_tflops_wall": flops / dt / 1e12,
-                    "mfma_utilisation_vs_2500_tflops": flops / dev / 2.5e15}),
-        flush=True)
+  print(json.dumps(measure(a.batch, a.size, a.steps)), flush=True)
 
 
 if __name__ == "__main__":
