@@ -69,7 +69,7 @@ struct pie_unet {
   // fp32 host-facing buffers and gradients
   DevBuf<float> x32, out, dout, target;
   DevBuf<uint8_t> mask;
-  DevBuf<float> gC0, gC1, gC2, gP;
+  DevBuf<float> gC0, gC1, gC2;
   DevBuf<bf16_t> dyb, dyb2;
   DevBuf<double> acc;
   double ms_total = 0.0, flops_total = 0.0;
@@ -111,7 +111,6 @@ int alloc_all(pie_unet* u) {
   PIE_HIP(u->gC0.alloc(P0 * 96));
   PIE_HIP(u->gC1.alloc(P1 * 192));
   PIE_HIP(u->gC2.alloc(P2 * 384));
-  PIE_HIP(u->gP.alloc(std::max({P1 * 32, P2 * 64, P3 * 128})));
   PIE_HIP(u->dyb.alloc(ga));
   PIE_HIP(u->dyb2.alloc(ga));
   PIE_HIP(u->acc.alloc(2));
@@ -143,9 +142,12 @@ int conv_fwd(pie_unet* u, int l, const Slice& x, const Slice& y,
              const Slice* pool = nullptr, const Slice* upf = nullptr) {
   const ConvSpec& c = kConvs[l];
   const int s = 1 << c.level;
+  pie::unet::ConvFolds f;
+  f.pool = pool;
+  f.upf = upf;
   PIE_HIP(pie::unet::conv3x3(x, u->B, u->H / s, u->W / s, u->wk[l].p, c.cout,
                              u->param.p + u->boff[l], true, &y, nullptr, 0,
-                             false, u->stream, nullptr, nullptr, pool, upf));
+                             false, u->stream, f));
   return PIE_OK;
 }
 
@@ -193,10 +195,13 @@ int forward(pie_unet* u) {
 // convolution's epilogue as la's output gradient, masked by xb > 0 and
 // rounded to bf16 once from the fp32 sum.  la's data gradient (fp32) goes to
 // dx unless dx is null; with up, its first up->ch channels (an upsampled
-// input) go through the upsampling backward into up->out instead.
+// input) go through the upsampling backward into up->out instead; with pb
+// (an average-pooled input), all of it goes through the pooling backward
+// into pb->out instead of dx.
 int conv_pair_bwd(pie_unet* u, int la, int lb, const Slice& xb,
                   const Slice& xa, float* dx,
-                  const pie::unet::UpSum* up = nullptr) {
+                  const pie::unet::UpSum* up = nullptr,
+                  const pie::unet::PoolBwd* pb = nullptr) {
   const ConvSpec& ca = kConvs[la];
   const ConvSpec& cb = kConvs[lb];
   const int s = 1 << cb.level;
@@ -207,21 +212,26 @@ int conv_pair_bwd(pie_unet* u, int la, int lb, const Slice& xb,
                                    u->grad.p + u->boff[lb], st));
   Slice dys{u->dyb.p, cb.cout, 0, cb.cout};
   Slice dya{u->dyb2.p, cb.cin, 0, cb.cin};
+  pie::unet::ConvFolds fb;
+  fb.mask = &xb;
   PIE_HIP(pie::unet::conv3x3(dys, u->B, h, w, u->wkt[lb].p, cb.cin, nullptr,
-                             false, &dya, nullptr, 0, false, st, &xb));
+                             false, &dya, nullptr, 0, false, st, fb));
   PIE_HIP(pie::unet::conv3x3_wgrad(xa, u->B, h, w, u->dyb2.p, ca.cout,
                                    u->grad.p + u->woff[la],
                                    u->grad.p + u->boff[la], st));
-  if (dx)
+  if (dx || pb) {
+    pie::unet::ConvFolds fa;
+    fa.up = up;
+    fa.pool_bwd = pb;
     PIE_HIP(pie::unet::conv3x3(dya, u->B, h, w, u->wkt[la].p, ca.cin, nullptr,
-                               false, nullptr, dx, ca.cin, false, st, nullptr,
-                               up));
+                               false, nullptr, pb ? nullptr : dx, ca.cin,
+                               false, st, fa));
+  }
   return PIE_OK;
 }
 
 int backward(pie_unet* u) {
   hipStream_t st = u->stream;
-  const int B = u->B, H = u->H, W = u->W;
   const long long hw = u->woff[kNumConv];
   bf16_t* dy = u->dyb.p;
   PIE_HIP(hipMemsetAsync(u->grad.p, 0, u->nparam * sizeof(float), st));
@@ -241,20 +251,21 @@ int backward(pie_unet* u) {
   const pie::unet::UpSum up2{dy, 256, sl(u->bnb, 256, 0, 256)};
   PIE_CHECK(conv_pair_bwd(u, D2A, D2B, sl(u->d2a, 128, 0, 128),
                           sl(u->c2, 384, 0, 384), u->gC2.p, &up2));
+  // encoder: the first convolutions take an average-pooled input: their
+  // data gradient's epilogue adds it (a quarter to each of the four pixels)
+  // to the skip channels' gradient from the decoder and applies the ReLU of
+  // the finer level's output
+  const pie::unet::PoolBwd pb2{dy, u->gC2.p, 384, 256,
+                               sl(u->c2, 384, 256, 128)};
   PIE_CHECK(conv_pair_bwd(u, BNA, BNB, sl(u->bna, 256, 0, 256),
-                          sl(u->p2, 128, 0, 128), u->gP.p));
-  // encoder: the skip channels' gradient from the decoder plus the pooling
-  // backward, with the ReLU of the encoder level's output folded in
-  PIE_HIP(pie::unet::pool_bwd_relu(u->gP.p, B, H / 8, W / 8, u->gC2.p, 384,
-                                   256, sl(u->c2, 384, 256, 128), dy, st));
+                          sl(u->p2, 128, 0, 128), nullptr, nullptr, &pb2));
+  const pie::unet::PoolBwd pb1{dy, u->gC1.p, 192, 128,
+                               sl(u->c1, 192, 128, 64)};
   PIE_CHECK(conv_pair_bwd(u, E2A, E2B, sl(u->e2a, 128, 0, 128),
-                          sl(u->p1, 64, 0, 64), u->gP.p));
-  PIE_HIP(pie::unet::pool_bwd_relu(u->gP.p, B, H / 4, W / 4, u->gC1.p, 192,
-                                   128, sl(u->c1, 192, 128, 64), dy, st));
+                          sl(u->p1, 64, 0, 64), nullptr, nullptr, &pb1));
+  const pie::unet::PoolBwd pb0{dy, u->gC0.p, 96, 64, sl(u->c0, 96, 64, 32)};
   PIE_CHECK(conv_pair_bwd(u, E1A, E1B, sl(u->e1a, 64, 0, 64),
-                          sl(u->p0, 32, 0, 32), u->gP.p));
-  PIE_HIP(pie::unet::pool_bwd_relu(u->gP.p, B, H / 2, W / 2, u->gC0.p, 96, 64,
-                                   sl(u->c0, 96, 64, 32), dy, st));
+                          sl(u->p0, 32, 0, 32), nullptr, nullptr, &pb0));
   PIE_CHECK(conv_pair_bwd(u, E0A, E0B, sl(u->e0a, 32, 0, 32),
                           sl(u->in, 8, 0, 8), nullptr));
   return PIE_OK;

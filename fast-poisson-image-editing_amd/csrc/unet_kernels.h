@@ -23,34 +23,54 @@ struct Slice {
 // v_mfma_f32_16x16x32_bf16: out[p][co] = sum_k A[p][k] B[k][co] with
 // k = 8 * kb + e, kb = tap * (Cin / 8) + cb, A = in[p + offset(tap)][8cb+e].
 // wk holds B as [kb][Cout][8] bf16.  Output: bf16 (+bias, optional ReLU)
-// into out_bf, and/or fp32 into out_f32 (optionally accumulated).  With a
-// mask slice (Cout bf16 channels), the bf16 output is masked by it: element
-// (p, co) is written as mask[p][co] > 0 ? v : 0, the backward of a ReLU whose
-// output is the mask, rounded once from the fp32 value (the LDS-tiled path
-// only: Cin and Cout multiples of 32; otherwise hipErrorInvalidValue).
-// With up (data gradient of a convolution whose input is a nearest 2x
-// upsample in its first up->ch channels, concatenated with a skip
-// connection): the first up->ch output channels are not written to out_f32
-// but summed over each 2 x 2 pixel block (fine pixels (0,0), (0,1), (1,0),
-// (1,1) in that order), masked by up->mask > 0 of the half-resolution level
-// and written as bf16 [n][h/2][w/2][up->ch] to up->out (the upsampling
-// backward with the backward of the ReLU that made its input); up->ch a
-// multiple of 64, Cin and Cout of 32, h and w even, no bf16 output.
-// With pool_out / upf_out (Cout channels each; needs out_bf): the bf16
-// output's 2 x 2 average pool (the four bf16 values summed in the order
-// (0,0), (0,1), (1,0), (1,1), times 1/4, rounded; h and w even) and its 2x
-// nearest upsample are written there as well.
+// into out_bf, and/or fp32 into out_f32 (optionally accumulated).
+//
+// Epilogue folds (ConvFolds; the LDS-tiled path only: Cin and Cout
+// multiples of 32, otherwise hipErrorInvalidValue; each rounds the fp32
+// value once, as the separate pass it replaces did):
+// - mask (Cout bf16 channels; needs out_bf): the bf16 output element (p, co)
+//   is written as mask[p][co] > 0 ? v : 0, the backward of a ReLU whose
+//   output is the mask;
+// - up (data gradient of a convolution whose input is a nearest 2x upsample
+//   in its first up->ch channels, concatenated with a skip connection): the
+//   first up->ch output channels are not written to out_f32 but summed over
+//   each 2 x 2 pixel block (fine pixels (0,0), (0,1), (1,0), (1,1) in that
+//   order), masked by up->mask > 0 of the half-resolution level and written
+//   as bf16 [n][h/2][w/2][up->ch] to up->out (the upsampling backward with
+//   the backward of the ReLU that made its input); up->ch a multiple of 64,
+//   h and w even, no bf16 output;
+// - pool / upf (Cout channels each; needs out_bf): the bf16 output's 2 x 2
+//   average pool (the four bf16 values summed in the order (0,0), (0,1),
+//   (1,0), (1,1), times 1/4, rounded; h and w even) and its 2x nearest
+//   upsample are written there as well;
+// - pool_bwd (data gradient of a convolution whose input is the 2 x 2
+//   average pool of a skip connection; no out_bf, no out_f32): for each of
+//   the four pixels q of the double-resolution level under output pixel p,
+//   out[q][co] = bf16(y[q][co] > 0 ? gs[q][co] + v[p][co] / 4 : 0) (gs: the
+//   skip channels' fp32 gradient from the decoder, y: the skip output),
+//   out [n][2h][2w][Cout].
 struct UpSum {
   bf16_t* out;
   int ch;
   Slice mask;
 };
+struct PoolBwd {
+  bf16_t* out;
+  const float* gs;
+  int gs_stride, gs_off;
+  Slice y;
+};
+struct ConvFolds {
+  const Slice* mask = nullptr;
+  const UpSum* up = nullptr;
+  const Slice* pool = nullptr;
+  const Slice* upf = nullptr;
+  const PoolBwd* pool_bwd = nullptr;
+};
 hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
                    int cout, const float* bias, bool relu, const Slice* out_bf,
                    float* out_f32, int out_f32_stride, bool accumulate,
-                   hipStream_t st, const Slice* mask = nullptr,
-                   const UpSum* up = nullptr, const Slice* pool_out = nullptr,
-                   const Slice* upf_out = nullptr);
+                   hipStream_t st, const ConvFolds& folds = ConvFolds{});
 
 // dW[tap][ci][co] += sum_p x[p + offset(tap)][ci] * dy[p][co] (fp32
 // atomics) and db[co] += sum_p dy[p][co]; x is a bf16 slice, dy bf16 with
@@ -68,13 +88,6 @@ hipError_t pack_weights(const float* w, int cin, int cout, bf16_t* wk,
 // input fp32 [P][8] -> bf16 slice
 hipError_t to_bf16(const float* x, int p, int ch, const Slice& out,
                    hipStream_t st);
-// Backward of a 2x2 average pool feeding a skip connection, with the ReLU
-// of the skip output y folded in: out[n][2i+a][2j+b][c] =
-// bf16(y > 0 ? gs[..][gs_off + c] + dy[n][i][j][c] / 4 : 0), out [P][y.ch]
-// (gs: fp32, stride gs_stride; channels, strides, offsets multiples of 8)
-hipError_t pool_bwd_relu(const float* dy, int n, int h2, int w2,
-                         const float* gs, int gs_stride, int gs_off,
-                         const Slice& y, bf16_t* out, hipStream_t st);
 // head: out[p][0..3) = sum_c x[p][c] * bf16(w[c][o]) + b[o]  (x bf16 32 ch)
 hipError_t head_fwd(const Slice& x, int p, const float* w, const float* b,
                     float* out, hipStream_t st);

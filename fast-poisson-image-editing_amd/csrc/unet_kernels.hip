@@ -105,6 +105,14 @@ struct ConvArgs {
   int pool_stride, pool_off;
   bf16_t* upf_out;
   int upf_stride, upf_off;
+  // pooling backward folded in (data gradient; no fp32 output): for the
+  // four double-resolution pixels q under output pixel p, pb_out[q][co] =
+  // bf16(pb_y[q][co] > 0 ? pb_gs[q][co] + v[p][co] / 4 : 0)
+  bf16_t* pb_out;
+  const float* pb_gs;
+  int pb_gs_stride, pb_gs_off;
+  const bf16_t* pb_y;
+  int pb_y_stride, pb_y_off;
 };
 
 // Wave tile: MT x 16 pixels by NT x 16 output channels; a block is four
@@ -451,6 +459,34 @@ __global__ void __launch_bounds__(kBlock) conv3x3_lds_kernel(ConvArgs a) {
             a.up_mask + q * a.up_mask_stride + a.up_mask_off + co0 + c8 * 8);
         *reinterpret_cast<u32x4*>(a.up_out + q * a.up_ch + co0 + c8 * 8) =
             relu_mask_bf16(m, s0, s1);
+      }
+    }
+  }
+  if (a.pb_out) {
+    constexpr int kJobs = TH * TW * (CT / 8);
+    const int W2 = 2 * a.w;
+#pragma unroll
+    for (int k = 0; k < (kJobs + kBlock - 1) / kBlock; ++k) {
+      const int job = threadIdx.x + k * kBlock;
+      const int pix = job / (CT / 8), c8 = job % (CT / 8);
+      const int y = row0 + pix / TW, x = col0 + pix % TW;
+      if (job < kJobs && y < a.h && x < a.w) {
+        const f32x4 v0 =
+            *reinterpret_cast<const f32x4*>(&T[pix * TS + c8 * 8]) * 0.25f;
+        const f32x4 v1 =
+            *reinterpret_cast<const f32x4*>(&T[pix * TS + c8 * 8 + 4]) * 0.25f;
+        const long long f = ((long long)img * 2 * a.h + 2 * y) * W2 + 2 * x;
+        const long long q[4] = {f, f + 1, f + W2, f + W2 + 1};
+        const int c = co0 + c8 * 8;
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) {
+          const f32x4* gp = reinterpret_cast<const f32x4*>(
+              a.pb_gs + q[k2] * a.pb_gs_stride + a.pb_gs_off + c);
+          const u32x4 yv = *reinterpret_cast<const u32x4*>(
+              a.pb_y + q[k2] * a.pb_y_stride + a.pb_y_off + c);
+          *reinterpret_cast<u32x4*>(a.pb_out + q[k2] * a.cout + c) =
+              relu_mask_bf16(yv, gp[0] + v0, gp[1] + v1);
+        }
       }
     }
   }
@@ -819,37 +855,6 @@ __global__ void __launch_bounds__(kBlock)
     out[(id / ch) * stride + off + id % ch] = f2bf(x[id]);
 }
 
-// Average-pooling backward into a skip connection: for the four pixels p of
-// source pixel s, g = gs[p] + dy[s] / 4 (gs: the skip channels' gradient
-// from the decoder), masked by the skip output y, into out [P][ch].
-__global__ void __launch_bounds__(kBlock)
-    pool_bwd_relu8_kernel(const float* __restrict__ dy, int h2, int w2,
-                          int cg, int total, const float* __restrict__ gs,
-                          int gs_stride, int gs_off,
-                          const bf16_t* __restrict__ y, int y_stride,
-                          int y_off, bf16_t* __restrict__ out) {
-  const int w = 2 * w2;
-  for (int id = blockIdx.x * kBlock + threadIdx.x; id < total;
-       id += gridDim.x * kBlock) {
-    const int c = (id % cg) * 8, s = id / cg;  // source pixel [n][h2][w2]
-    const int x2 = s % w2, t = s / w2;
-    const int y2 = t % h2, img = t / h2;
-    const f32x4* dp = reinterpret_cast<const f32x4*>(dy + (long long)s * cg * 8 + c);
-    const f32x4 v0 = dp[0] * 0.25f, v1 = dp[1] * 0.25f;
-    const long long b = ((long long)img * 2 * h2 + 2 * y2) * w + 2 * x2;
-    const long long q[4] = {b, b + 1, b + w, b + w + 1};
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const f32x4* gp =
-          reinterpret_cast<const f32x4*>(gs + q[k] * gs_stride + gs_off + c);
-      const u32x4 yv =
-          *reinterpret_cast<const u32x4*>(y + q[k] * y_stride + y_off + c);
-      *reinterpret_cast<u32x4*>(out + q[k] * cg * 8 + c) =
-          relu_mask_bf16(yv, gp[0] + v0, gp[1] + v1);
-    }
-  }
-}
-
 __global__ void __launch_bounds__(kBlock)
     head_fwd_kernel(const bf16_t* __restrict__ x, int x_stride, int x_off,
                     long long p, const float* __restrict__ w,
@@ -1008,9 +1013,18 @@ int blocks_for(long long n, int cap = 8192) {
 hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
                    int cout, const float* bias, bool relu, const Slice* out_bf,
                    float* out_f32, int out_f32_stride, bool accumulate,
-                   hipStream_t st, const Slice* mask, const UpSum* up,
-                   const Slice* pool_out, const Slice* upf_out) {
+                   hipStream_t st, const ConvFolds& folds) {
+  const Slice* mask = folds.mask;
+  const UpSum* up = folds.up;
+  const Slice* pool_out = folds.pool;
+  const Slice* upf_out = folds.upf;
+  const PoolBwd* pb = folds.pool_bwd;
   if (in.ch % 8 || cout % 16 || in.stride % 8 || in.off % 8)
+    return hipErrorInvalidValue;
+  if (pb && (!pb->out || !pb->gs || pb->y.ch != cout || pb->y.stride % 8 ||
+             pb->y.off % 8 || pb->gs_stride % 8 || pb->gs_off % 8 ||
+             pb->gs_off + cout > pb->gs_stride || out_bf || out_f32 || up ||
+             in.ch % 32 || cout % 32))
     return hipErrorInvalidValue;
   if (mask && (!out_bf || mask->ch != cout || mask->stride % 8 ||
                mask->off % 8 || in.ch % 32 || cout % 32))
@@ -1057,6 +1071,13 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
   a.upf_out = upf_out ? static_cast<bf16_t*>(upf_out->base) : nullptr;
   a.upf_stride = upf_out ? upf_out->stride : 0;
   a.upf_off = upf_out ? upf_out->off : 0;
+  a.pb_out = pb ? pb->out : nullptr;
+  a.pb_gs = pb ? pb->gs : nullptr;
+  a.pb_gs_stride = pb ? pb->gs_stride : 0;
+  a.pb_gs_off = pb ? pb->gs_off : 0;
+  a.pb_y = pb ? static_cast<const bf16_t*>(pb->y.base) : nullptr;
+  a.pb_y_stride = pb ? pb->y.stride : 0;
+  a.pb_y_off = pb ? pb->y.off : 0;
   const long long P = (long long)n * h * w;
   const bool vec_out = (!out_bf || (out_bf->stride % 8 == 0 &&
                                     out_bf->off % 8 == 0)) &&
@@ -1082,7 +1103,7 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
                          a);
     return hipGetLastError();
   }
-  if (mask || up || pool_out || upf_out)
+  if (mask || up || pool_out || upf_out || pb)
     return hipErrorInvalidValue;  // unaligned output slice
   constexpr int MT = 2;
   if (cout % 32 == 0) {
@@ -1147,21 +1168,6 @@ hipError_t to_bf16(const float* x, int p, int ch, const Slice& out,
   hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for((long long)p * ch)),
                      dim3(kBlock), 0, st, x, (long long)p, ch,
                      static_cast<bf16_t*>(out.base), out.stride, out.off);
-  return hipGetLastError();
-}
-
-hipError_t pool_bwd_relu(const float* dy, int n, int h2, int w2,
-                         const float* gs, int gs_stride, int gs_off,
-                         const Slice& y, bf16_t* out, hipStream_t st) {
-  const int ch = y.ch;
-  const long long total8 = (long long)n * h2 * w2 * (ch / 8);
-  if (ch % 8 || gs_stride % 8 || gs_off % 8 || y.stride % 8 || y.off % 8 ||
-      total8 >= (1LL << 31))
-    return hipErrorInvalidValue;
-  hipLaunchKernelGGL(pool_bwd_relu8_kernel, dim3(blocks_for(total8)),
-                     dim3(kBlock), 0, st, dy, h2, w2, ch / 8, (int)total8, gs,
-                     gs_stride, gs_off, static_cast<const bf16_t*>(y.base),
-                     y.stride, y.off, out);
   return hipGetLastError();
 }
 
