@@ -73,7 +73,8 @@ def bench_batch(a):
   process.register_backend("grab", lambda **kw: Grab(), lambda **kw: Grab())
   tile = ([int(v) for v in a.tile.split(",")] if a.tile else [8, 1, 64])
   b = core_hip.BatchSolver(sweeps_per_launch=tile[0], cols_per_lane=tile[1],
-                           rows_per_task=tile[2])
+                           rows_per_task=tile[2],
+                           channels=tile[3] if len(tile) > 3 else 0)
   masked = []
   for w in items:
     p = process.GridProcessor(w.gradient, "grab")
@@ -229,7 +230,7 @@ def main():
   ap.add_argument("--iters", type=int, default=0,
                   help="sweeps per step (default: the workload's)")
   ap.add_argument("--tile", type=str, default="",
-                  help="S,C,rows to skip the autotuner")
+                  help="S,C,rows[,channels] to skip the autotuner")
   ap.add_argument("--cpu-budget", type=float, default=12.0)
   ap.add_argument("--no-cpu-baseline", action="store_true")
   ap.add_argument("--no-warm-start", action="store_true",

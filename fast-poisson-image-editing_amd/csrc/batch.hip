@@ -205,6 +205,23 @@ int pie_batch_create(int order, int sweeps_per_launch, int cols_per_lane,
   return PIE_OK;
 }
 
+int pie_batch_set_tile_channels(pie_batch* b, int channels) {
+  if (!b) return fail(PIE_EINVAL, "NULL batch");
+  if (channels != 0 && channels != 1 && channels != 3)
+    return fail(PIE_EINVAL, "channels per task must be 0, 1 or 3");
+  TileCfg c = b->cfg;
+  c.N = channels;
+  if (!pie::tile_supported(c))
+    return fail(PIE_EUNSUPPORTED,
+                "tile (sweeps " + std::to_string(c.S) + ", columns/lane " +
+                    std::to_string(c.C) + ", channels " +
+                    std::to_string(pie::tile_nch(c)) + ") not compiled");
+  b->cfg = c;
+  b->dirty = true;  // the task table depends on the channel split
+  drop_graph(b);
+  return PIE_OK;
+}
+
 int pie_batch_destroy(pie_batch* b) {
   if (!b) return PIE_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);

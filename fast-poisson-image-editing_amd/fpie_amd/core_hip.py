@@ -362,7 +362,8 @@ class BatchSolver:
   shared tile queue with the round loop in a hipGraph."""
 
   def __init__(self, order: str = "ulrd", sweeps_per_launch: int = 8,
-               cols_per_lane: int = 1, rows_per_task: int = 64):
+               cols_per_lane: int = 1, rows_per_task: int = 64,
+               channels: int = 0):
     lib = native.load()
     h = ctypes.c_void_p()
     native.check(lib.pie_batch_create(native.ORDERS[order], sweeps_per_launch,
@@ -370,6 +371,13 @@ class BatchSolver:
                                       ctypes.byref(h)))
     self._h, self._lib = h, lib
     self.shapes: list[tuple[int, int]] = []
+    if channels:
+      self.set_tile_channels(channels)
+
+  def set_tile_channels(self, channels: int) -> None:
+    """Channels per wave task (1 or 3; 0 = the tile's default); the results
+    do not depend on it."""
+    native.check(self._lib.pie_batch_set_tile_channels(self._h, channels))
 
   def __del__(self):
     h = getattr(self, "_h", None)

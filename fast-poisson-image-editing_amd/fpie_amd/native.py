@@ -79,6 +79,7 @@ _SIGNATURES = {
                         ctypes.POINTER(ctypes.c_double)], _ip),
     "pie_batch_create": ([_ip, _ip, _ip, _ip, ctypes.POINTER(_vp)], _ip),
     "pie_batch_destroy": ([_vp], _ip),
+    "pie_batch_set_tile_channels": ([_vp, _ip], _ip),
     "pie_batch_add": ([_vp, _ip, _ip, _vp, _vp, _vp, _ip, ctypes.c_float,
                        ctypes.POINTER(_ip)], _ip),
     "pie_batch_solve": ([_vp, _ip, ctypes.POINTER(ctypes.c_double),

@@ -240,6 +240,11 @@ typedef struct pie_batch pie_batch;
  * task (0 = default 8, 1, 64). */
 int pie_batch_create(int order, int sweeps_per_launch, int cols_per_lane,
                      int seg_rows, pie_batch** out);
+/* Channels per wave task of the batch's sweep kernel (1 or 3; 0 = the
+ * tile's default: 3 for one column per lane, 1 for two), as
+ * pie_set_tile_channels; results are identical, only the split of the
+ * work over waves differs.  Any time before pie_batch_solve. */
+int pie_batch_set_tile_channels(pie_batch* b, int channels);
 int pie_batch_destroy(pie_batch* b);
 /* Add one GridSolver problem (arguments as pie_reset_grid); *index gets its
  * position in the batch. */

@@ -96,6 +96,30 @@ def test_solve_restarts_from_the_initial_state(hip):
     np.testing.assert_array_equal(bits(b.read(i, state=True)[2]), first[i])
 
 
+@pytest.mark.parametrize("tile", [(8, 1, 16, 1), (8, 2, 32, 0), (4, 1, 64, 3)])
+def test_channel_split_tiles_match_oracle(hip, oracle, tile):
+  """One channel a wave task (or the two-column tile's default) changes only
+  how the work is split over waves: every item still equals the oracle."""
+  items = _items(6, 17)
+  budgets = [3, 8, 21, 40, 64, 13]
+  s, c, rows, ch = tile
+  b = hip.BatchSolver(sweeps_per_launch=s, cols_per_lane=c,
+                      rows_per_task=rows, channels=ch)
+  for (m, t, g), k in zip(items, budgets):
+    b.add(m, t, g, max_sweeps=k, tol=0.0)
+  b.solve(rounds_per_graph=4)
+  for i, ((m, t, g), k) in enumerate(zip(items, budgets)):
+    assert b.item(i)["sweeps"] == k
+    want = oracle.grid_jacobi(m, t, g, k, "ulrd")
+    assert np.array_equal(bits(b.read(i, state=True)[2]), bits(want)), (i, k)
+
+
+def test_batch_rejects_bad_channels(hip):
+  b = hip.BatchSolver()
+  with pytest.raises(RuntimeError, match="channels per task"):
+    b.set_tile_channels(2)
+
+
 def test_batch_rejects_bad_items(hip):
   b = hip.BatchSolver()
   m = np.zeros((10, 10), np.int32)
