@@ -1087,9 +1087,12 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
     constexpr int LMT = 2;
     const unsigned tiles = (unsigned)n * ((h + 4 * LMT - 1) / (4 * LMT)) *
                            ((w + 15) / 16);
-    if (cout % 64 == 0 && in.ch >= 128) {
+    const unsigned tiles4 = (unsigned)n * ((h + 15) / 16) * ((w + 15) / 16);
+    if (cout % 64 == 0 && in.ch >= 128 && tiles4 * (cout / 64) >= 512) {
       // deep inputs: 16 x 16-pixel tiles, twice the MFMAs per staged weight
-      const unsigned tiles4 = (unsigned)n * ((h + 15) / 16) * ((w + 15) / 16);
+      // (when they still give two blocks a CU: small inputs, such as one
+      // 256 x 256 image, keep the 8 x 16 tiles, whose extra blocks hide
+      // more latency)
       hipLaunchKernelGGL((conv3x3_lds_kernel<4, 4>),
                          dim3(xcd_grid(tiles4, cout / 64)), dim3(kBlock), 0,
                          st, a);
