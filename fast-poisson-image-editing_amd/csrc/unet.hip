@@ -473,6 +473,12 @@ int pie_unet_train_step(pie_unet* u, const float* x, const float* target,
   return PIE_OK;
 }
 
+int pie_unet_set_conv_tiles(int min_blocks16) {
+  if (min_blocks16 < -1) return fail(PIE_EINVAL, "min_blocks16 below -1");
+  pie::unet::set_tile16_min_blocks(min_blocks16);
+  return PIE_OK;
+}
+
 int pie_unet_stats(pie_unet* u, double* ms, double* conv_flops_done) {
   if (!u) return fail(PIE_EINVAL, "NULL argument");
   if (ms) *ms = u->ms_total;

@@ -67,6 +67,10 @@ struct ConvFolds {
   const Slice* upf = nullptr;
   const PoolBwd* pool_bwd = nullptr;
 };
+// Layers with 128+ input channels (and Cout a multiple of 64) use 16 x 16
+// tiles when that gives at least this many blocks (default 512; 0 =
+// always, -1 = never); the result does not depend on it.
+void set_tile16_min_blocks(int min_blocks);
 hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
                    int cout, const float* bias, bool relu, const Slice* out_bf,
                    float* out_f32, int out_f32_stride, bool accumulate,

@@ -6,6 +6,7 @@
 //   A (16 x 32): row i, k = 8g .. 8g+7   (8 bf16 in four VGPRs)
 //   B (32 x 16): col i, k = 8g .. 8g+7
 //   D (16 x 16): col i, rows 4g .. 4g+3  (4 fp32)
+#include <atomic>
 #include <hip/hip_runtime.h>
 
 #include "unet_kernels.h"
@@ -1010,6 +1011,14 @@ int blocks_for(long long n, int cap = 8192) {
 
 }  // namespace
 
+namespace {
+std::atomic<int> g_tile16_min_blocks{512};
+}  // namespace
+
+void set_tile16_min_blocks(int min_blocks) {
+  g_tile16_min_blocks.store(min_blocks, std::memory_order_relaxed);
+}
+
 hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
                    int cout, const float* bias, bool relu, const Slice* out_bf,
                    float* out_f32, int out_f32_stride, bool accumulate,
@@ -1088,7 +1097,9 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
     const unsigned tiles = (unsigned)n * ((h + 4 * LMT - 1) / (4 * LMT)) *
                            ((w + 15) / 16);
     const unsigned tiles4 = (unsigned)n * ((h + 15) / 16) * ((w + 15) / 16);
-    if (cout % 64 == 0 && in.ch >= 128 && tiles4 * (cout / 64) >= 512) {
+    const int min16 = g_tile16_min_blocks.load(std::memory_order_relaxed);
+    if (cout % 64 == 0 && in.ch >= 128 && min16 >= 0 &&
+        (long long)tiles4 * (cout / 64) >= min16) {
       // deep inputs: 16 x 16-pixel tiles, twice the MFMAs per staged weight
       // (when they still give two blocks a CU: small inputs, such as one
       // 256 x 256 image, keep the 8 x 16 tiles, whose extra blocks hide

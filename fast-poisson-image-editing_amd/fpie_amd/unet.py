@@ -94,6 +94,13 @@ def pad8(a: np.ndarray, mode: str = "edge") -> np.ndarray:
   return np.pad(a, width, mode=mode)
 
 
+def set_conv_tiles(min_blocks16: int = 512) -> None:
+  """Convolution tile policy of this process: layers with 128+ input
+  channels use 16 x 16-pixel tiles when that gives at least min_blocks16
+  blocks (0 = always, -1 = never).  Results do not depend on it."""
+  native.check(native.load().pie_unet_set_conv_tiles(int(min_blocks16)))
+
+
 class UNet:
   """The network on the GPU for one batch shape (pie_unet_* C ABI)."""
 

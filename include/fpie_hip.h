@@ -294,6 +294,11 @@ int pie_unet_train_step(pie_unet* u, const float* x, const float* target,
 /* Device time of the forward + backward passes so far and the convolution
  * FLOPs they did (for MFMA utilisation). */
 int pie_unet_stats(pie_unet* u, double* ms, double* conv_flops);
+/* Convolution tile policy of this process (all networks): layers with 128+
+ * input channels use 16 x 16-pixel tiles when that still gives at least
+ * min_blocks16 blocks (default 512), 8 x 16 otherwise; 0 = always, -1 =
+ * never.  Results do not depend on it. */
+int pie_unet_set_conv_tiles(int min_blocks16);
 
 #ifdef __cplusplus
 }

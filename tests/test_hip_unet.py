@@ -50,11 +50,13 @@ def test_forward_matches_oracle(unet):
   assert rms < 1e-2 and worst < 5e-2, (rms, worst)
 
 
-def test_gradients_match_oracle(unet):
+@pytest.mark.parametrize("hw", [(32, 32), (48, 40)])
+def test_gradients_match_oracle(unet, hw):
   import unet_ref as R
-  x, y, m = _problem(2, 32, 32, 2)
+  h, w = hw
+  x, y, m = _problem(2, h, w, 2)
   p = unet.init_params(5)
-  net = unet.UNet(2, 32, 32, params=p)
+  net = unet.UNet(2, h, w, params=p)
   loss, g = net.loss_grad(x, y, m)
   po = _oracle_params(unet, p)
   out, act = R.forward(po, x.astype(np.float64))
@@ -71,6 +73,35 @@ def test_gradients_match_oracle(unet):
     cos = a @ b / (np.linalg.norm(a) * nb)
     assert cos > 0.99, (name, cos)
     assert abs(np.linalg.norm(a) / nb - 1) < 5e-2, (name, np.linalg.norm(a) / nb)
+
+
+def test_tile_policy_does_not_change_results(unet):
+  """16 x 16 tiles (and the epilogue folds run with them: pooling, the
+  upsampling backward, the pooling backward) on every eligible layer give
+  the same forward output bit for bit, and the same gradients up to the
+  order of the weight gradient's fp32 atomics."""
+  x, y, m = _problem(2, 48, 40, 7)
+  p = unet.init_params(11)
+  try:
+    res = {}
+    for policy in (-1, 0):
+      unet.set_conv_tiles(policy)
+      net = unet.UNet(2, 48, 40, params=p)
+      res[policy] = (net.forward(x), net.loss_grad(x, y, m))
+  finally:
+    unet.set_conv_tiles(512)
+  (f0, (l0, g0)), (f1, (l1, g1)) = res[-1], res[0]
+  assert np.array_equal(f0.view(np.uint32), f1.view(np.uint32))
+  assert l0 == l1
+  gd0, gd1 = unet.unflatten(g0), unet.unflatten(g1)
+  for name, _ in unet.param_shapes():
+    a, b = gd0[name].astype(np.float64), gd1[name].astype(np.float64)
+    assert np.linalg.norm(a - b) <= 1e-5 * (np.linalg.norm(a) + 1e-30), name
+
+
+def test_tile_policy_rejects_bad_values(unet):
+  with pytest.raises(RuntimeError):
+    unet.set_conv_tiles(-2)
 
 
 def test_adam_steps_reduce_the_loss(unet):
