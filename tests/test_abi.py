@@ -54,6 +54,17 @@ def test_last_error_starts_empty():
   assert native.load().pie_last_error() == b""
 
 
+@pytest.mark.skipif(not native.available(), reason="libfpie_hip.so not built")
+def test_tuning_calls_validate_before_any_device_work():
+  """Host-side argument checks of the tile controls (no HIP call made)."""
+  lib = native.load()
+  assert lib.pie_batch_set_tile_channels(None, 1) < 0
+  assert b"NULL batch" in lib.pie_last_error()
+  assert lib.pie_unet_set_conv_tiles(-2) < 0
+  assert b"below -1" in lib.pie_last_error()
+  assert lib.pie_unet_set_conv_tiles(512) == 0  # the default, restored
+
+
 def test_cli_check_backend_lists_hip():
   code = ("import sys; sys.path.insert(0, %r); "
           "from fpie_amd.cli import main; main(['--check-backend'])" % PKG)
