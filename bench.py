@@ -1,23 +1,25 @@
 """Benchmark of the hip Poisson solver on one MI355X.
 
-Default workload (``grid4096``): GridSolver, 4096 x 4096 x 3 uint8 blob
-composites, centred square mask of side 3800 (14 440 000 solved pixels in a
-3802 x 3802 crop), gradient ``src``, float32 state, 5000 Jacobi sweeps per
-step (the g+U+L+R+D order of the openmp GridSolver).  A step is one
-``GridSolver.step(5000)``: 5000 sweeps, then the uint8 crop and the residual
-sums copied back.  Inputs are resident in HBM (``reset`` is outside the
-timed region).
+Default workload (``grid8192``, the roofline run): GridSolver on an 8192 x
+8192 x 3 uint8 blob composite whose mask is the union of 6 seeded ellipses
+(60 % coverage; a 7183 x 7546 crop), gradient ``max``, float32 state, 10000
+Jacobi sweeps per step (the g+U+L+R+D order of the openmp GridSolver).  A
+step is one ``GridSolver.step(10000)``: the sweeps, then the uint8 crop and
+the residual sums copied back.  Inputs are resident in HBM (``reset`` is
+outside the timed region).
 
 Printed: one JSON line with the metric (solved-pixel updates per second),
 the roofline of the sweep kernel (algorithmic bytes per launch over its
-event-timed average launch duration, against the 8 TB/s HBM peak), the
-time to a mean residual of 1e-3 on an unseen 1024 x 1024 composite with and
-without the learned warm start (``time_to_residual``), the MFMA
-utilisation of the warm start's U-Net in a training step (``unet_mfma``:
-batch 64 x 256 x 256, bf16, device time) and the
-reference's CPU path timed on this host (its openmp GridSolver compiled from
-the reference sources, or the oracle's restatement when that is absent) on a
-bounded sample of the same workload.
+event-timed average launch duration, against the 8 TB/s HBM peak, and the
+rocprofv3-measured HBM bytes of the same kernel and tile when recorded in
+profiles/traffic.json), the 4096 x 4096 workload under Jacobi and under
+the gcc backend's Gauss-Seidel order (``extra``), the time to a mean
+residual of 1e-3 on an unseen 1024 x 1024 composite with and without the
+learned warm start (``time_to_residual``), the MFMA utilisation of the warm
+start's U-Net in a training step (``unet_mfma``) and the reference's CPU
+path timed on this host (its openmp GridSolver compiled from the reference
+sources, or the oracle's restatement when that is absent) on a bounded
+sample of the same workload, on all host threads and on 8.
 """
 
 from __future__ import annotations
@@ -119,59 +121,100 @@ def bench_batch(a):
   print(json.dumps(line), flush=True)
 
 
-def traffic_from_profiles(workload: str, schedule: str):
+def kernel_name(tile, schedule: str, order: str = "ulrd") -> str:
+  """Name of the sweep-kernel instantiation a tile runs (as rocprofv3
+  prints it)."""
+  S, C, rows, nch = tile
+  sched = {"jacobi": 0, "gs_lex": 1, "redblack": 2}[schedule]
+  return (f"void pie::detail::march_kernel<{S}, {C}, {nch}, {sched}, "
+          f"{1 if order == 'ulrd' else 0}>(pie::MarchArgs)")
+
+
+def traffic_from_profiles(workload: str, schedule: str, tile):
   """HBM bytes per launch of the sweep kernel measured with rocprofv3 PMC
-  counters (profiles/traffic_<workload>_<schedule>.json), if recorded."""
-  path = os.path.join(ROOT, "profiles",
-                      f"traffic_{workload}_{schedule}.json")
+  counters (profiles/traffic.json) for exactly this workload, schedule,
+  tile (sweeps, columns per lane, rows per task, channels per task) and
+  kernel; None when no such record exists."""
+  path = os.path.join(ROOT, "profiles", "traffic.json")
   if not os.path.exists(path):
     return None
   with open(path) as f:
-    return json.load(f)
+    records = json.load(f)["records"]
+  for r in records:
+    if (r["workload"] == workload and r["schedule"] == schedule and
+        list(r["tile"]) == list(tile) and
+        r["kernel"] == kernel_name(tile, schedule)):
+      return r
+  return None
+
+
+def cpu_model() -> str:
+  try:
+    with open("/proc/cpuinfo") as f:
+      for line in f:
+        if line.startswith("model name"):
+          return line.split(":", 1)[1].strip()
+  except OSError:
+    pass
+  return "unknown"
 
 
 def cpu_baseline(w, budget_s: float):
   """The reference's openmp solver core (compiled from the reference sources
   into oracle/_ref) driven by the processor on the same composite, timed on
-  a number of sweeps sized to about budget_s seconds; the oracle's port of
-  the same loops when oracle/_ref is absent."""
+  a number of sweeps sized to about budget_s / 2 seconds, once on all host
+  threads (OMP_NUM_THREADS, else the CPUs this process may run on) and once
+  on 8 (the reference's default cap, fpie/process.py:11,26); the oracle's
+  port of the same loops, one thread, when oracle/_ref is absent."""
   sys.path.insert(0, os.path.join(ROOT, "oracle"))
   import oracle as O
   from fpie_amd import process
 
-  threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (
-      os.cpu_count() or 1)
-  threads = min(threads, 16)
+  avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") \
+      else (os.cpu_count() or 1)
+  threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or avail
   ref = O.ref_core("core_openmp")
+  masked = int(process.prepare_mask(w.mask)[0].sum())
+  cls = process.EquProcessor if w.method == "equ" else process.GridProcessor
+
+  def timed(make_equ, make_grid, seconds):
+    process.register_backend("cpu-baseline", make_equ, make_grid)
+    p = cls(w.gradient, "cpu-baseline")
+    p.reset(w.src, w.mask, w.tgt, w.mask_on_src, w.mask_on_tgt)
+    p.core.step(1)  # warm-up
+    t0 = time.perf_counter()
+    p.core.step(2)
+    dt = (time.perf_counter() - t0) / 2
+    k = max(2, int(seconds / max(dt, 1e-6)))
+    t0 = time.perf_counter()
+    p.core.step(k)
+    dt = time.perf_counter() - t0
+    return masked * k / dt, k, dt
+
   if ref is not None:
     # openmp flags of docs/benchmark.md (GridSolver --grid-x 2 --grid-y 16)
-    process.register_backend("cpu-baseline",
-                             lambda **kw: ref.EquSolver(threads),
-                             lambda **kw: ref.GridSolver(2, 16, threads))
-    kind, cores = "reference", threads
-    what = "fpie/core/openmp compiled from the reference sources"
-  else:
-    process.register_backend("cpu-baseline", lambda **kw: O.EquCore("openmp"),
-                             lambda **kw: O.GridCore(2, 16, "openmp"))
-    kind, cores = "port", 1
-    what = "oracle restatement of fpie/core/openmp (1 thread)"
-  cls = process.EquProcessor if w.method == "equ" else process.GridProcessor
-  p = cls(w.gradient, "cpu-baseline")
-  p.reset(w.src, w.mask, w.tgt, w.mask_on_src, w.mask_on_tgt)
-  masked = int(process.prepare_mask(w.mask)[0].sum())
-  p.core.step(1)  # warm-up
-  t0 = time.perf_counter()
-  p.core.step(2)
-  dt = (time.perf_counter() - t0) / 2
-  k = max(2, int(budget_s / max(dt, 1e-6)))
-  t0 = time.perf_counter()
-  p.core.step(k)
-  dt = time.perf_counter() - t0
-  return {"value": masked * k / dt, "unit": "pixel-updates/s",
-          "cores": cores, "kind": kind,
-          "sample": f"{what}: {k} sweeps of the {w.name} problem "
-                    f"({masked} solved pixels), {dt:.2f} s, "
-                    f"{cores} thread(s)"}
+    runs = {}
+    for n in sorted({threads, 8}, reverse=True):
+      runs[n] = timed(lambda **kw: ref.EquSolver(n),
+                      lambda **kw: ref.GridSolver(2, 16, n), budget_s / 2)
+    v, k, dt = runs[threads]
+    out = {"value": v, "unit": "pixel-updates/s", "cores": threads,
+           "kind": "reference",
+           "sample": f"fpie/core/openmp compiled from the reference sources: "
+                     f"{k} sweeps of the {w.name} problem ({masked} solved "
+                     f"pixels), {dt:.2f} s, {threads} threads",
+           "cpu_model": cpu_model(), "host_cpus": avail}
+    if 8 in runs and threads != 8:
+      v8, k8, dt8 = runs[8]
+      out["threads_8"] = {"value": v8, "sweeps": k8, "seconds": dt8}
+    return out
+  v, k, dt = timed(lambda **kw: O.EquCore("openmp"),
+                   lambda **kw: O.GridCore(2, 16, "openmp"), budget_s)
+  return {"value": v, "unit": "pixel-updates/s", "cores": 1, "kind": "port",
+          "sample": f"oracle restatement of fpie/core/openmp (1 thread): {k} "
+                    f"sweeps of the {w.name} problem ({masked} solved "
+                    f"pixels), {dt:.2f} s",
+          "cpu_model": cpu_model(), "host_cpus": avail}
 
 
 def bench_warm_start(size: int = 1024, seed: int = 900_001,
@@ -218,21 +261,96 @@ def bench_warm_start(size: int = 1024, seed: int = 900_001,
   }
 
 
+def run_grid(name: str, schedule: str, steps: int, warmup: int, tile: str,
+             iters: int = 0) -> dict:
+  """K timed steps of one workload on the GPU (after W untimed ones), with
+  the sweep kernel's roofline; the solver is freed before returning."""
+  from fpie_amd import core_hip
+
+  t_in = time.perf_counter()
+  w, args = build_inputs(name)
+  iters = iters or w.iters
+  print(f"# inputs {w.name} built in {time.perf_counter() - t_in:.1f} s",
+        file=sys.stderr, flush=True)
+  if w.method == "grid":
+    _, m, t, g = args
+    core = core_hip.GridSolver(schedule=schedule, order="ulrd")
+    core.reset(0, m, t, g)
+  else:
+    N, A, X, B = args
+    core = core_hip.EquSolver(schedule=schedule)
+    from fpie_amd.process import prepare_mask
+    pm, *_ = prepare_mask(w.mask)
+    core.partition(pm)
+    core.reset(N, A, X, B)
+  masked = core.shape()["masked"]
+  if tile:
+    core.set_tile(*[int(x) for x in tile.split(",")])
+  else:
+    core.autotune(8)
+  t4 = list(core.tile()) + [core.tile_channels()]
+  print(f"# {name} {schedule}: tile S={t4[0]} C={t4[1]} rows={t4[2]} "
+        f"channels/task {t4[3]}", file=sys.stderr, flush=True)
+  for _ in range(warmup):
+    core.step(iters)
+  core.sync()
+  core.set_timing(True)
+  t0 = time.perf_counter()
+  for _ in range(steps):
+    core.step(iters)
+  core.sync()
+  dt = time.perf_counter() - t0
+  tm = core.timing()
+  del core
+  updates = masked * iters * steps
+  launch_ms = tm["kernel_ms"] / max(tm["launches"], 1)
+  launches_per_step = tm["launches"] / max(steps, 1)
+  sweeps_per_launch = iters / max(launches_per_step, 1)
+  alg_bytes = BYTES_PER_UPDATE * masked * sweeps_per_launch
+  achieved = alg_bytes / (launch_ms * 1e-3) / 1e9
+  traffic = traffic_from_profiles(w.name, schedule, t4)
+  hbm = (traffic["bytes_per_launch"] / (launch_ms * 1e-3) / 1e9
+         if traffic else None)
+  return {
+      "workload": w,
+      "value": updates / dt,
+      "ms_per_step": dt * 1e3 / steps,
+      "iters": iters,
+      "masked": masked,
+      "tile": t4,
+      "roofline": {
+          "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
+          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
+          "traffic": traffic["bytes_per_launch"] if traffic else None,
+          # measured HBM bytes per launch over the launch time
+          "hbm_achieved": hbm,
+          "hbm_frac": hbm / HBM_PEAK_GBPS if hbm else None,
+          "kernel": kernel_name(t4, schedule),
+          "bytes_per_update": BYTES_PER_UPDATE,
+          "sweeps_per_launch": sweeps_per_launch,
+          "avg_launch_ms": launch_ms,
+          "launches_timed": tm["launches"],
+          "traffic_profile": traffic["summary"] if traffic else None},
+  }
+
+
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument("--gpus", type=int, default=1)
   ap.add_argument("--steps", type=int, default=3)
   ap.add_argument("--warmup", type=int, default=1)
-  ap.add_argument("--workload", default="grid4096",
-                  choices=["grid4096", "equ512", "grid8192", "batch256"])
+  ap.add_argument("--workload", default="grid8192",
+                  choices=["grid8192", "grid4096", "equ512", "batch256"])
   ap.add_argument("--schedule", default="jacobi",
                   choices=["jacobi", "gs_lex", "redblack"])
   ap.add_argument("--iters", type=int, default=0,
                   help="sweeps per step (default: the workload's)")
   ap.add_argument("--tile", type=str, default="",
                   help="S,C,rows[,channels] to skip the autotuner")
-  ap.add_argument("--cpu-budget", type=float, default=12.0)
+  ap.add_argument("--cpu-budget", type=float, default=16.0)
   ap.add_argument("--no-cpu-baseline", action="store_true")
+  ap.add_argument("--no-extra", action="store_true",
+                  help="skip the 4096 x 4096 jacobi / gs_lex lines")
   ap.add_argument("--no-warm-start", action="store_true",
                   help="skip the time-to-residual comparison")
   ap.add_argument("--no-unet", action="store_true",
@@ -246,87 +364,52 @@ def main():
   if a.workload == "batch256":
     return bench_batch(a)
 
-  from fpie_amd import core_hip, native
+  from fpie_amd import native
 
   print(f"# {native.device_info()}", file=sys.stderr, flush=True)
-  t_in = time.perf_counter()
-  w, args = build_inputs(a.workload)
-  iters = a.iters or w.iters
-  print(f"# inputs {w.name} built in {time.perf_counter() - t_in:.1f} s",
-        file=sys.stderr, flush=True)
-  if w.method == "grid":
-    _, m, t, g = args
-    core = core_hip.GridSolver(schedule=a.schedule, order="ulrd")
-    core.reset(0, m, t, g)
-  else:
-    N, A, X, B = args
-    core = core_hip.EquSolver(schedule=a.schedule)
-    from fpie_amd.process import prepare_mask
-    pm, *_ = prepare_mask(w.mask)
-    core.partition(pm)
-    core.reset(N, A, X, B)
-  masked = core.shape()["masked"]
-  if a.tile:
-    core.set_tile(*[int(v) for v in a.tile.split(",")])
-  else:
-    core.autotune(8)
-  tile = core.tile()
-  print(f"# tile S={tile[0]} C={tile[1]} rows={tile[2]} channels/task "
-        f"{core.tile_channels()}", file=sys.stderr, flush=True)
-
-  for _ in range(a.warmup):
-    core.step(iters)
-  core.sync()
-  core.set_timing(True)
-  t0 = time.perf_counter()
-  for _ in range(a.steps):
-    core.step(iters)
-  core.sync()
-  dt = time.perf_counter() - t0
-  tm = core.timing()
-  updates = masked * iters * a.steps
-  value = updates / dt
-  launch_ms = tm["kernel_ms"] / max(tm["launches"], 1)
-  sweeps_per_launch = tm["pixel_updates"] / max(tm["launches"], 1) / masked
-  alg_bytes = BYTES_PER_UPDATE * masked * sweeps_per_launch
-  achieved = alg_bytes / (launch_ms * 1e-3) / 1e9
-  traffic = traffic_from_profiles(w.name, a.schedule)
-  if traffic and traffic.get("tile") != list(tile):
-    traffic = None  # measured for another tile
+  r = run_grid(a.workload, a.schedule, a.steps, a.warmup, a.tile, a.iters)
+  w = r["workload"]
   line = {
       "metric": "Jacobi pixel-updates/s + % HBM peak; time-to-residual "
                 "warm-start vs plain",
-      "value": value,
+      "value": r["value"],
       "unit": "pixel-updates/s",
       "n_gpus": 1,
       "steps": a.steps,
       "warmup": a.warmup,
-      "ms_per_step": dt * 1e3 / a.steps,
+      "ms_per_step": r["ms_per_step"],
       "higher_is_better": True,
       "scaling": "weak",
       "vs_baseline": None,
       "dtype": "f32",
       "data": "synthetic (seeded blob composites, fpie_amd.synth)",
       "config": {"workload": w.name, "method": w.method,
-                 "image": list(w.src.shape), "solved_pixels": masked,
-                 "gradient": w.gradient, "sweeps_per_step": iters,
+                 "image": list(w.src.shape), "solved_pixels": r["masked"],
+                 "gradient": w.gradient, "sweeps_per_step": r["iters"],
                  "schedule": a.schedule,
                  "order": "ulrd" if w.method == "grid" else "udlr",
-                 "tile": {"sweeps_per_launch": tile[0],
-                          "cols_per_lane": tile[1], "rows_per_task": tile[2]},
+                 "tile": {"sweeps_per_launch": r["tile"][0],
+                          "cols_per_lane": r["tile"][1],
+                          "rows_per_task": r["tile"][2],
+                          "channels_per_task": r["tile"][3]},
                  "parallelism": "single GPU"},
-      "roofline": {"bound": "hbm", "achieved": achieved,
-                   "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                   "frac": achieved / HBM_PEAK_GBPS,
-                   "traffic": traffic["bytes_per_launch"] if traffic else None,
-                   "kernel": "march_kernel (temporally blocked sweep)",
-                   "bytes_per_update": BYTES_PER_UPDATE,
-                   "sweeps_per_launch": sweeps_per_launch,
-                   "avg_launch_ms": launch_ms,
-                   "launches_timed": tm["launches"]},
+      "roofline": r["roofline"],
   }
+  if not a.no_extra and a.workload == "grid8192":
+    # the 4096 x 4096 square workload: Jacobi, and the gcc backend's
+    # Gauss-Seidel order (bit-exact to fpie/core/gcc/grid.cc)
+    extra = {}
+    for sched, steps in (("jacobi", 3), ("gs_lex", 2)):
+      e = run_grid("grid4096", sched, steps, 1, "")
+      extra[f"grid4096_{sched}"] = {
+          "value": e["value"], "unit": "pixel-updates/s",
+          "ms_per_step": e["ms_per_step"], "sweeps_per_step": e["iters"],
+          "steps": steps, "tile": e["tile"],
+          "avg_launch_ms": e["roofline"]["avg_launch_ms"],
+          "frac": e["roofline"]["frac"],
+          "hbm_frac": e["roofline"]["hbm_frac"]}
+    line["extra"] = extra
   if not a.no_warm_start and w.method == "grid":
-    del core
     line["time_to_residual"] = bench_warm_start()
   if not a.no_unet:
     # the warm start's network: one Adam step of UNet(64, 256, 256) in bf16

@@ -287,12 +287,23 @@ int pie_batch_add(pie_batch* b, int h, int w, const int32_t* mask,
   PIE_HIP(pie::launch_deinterleave(reinterpret_cast<const float*>(stage.p),
                                    it->g.p + pie::kPadLeft, it->plane, h, w,
                                    it->P, b->stream));
+  // NaN guidance at unsolved pixels and in the padding (the sweep kernel
+  // tells solved pixels by it)
+  PIE_HIP(hipMemsetAsync(counts.p, 0, sizeof(unsigned long long), b->stream));
+  PIE_HIP(pie::launch_box_guidance(it->g.p, it->m.p, it->plane, h, it->P,
+                                   pitch_of(w), counts.p, b->stream));
   // both state buffers start as the initial state (padding included)
   PIE_HIP(hipMemcpyAsync(it->xa.p, it->x0.p, fl * sizeof(float),
                          hipMemcpyDeviceToDevice, b->stream));
   PIE_HIP(hipMemcpyAsync(it->xb.p, it->x0.p, fl * sizeof(float),
                          hipMemcpyDeviceToDevice, b->stream));
+  unsigned long long nan_solved = 0;
+  PIE_HIP(hipMemcpyAsync(&nan_solved, counts.p, sizeof(nan_solved),
+                         hipMemcpyDeviceToHost, b->stream));
   PIE_HIP(hipStreamSynchronize(b->stream));
+  if (nan_solved)
+    return fail(PIE_EINVAL, std::to_string(nan_solved) +
+                                " guidance values of solved pixels are NaN");
   if (index) *index = (int)b->items.size();
   b->items.push_back(std::move(it));
   b->dirty = true;

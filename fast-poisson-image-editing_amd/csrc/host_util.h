@@ -40,11 +40,18 @@ struct DevBuf {
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
+  // n records the capacity only once the allocation succeeded, so a failed
+  // alloc leaves an empty buffer (p null, n 0) that the next grow-if-needed
+  // test allocates again.
   hipError_t alloc(size_t count) {
     release();
+    T* q = nullptr;
+    const hipError_t e = hipMalloc(reinterpret_cast<void**>(&q),
+                                   std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) return e;
+    p = q;
     n = count;
-    return hipMalloc(reinterpret_cast<void**>(&p),
-                     std::max<size_t>(count, 1) * sizeof(T));
+    return hipSuccess;
   }
   void release() {
     if (p) (void)hipFree(p);

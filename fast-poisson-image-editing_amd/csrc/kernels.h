@@ -21,6 +21,7 @@ namespace pie {
 
 constexpr int kPadLeft = 64;
 constexpr int kPadRight = 256;  // >= 64 * (largest columns per lane)
+constexpr int kSweepBlockWaves = 4;  // waves per work-group of the sweep kernel
 
 enum Sched { SCHED_JACOBI = 0, SCHED_GS_LEX = 1, SCHED_REDBLACK = 2 };
 enum Order { ORDER_UDLR = 0, ORDER_ULRD = 1 };
@@ -38,7 +39,6 @@ struct MarchArgs {
   int H, W, P;
   int t0, t_end, band;
   int stripes, segs, seg_rows, ntask;
-  unsigned* dmax;  // optional: max |change| of the last step (float bits)
 };
 
 // One composite of a batch.  The host fills the geometry; the device keeps
@@ -100,7 +100,8 @@ constexpr int tile_groups(const TileCfg& c) { return 3 / tile_nch(c); }
   hipError_t launch_batch_##s##_##c##_##n(int sched, int order,           \
                                           const BatchArgs& b, int blocks, \
                                           hipStream_t st);                \
-  int batch_blocks_per_cu_##s##_##c##_##n(int sched, int order);
+  int batch_blocks_per_cu_##s##_##c##_##n(int sched, int order);         \
+  int march_blocks_per_cu_##s##_##c##_##n(int sched, int order);
 PIE_TILES(PIE_DECL)
 #undef PIE_DECL
 
@@ -112,6 +113,9 @@ void tile_tasks(const TileCfg& cfg, int H, int W, int* stripes, int* segs);
 
 hipError_t launch_march(const TileCfg& cfg, int sched, int order,
                         MarchArgs a, hipStream_t st);
+// Resident work-groups (kSweepBlockWaves waves each) of the sweep kernel
+// per CU.
+int march_blocks_per_cu(const TileCfg& cfg, int sched, int order);
 
 // Persistent batched sweep kernel (S Jacobi sweeps of every live item) and
 // its epilogue (advance live items, apply the stop rules, reset the work
@@ -148,6 +152,14 @@ hipError_t launch_divergence(const uint8_t* src, const uint8_t* tgt,
                              const uint8_t* m, float* x, float* g,
                              long long plane, int H, int W, int P, int mode,
                              int equ, hipStream_t st);
+// Guidance for the sweep kernel: NaN at every unsolved pixel and in the
+// padding of the buffer g (allocation start; mask m likewise, both with row
+// pitch P, channel stride plane, segment length seg); solved pixels whose
+// guidance is NaN are counted into *nan_solved.
+hipError_t launch_box_guidance(float* g, const uint8_t* m, long long plane,
+                               int H, int P, int seg,
+                               unsigned long long* nan_solved,
+                               hipStream_t st);
 // fill the n elements of a three-channel buffer (row pitch P, channel
 // stride plane) with a per-channel constant
 hipError_t launch_fill3(float* dst, long long n, int P, long long plane,

@@ -552,6 +552,33 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// The sweep kernel tells solved pixels by their guidance: every guidance
+// value of an unsolved pixel or of the padding becomes NaN (a quiet NaN; a
+// solved pixel's guidance keeps its value).  n = 3 * H * seg elements, seg
+// = the padded segment length; element (c, i, jp) sits at c * plane + i * P
+// + jp of the allocation and its mask byte at i * P + jp.  *nan_solved
+// counts solved pixels whose guidance already was NaN (rejected by reset).
+__global__ void __launch_bounds__(kBlock)
+    box_kernel(float* __restrict__ g, const uint8_t* __restrict__ m,
+               long long plane, int H, int P, int seg,
+               unsigned long long* __restrict__ nan_solved) {
+  const long long n = 3LL * H * seg;
+  unsigned long long bad = 0;
+  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < n;
+       id += (long long)gridDim.x * kBlock) {
+    const long long per = (long long)H * seg;
+    const int c = (int)(id / per);
+    const long long rem = id % per;
+    const int i = (int)(rem / seg), jp = (int)(rem % seg);
+    float& v = g[c * plane + (long long)i * P + jp];
+    if (m[(long long)i * P + jp])
+      bad += v != v;
+    else
+      v = __builtin_nanf("");
+  }
+  if (bad) atomicAdd(nan_solved, bad);
+}
+
 int grid_blocks(long long n, int cap) {
   long long b = (n + kBlock - 1) / kBlock;
   if (b < 1) b = 1;
@@ -616,6 +643,16 @@ hipError_t launch_batch(const TileCfg& cfg, int sched, int order,
   return hipErrorInvalidValue;
 }
 
+int march_blocks_per_cu(const TileCfg& cfg, int sched, int order) {
+  const int N = tile_nch(cfg);
+#define PIE_T(s, c, n)                              \
+  if (cfg.S == s && cfg.C == c && N == n)           \
+    return march_blocks_per_cu_##s##_##c##_##n(sched, order);
+  PIE_TILES(PIE_T)
+#undef PIE_T
+  return 1;
+}
+
 int batch_blocks_per_cu(const TileCfg& cfg, int sched, int order) {
   const int N = tile_nch(cfg);
 #define PIE_T(s, c, n)                              \
@@ -676,6 +713,15 @@ hipError_t launch_keep_unsolved(uint8_t* img, const uint8_t* tgt,
   hipLaunchKernelGGL(keep_unsolved_kernel,
                      dim3(grid_blocks((long long)H * W, 8192)), dim3(kBlock),
                      0, st, img, tgt, m, H, W, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_box_guidance(float* g, const uint8_t* m, long long plane,
+                               int H, int P, int seg,
+                               unsigned long long* nan_solved,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(box_kernel, dim3(grid_blocks(3LL * H * seg, 8192)),
+                     dim3(kBlock), 0, st, g, m, plane, H, P, seg, nan_solved);
   return hipGetLastError();
 }
 

@@ -33,7 +33,7 @@ namespace pie {
 namespace detail {
 
 constexpr int kWave = 64;
-constexpr int kBlockWaves = 4;
+constexpr int kBlockWaves = kSweepBlockWaves;
 constexpr int kBlock = kWave * kBlockWaves;
 
 template <int I, int N, typename F>
@@ -76,14 +76,16 @@ __device__ __forceinline__ int xcd_block() {
 
 // Rows are padded (kPadLeft zero columns before column 0, at least
 // kPadRight after column W-1, see kernels.h), so a stripe's vector load at
-// col0 in [-S, W + 64C) always stays inside the row: loads need no per-lane
-// bounds test, and a prefetched value is not forced through a branch.
-// Padding and clamped rows only ever feed halo lanes or unsolved pixels.
+// columns [cu, cu + 64C) with cu >= -S always stays inside the row: loads
+// need no per-lane bounds test, and a prefetched value is not forced
+// through a branch.  Padding and clamped rows only ever feed halo lanes or
+// unsolved pixels.  `rowp` is the wave-uniform address of the stripe's
+// first column in the row (scalar registers), `lo` the lane's offset.
 template <int C, int NCH>
-__device__ __forceinline__ void load_row(const float* __restrict__ base,
-                                         long long plane, int P, int row,
-                                         int col0, float (&v)[C][NCH]) {
-  const float* p = base + (long long)row * P + col0;
+__device__ __forceinline__ void load_row(const float* __restrict__ rowp,
+                                         long long plane, int lo,
+                                         float (&v)[C][NCH]) {
+  const float* p = rowp + lo;
 #pragma unroll
   for (int ch = 0; ch < NCH; ++ch) {
     if constexpr (C == 2) {
@@ -102,29 +104,13 @@ __device__ __forceinline__ void load_row(const float* __restrict__ base,
   }
 }
 
-// Mask bits of C adjacent columns (mask bytes are 0 or 1).
-template <int C>
-__device__ __forceinline__ uint32_t load_mask(const uint8_t* __restrict__ m,
-                                              int P, int row, int col0) {
-  const uint8_t* p = m + (long long)row * P + col0;
-  if constexpr (C == 4) {
-    const uint32_t v = *reinterpret_cast<const uint32_t*>(p);
-    return (v & 1u) | ((v >> 7) & 2u) | ((v >> 14) & 4u) | ((v >> 21) & 8u);
-  } else if constexpr (C == 2) {
-    const uint32_t v = *reinterpret_cast<const uint16_t*>(p);
-    return (v & 1u) | ((v >> 7) & 2u);
-  } else {
-    return *p;
-  }
-}
-
 // Stores may land in the right padding (col >= W): those values are finite
 // and only ever read by halo lanes.
 template <int C, int NCH>
-__device__ __forceinline__ void store_row(float* __restrict__ base,
-                                          long long plane, int P, int row,
-                                          int col0, const float (&v)[C][NCH]) {
-  float* p = base + (long long)row * P + col0;
+__device__ __forceinline__ void store_row(float* __restrict__ rowp,
+                                          long long plane, int lo,
+                                          const float (&v)[C][NCH]) {
+  float* p = rowp + lo;
 #pragma unroll
   for (int ch = 0; ch < NCH; ++ch) {
     if constexpr (C == 2) {
@@ -139,21 +125,118 @@ __device__ __forceinline__ void store_row(float* __restrict__ base,
   }
 }
 
-// Row steps of look-ahead of the sweep kernel's loads (divides S, so the
-// ring slot of every unrolled step is a compile-time constant).
-#ifndef PIE_PREFETCH
-template <int S>
-constexpr int kPrefetch = S % 4 == 0 ? 4 : 2;
-#else
-template <int S>
-constexpr int kPrefetch = PIE_PREFETCH;
-#endif
+// Prefetch ring of the sweep kernel in LDS.
+//
+// Each wave streams its stripe's rows (state and guidance, every channel of
+// the task) into a private LDS ring of PF slots with gfx950's LDS-DMA loads
+// (global_load_lds_dword: each lane's dword lands at slot base + 4 * lane),
+// PF row steps ahead of their use.  Register prefetch rings did not work
+// here: the register allocator copied some just-issued loads across the
+// loop back-edge, and each copy waited for every load in flight.  LDS-DMA
+// loads have no register destination, so nothing in flight is ever copied.
+//
+// A slot is read back by one asm block that first waits until the slot's
+// loads have landed (vmcnt: all vector-memory operations issued after them
+// may still be in flight, and at least (PF - 1) * D loads were issued after
+// them, D = loads per row; stores only make the wait stricter), then reads
+// the slot with ds_read and waits for the reads (lgkmcnt(0)) before the
+// block ends, so its outputs are complete and the slot may be reloaded.
+// The compiler's own wait insertion would wait for all LDS-DMA loads in
+// flight before any LDS read it can see.
+template <int S, int C, int NCH>
+struct SweepRing {
+  // slots: a divisor of S (the slot of every unrolled step is a constant);
+  // three-channel tasks keep 4 rows in flight, one-channel tasks S rows
+  static constexpr int PF = NCH == 3 ? (S % 4 == 0 ? 4 : (S % 3 == 0 ? 3 : 2))
+                                     : S;
+  static constexpr int kParts = C == 2 ? 2 : 1;      // dword loads per row
+  static constexpr int D = 2 * NCH * kParts;          // loads per row step
+  static constexpr int kRowFloats = kWave * C;        // one channel row
+  static constexpr int kSlotFloats = 2 * NCH * kRowFloats;
+  static constexpr int kWaveFloats = PF * kSlotFloats;
+  static constexpr int kWait = (PF - 1) * D;
+  static_assert(S % PF == 0, "ring slots divide S");
+  static_assert(kWait <= 63, "vmcnt field is 6 bits");
+  static_assert(C == 1 || C == 2, "one or two columns per lane");
+};
 
-// Guidance ring of the last S rows: registers (0) or LDS (1, each wave a
-// private region, so no barrier is needed and fewer VGPRs are live).
-#ifndef PIE_GQ_LDS
-#define PIE_GQ_LDS 0
-#endif
+// LDS byte address of a __shared__ location: the low 32 bits of its flat
+// address (the shared aperture maps LDS offset o to {aperture, o}).
+__device__ __forceinline__ uint32_t lds_offset(const void* p) {
+  return (uint32_t)(uintptr_t)p;
+}
+
+// Issue the LDS-DMA loads of one row into ring slot `slot` (wave-uniform
+// LDS pointer).  Array 0 is the state, 1 the guidance; channel rows are
+// kRowFloats apart.  For two columns per lane, lane l loads columns l and
+// 64 + l of the stripe, so that columns 2l and 2l + 1 sit side by side.
+template <int S, int C, int NCH>
+__device__ __forceinline__ void ring_fill(float* slot, const float* xrow,
+                                         const float* grow, long long plane,
+                                         int lane) {
+  using R = SweepRing<S, C, NCH>;
+#pragma unroll
+  for (int arr = 0; arr < 2; ++arr)
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch)
+#pragma unroll
+      for (int part = 0; part < R::kParts; ++part) {
+        const float* src = (arr ? grow : xrow) + ch * plane + part * kWave;
+        __builtin_amdgcn_global_load_lds(
+            src + lane, slot + (arr * NCH + ch) * R::kRowFloats + part * kWave,
+            4, 0, 0);
+      }
+}
+
+// Read one ring slot (byte offset `off` from the lane's address `addr` in
+// the wave's ring) into x[C][NCH] and g[C][NCH]; WAIT = vmcnt bound.
+template <int C, int NCH, int WAIT, int OFF>
+__device__ __forceinline__ void ring_read(uint32_t addr, float (&x)[C][NCH],
+                                          float (&g)[C][NCH]) {
+  constexpr int RB = kWave * C * 4;  // bytes of one channel row
+  if constexpr (C == 1 && NCH == 3) {
+    asm volatile(
+        "s_waitcnt vmcnt(%6)\n\t"
+        "ds_read_b32 %0, %7 offset:%8\n\t"
+        "ds_read_b32 %1, %7 offset:%9\n\t"
+        "ds_read_b32 %2, %7 offset:%10\n\t"
+        "ds_read_b32 %3, %7 offset:%11\n\t"
+        "ds_read_b32 %4, %7 offset:%12\n\t"
+        "ds_read_b32 %5, %7 offset:%13\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(x[0][0]), "=&v"(x[0][1]), "=&v"(x[0][2]), "=&v"(g[0][0]),
+          "=&v"(g[0][1]), "=&v"(g[0][2])
+        : "i"(WAIT), "v"(addr), "i"(OFF), "i"(OFF + RB), "i"(OFF + 2 * RB),
+          "i"(OFF + 3 * RB), "i"(OFF + 4 * RB), "i"(OFF + 5 * RB)
+        : "memory");
+  } else if constexpr (C == 1 && NCH == 1) {
+    asm volatile(
+        "s_waitcnt vmcnt(%2)\n\t"
+        "ds_read_b32 %0, %3 offset:%4\n\t"
+        "ds_read_b32 %1, %3 offset:%5\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(x[0][0]), "=&v"(g[0][0])
+        : "i"(WAIT), "v"(addr), "i"(OFF), "i"(OFF + RB)
+        : "memory");
+  } else if constexpr (C == 2 && NCH == 1) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    f2 xv, gv;
+    asm volatile(
+        "s_waitcnt vmcnt(%2)\n\t"
+        "ds_read_b64 %0, %3 offset:%4\n\t"
+        "ds_read_b64 %1, %3 offset:%5\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(xv), "=&v"(gv)
+        : "i"(WAIT), "v"(addr), "i"(OFF), "i"(OFF + RB)
+        : "memory");
+    x[0][0] = xv.x;
+    x[1][0] = xv.y;
+    g[0][0] = gv.x;
+    g[1][0] = gv.y;
+  } else {
+    static_assert(C == 0, "no ring read for this tile");
+  }
+}
 
 template <bool B, class T>
 __device__ __forceinline__ T* pick(T* x, T* y) {
@@ -163,83 +246,77 @@ __device__ __forceinline__ T* pick(T* x, T* y) {
     return x;
 }
 
-// One wave task: column stripe `stripe`, row range `seg`, S steps.  When
-// dmax is not null the largest |change| of the last step over the pixels
-// the task writes is merged into *dmax (float bits, non-negative).
-template <int S, int C, int NCH, int SCHED, int ORDER>
-__device__ __forceinline__ void march_task(const MarchArgs& a, int stripe,
-                                           int seg, int group,
-                                           unsigned* dmax) {
-  static_assert(S % 2 == 0 && S % C == 0, "S even and a multiple of C");
-  static_assert(S * C <= 32, "mask queue is one 32-bit word");
-  static_assert(NCH == 1 || NCH == 3, "one or three channels a task");
-  // channels of this task: all three, or channel `group` only
-  const long long choff = (long long)group * NCH * a.plane;
-  const float* __restrict__ xin = a.xin + choff;
-  float* __restrict__ xout = a.xout + choff;
-  const float* __restrict__ gin = a.g + choff;
-  constexpr int U = kWave * C - 2 * S;  // written columns per stripe
-  const int lane = threadIdx.x & (kWave - 1);
-  const int col0 = stripe * U - S + lane * C;
-  const int r0 = seg * a.seg_rows;
-  const int r1 = min(r0 + a.seg_rows, a.H);
-  const bool lane_out = lane * C >= S && lane * C < kWave * C - S;
+// Geometry of one wave task (all wave-uniform except lane, lo, col0 and
+// lane_out).
+struct TaskGeom {
+  const float* xin;  // channel planes of the task
+  float* xout;
+  const float* gin;
+  long long P;       // row pitch
+  int lane, lo;      // lane, its column offset inside the stripe
+  int cu, col0;      // first column of the stripe, of the lane
+  int r0, r1;        // rows written
+  bool lane_out;     // the lane writes its columns
+};
 
-  if constexpr (SCHED == SCHED_GS_LEX) {
-    // A task none of whose written pixels meets the wavefront band during
-    // this launch only carries its rows over to the output buffer.
-    const int t_last = min(a.t0 + S, a.t_end) - 1;
-    const int c_lo = stripe * U, c_hi = min(c_lo + U, a.W) - 1;
-    if (r0 + c_lo > t_last || (r1 - 1) + c_hi <= a.t0 - a.band) {
-      for (int r = r0; r < r1; ++r) {
-        float v[C][NCH];
-        load_row<C, NCH>(xin, a.plane, a.P, r, col0, v);
-        if (lane_out) store_row<C, NCH>(xout, a.plane, a.P, r, col0, v);
-      }
-      return;
-    }
-  }
+// The rows of one wave task (see march_task).  COLOUR variants (two
+// columns per lane, gs_lex or red-black) know at compile time which of the
+// lane's two columns may change at each level: both schedules update one
+// colour of the checkerboard per step, and with an even first column the
+// colour is the same for every lane of a row.  PI is the task's parity
+// (t0 - rb - 1) & 1; with BAND the gs_lex band test 0 <= t - i - j < band
+// is made per pixel, without it the whole task lies inside the band.
+template <int S, int C, int NCH, int SCHED, int ORDER, bool DM, int PI,
+          bool BAND>
+__device__ __forceinline__ void march_rows(const MarchArgs& a,
+                                           const TaskGeom& tg, float* ring,
+                                           unsigned* dmax) {
+  constexpr bool COLOUR = SCHED != SCHED_JACOBI && C == 2;
+  const int lane = tg.lane, lo = tg.lo, cu = tg.cu, col0 = tg.col0;
+  const int r0 = tg.r0, r1 = tg.r1;
+  const bool lane_out = tg.lane_out;
+  const long long P = tg.P;
+  float* __restrict__ xout = tg.xout;
+  auto xrow = [&](int row) { return tg.xin + row * P + cu; };
+  auto grow = [&](int row) { return tg.gin + row * P + cu; };
 
   float dm = 0.0f;
+  const int t_last_live = min(a.t0 + S, a.t_end) - 1;
   const int rb = max(0, r0 - S);
   const int nsteps = ((r1 - 1 + S) - rb + 1 + S - 1) / S * S;
 
   // Level j (0..S-1) keeps two rows: wa/wb alternate as the older and the
-  // newer one, so a step renames instead of moving.  gq/mq hold the
-  // guidance and mask of the last S rows (slot = row step mod S).
+  // newer one, so a step renames instead of moving.  gq holds the guidance
+  // of the last S rows and mb their solved flags (slot = row step mod S);
+  // the flags are lane masks, held in scalar registers.  A pixel is solved
+  // where its guidance is a number: the host stores NaN as the guidance of
+  // every unsolved pixel and of the padding.
   float wa[S][C][NCH], wb[S][C][NCH];
-#if PIE_GQ_LDS
-  __shared__ float gl_all[kBlockWaves][S][NCH][kWave * C];
-  float(&gl)[S][NCH][kWave * C] =
-      gl_all[(threadIdx.x / kWave) % kBlockWaves];
-#define PIE_GQ(slot, cc, ch) gl[slot][ch][lane * C + (cc)]
-#else
   float gq[S][C][NCH];
-#define PIE_GQ(slot, cc, ch) gq[slot][cc][ch]
-#endif
-  uint32_t mq = 0;
+  bool mb[S][C];
 #pragma unroll
   for (int j = 0; j < S; ++j)
 #pragma unroll
-    for (int cc = 0; cc < C; ++cc)
+    for (int cc = 0; cc < C; ++cc) {
+      mb[j][cc] = false;
 #pragma unroll
       for (int ch = 0; ch < NCH; ++ch) {
         wa[j][cc][ch] = wb[j][cc][ch] = 0.0f;
-        PIE_GQ(j, cc, ch) = 0.0f;
+        gq[j][cc][ch] = 0.0f;
       }
+    }
 
-  // Rows are loaded PF row steps before they are used (a register ring of
-  // PF slots), so each wave keeps PF rows of loads in flight instead of
-  // waiting for every row it consumes.
-  constexpr int PF = kPrefetch<S>;
-  float px[PF][C][NCH], pg[PF][C][NCH];
-  uint32_t pm[PF];
+  // This wave's LDS ring; rows are loaded PF row steps before their use.
+  // Rows past the grid are clamped to row H - 1, which is never solved, so
+  // every load is unconditional.
+  using R = SweepRing<S, C, NCH>;
+  constexpr int PF = R::PF;
+  const uint32_t ring_addr = lds_offset(ring + lo);
   static_for<0, PF>([&](auto pc) {
     constexpr int q = decltype(pc)::value;
     const int row = min(rb + q, a.H - 1);
-    load_row<C, NCH>(xin, a.plane, a.P, row, col0, px[q]);
-    load_row<C, NCH>(gin, a.plane, a.P, row, col0, pg[q]);
-    pm[q] = rb + q < a.H ? load_mask<C>(a.m, a.P, row, col0) : 0u;
+    ring_fill<S, C, NCH>(ring + q * R::kSlotFloats, xrow(row), grow(row),
+                         a.plane, lane);
   });
 
   for (int base = 0; base < nsteps; base += S) {
@@ -248,19 +325,11 @@ __device__ __forceinline__ void march_task(const MarchArgs& a, int stripe,
       constexpr int q = u % PF;
       const int rho = rb + base + u;
       float fr[C][NCH], gn[C][NCH];
-#pragma unroll
-      for (int cc = 0; cc < C; ++cc)
-#pragma unroll
-        for (int ch = 0; ch < NCH; ++ch) {
-          fr[cc][ch] = px[q][cc][ch];
-          gn[cc][ch] = pg[q][cc][ch];
-        }
-      const uint32_t mn = pm[q];
+      ring_read<C, NCH, R::kWait, q * R::kSlotFloats * 4>(ring_addr, fr, gn);
       {
         const int row = min(rho + PF, a.H - 1);  // wave-uniform
-        load_row<C, NCH>(xin, a.plane, a.P, row, col0, px[q]);
-        load_row<C, NCH>(gin, a.plane, a.P, row, col0, pg[q]);
-        pm[q] = rho + PF < a.H ? load_mask<C>(a.m, a.P, row, col0) : 0u;
+        ring_fill<S, C, NCH>(ring + q * R::kSlotFloats, xrow(row), grow(row),
+                             a.plane, lane);
       }
 
       static_for<1, S + 1>([&](auto kc) {
@@ -276,76 +345,119 @@ __device__ __forceinline__ void march_task(const MarchArgs& a, int stripe,
         // the window registers are renamed rather than copied.
         const bool live = t < a.t_end;
         float out[C][NCH];
-        const uint32_t mrow = mq >> (slot * C);
-        bool upd[C];
+        if constexpr (COLOUR) {
+          // the column of the lane that may change at this level: t - r
+          // has the parity PI ^ (u & 1) (base and 2k are even)
+          constexpr int act = SCHED == SCHED_GS_LEX ? (PI ^ (u & 1))
+                                                    : (1 ^ PI ^ (u & 1));
+          constexpr int oth = 1 - act;
+          bool upd = live && mb[slot][act];
+          if constexpr (BAND) {
+            const int d = t - r - (col0 + act);
+            upd = upd && d >= 0 && d < a.band;
+          }
+          float acc[NCH];
 #pragma unroll
-        for (int cc = 0; cc < C; ++cc) {
-          const int col = col0 + cc;
-          upd[cc] = live && ((mrow >> cc) & 1u);
-          if constexpr (SCHED == SCHED_REDBLACK) {
-            upd[cc] = upd[cc] && ((r + col + t) & 1);
-          } else if constexpr (SCHED == SCHED_GS_LEX) {
-            const int d = t - r - col;
-            upd[cc] = upd[cc] && d >= 0 && d < a.band && !(d & 1);
+          for (int ch = 0; ch < NCH; ++ch) {
+            const float l = act == 0 ? from_left(newer[oth][ch])
+                                     : newer[oth][ch];
+            const float rr = act == 0 ? newer[oth][ch]
+                                      : from_right(newer[oth][ch]);
+            acc[ch] = gq[slot][act][ch] + older[act][ch];
+            if constexpr (ORDER == ORDER_UDLR) {
+              acc[ch] = acc[ch] + fr[act][ch];
+              acc[ch] = acc[ch] + l;
+              acc[ch] = acc[ch] + rr;
+            } else {
+              acc[ch] = acc[ch] + l;
+              acc[ch] = acc[ch] + rr;
+              acc[ch] = acc[ch] + fr[act][ch];
+            }
+            out[act][ch] = upd ? acc[ch] * 0.25f : newer[act][ch];
+            out[oth][ch] = newer[oth][ch];
+          }
+        } else {
+          bool upd[C];
+#pragma unroll
+          for (int cc = 0; cc < C; ++cc) {
+            const int col = col0 + cc;
+            upd[cc] = live && mb[slot][cc];
+            if constexpr (SCHED == SCHED_REDBLACK) {
+              upd[cc] = upd[cc] && ((r + col + t) & 1);
+            } else if constexpr (SCHED == SCHED_GS_LEX) {
+              const int d = t - r - col;
+              upd[cc] = upd[cc] && d >= 0 && d < a.band && !(d & 1);
+            }
+          }
+          // The C x NCH sums are independent chains; adding them in
+          // lockstep (one term of every chain, then the next term) keeps
+          // other work between an accumulator's write and the lane-shifted
+          // (DPP) add that reads it, which otherwise stalls on the DPP read
+          // hazard.  Each chain still adds its own terms in the reference's
+          // order.
+          float acc[C][NCH];
+          auto left = [&](int cc, int ch) {
+            return cc > 0 ? newer[cc > 0 ? cc - 1 : 0][ch]
+                          : from_left(newer[C - 1][ch]);
+          };
+          auto right = [&](int cc, int ch) {
+            return cc < C - 1 ? newer[cc < C - 1 ? cc + 1 : 0][ch]
+                              : from_right(newer[0][ch]);
+          };
+#pragma unroll
+          for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch)
+              acc[cc][ch] = gq[slot][cc][ch] + older[cc][ch];
+          if constexpr (ORDER == ORDER_UDLR) {
+#pragma unroll
+            for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+              for (int ch = 0; ch < NCH; ++ch)
+                acc[cc][ch] = acc[cc][ch] + fr[cc][ch];
+#pragma unroll
+            for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+              for (int ch = 0; ch < NCH; ++ch)
+                acc[cc][ch] = acc[cc][ch] + left(cc, ch);
+#pragma unroll
+            for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+              for (int ch = 0; ch < NCH; ++ch)
+                acc[cc][ch] = acc[cc][ch] + right(cc, ch);
+          } else {
+#pragma unroll
+            for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+              for (int ch = 0; ch < NCH; ++ch)
+                acc[cc][ch] = acc[cc][ch] + left(cc, ch);
+#pragma unroll
+            for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+              for (int ch = 0; ch < NCH; ++ch)
+                acc[cc][ch] = acc[cc][ch] + right(cc, ch);
+#pragma unroll
+            for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+              for (int ch = 0; ch < NCH; ++ch)
+                acc[cc][ch] = acc[cc][ch] + fr[cc][ch];
+          }
+#pragma unroll
+          for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch)
+              out[cc][ch] = upd[cc] ? acc[cc][ch] * 0.25f : newer[cc][ch];
+        }
+        if constexpr (DM) {
+          // the launch's last live step, on a row this task writes
+          if (t == t_last_live && r >= r0 && r < r1 && lane_out) {
+#pragma unroll
+            for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+              for (int ch = 0; ch < NCH; ++ch)
+                dm = fmaxf(dm, fabsf(out[cc][ch] - newer[cc][ch]));
           }
         }
-        // The C x NCH sums are independent chains; adding them in lockstep
-        // (one term of every chain, then the next term) keeps other work
-        // between an accumulator's write and the lane-shifted (DPP) add
-        // that reads it, which otherwise stalls on the DPP read hazard.
-        // Each chain still adds its own terms in the reference's order.
-        float acc[C][NCH];
-        auto left = [&](int cc, int ch) {
-          return cc > 0 ? newer[cc > 0 ? cc - 1 : 0][ch]
-                        : from_left(newer[C - 1][ch]);
-        };
-        auto right = [&](int cc, int ch) {
-          return cc < C - 1 ? newer[cc < C - 1 ? cc + 1 : 0][ch]
-                            : from_right(newer[0][ch]);
-        };
-#pragma unroll
-        for (int cc = 0; cc < C; ++cc)
-#pragma unroll
-          for (int ch = 0; ch < NCH; ++ch)
-            acc[cc][ch] = PIE_GQ(slot, cc, ch) + older[cc][ch];
-        if constexpr (ORDER == ORDER_UDLR) {
-#pragma unroll
-          for (int cc = 0; cc < C; ++cc)
-#pragma unroll
-            for (int ch = 0; ch < NCH; ++ch)
-              acc[cc][ch] = acc[cc][ch] + fr[cc][ch];
-#pragma unroll
-          for (int cc = 0; cc < C; ++cc)
-#pragma unroll
-            for (int ch = 0; ch < NCH; ++ch)
-              acc[cc][ch] = acc[cc][ch] + left(cc, ch);
-#pragma unroll
-          for (int cc = 0; cc < C; ++cc)
-#pragma unroll
-            for (int ch = 0; ch < NCH; ++ch)
-              acc[cc][ch] = acc[cc][ch] + right(cc, ch);
-        } else {
-#pragma unroll
-          for (int cc = 0; cc < C; ++cc)
-#pragma unroll
-            for (int ch = 0; ch < NCH; ++ch)
-              acc[cc][ch] = acc[cc][ch] + left(cc, ch);
-#pragma unroll
-          for (int cc = 0; cc < C; ++cc)
-#pragma unroll
-            for (int ch = 0; ch < NCH; ++ch)
-              acc[cc][ch] = acc[cc][ch] + right(cc, ch);
-#pragma unroll
-          for (int cc = 0; cc < C; ++cc)
-#pragma unroll
-            for (int ch = 0; ch < NCH; ++ch)
-              acc[cc][ch] = acc[cc][ch] + fr[cc][ch];
-        }
-#pragma unroll
-        for (int cc = 0; cc < C; ++cc)
-#pragma unroll
-          for (int ch = 0; ch < NCH; ++ch)
-            out[cc][ch] = upd[cc] ? acc[cc][ch] * 0.25f : newer[cc][ch];
 #pragma unroll
         for (int cc = 0; cc < C; ++cc)
 #pragma unroll
@@ -356,34 +468,103 @@ __device__ __forceinline__ void march_task(const MarchArgs& a, int stripe,
       });
 
       const int rs = rho - S;
-      if (lane_out && rs >= r0 && rs < r1) {
-        store_row<C, NCH>(xout, a.plane, a.P, rs, col0, fr);
-        if (dmax) {
-          // level S - 1 of row rs is still the newer row of that level
-          const float(&prev)[C][NCH] =
-              *pick<(u & 1) != 0>(&wb[S - 1], &wa[S - 1]);
-#pragma unroll
-          for (int cc = 0; cc < C; ++cc)
-#pragma unroll
-            for (int ch = 0; ch < NCH; ++ch)
-              dm = fmaxf(dm, fabsf(fr[cc][ch] - prev[cc][ch]));
-        }
-      }
+      if (lane_out && rs >= r0 && rs < r1)
+        store_row<C, NCH>(xout + rs * P + cu, a.plane, lo, fr);
 
-      // the guidance and mask of row rho enter slot u only now: level S of
+      // the guidance and flags of row rho enter slot u only now: level S of
       // this step still needed the row that slot held (rho - S)
 #pragma unroll
-      for (int cc = 0; cc < C; ++cc)
+      for (int cc = 0; cc < C; ++cc) {
+        mb[u][cc] = gn[cc][0] == gn[cc][0];  // false for NaN
 #pragma unroll
-        for (int ch = 0; ch < NCH; ++ch) PIE_GQ(u, cc, ch) = gn[cc][ch];
-      mq = (mq & ~(((1u << C) - 1u) << (u * C))) | (mn << (u * C));
+        for (int ch = 0; ch < NCH; ++ch) gq[u][cc][ch] = gn[cc][ch];
+      }
     });
   }
-#undef PIE_GQ
-  if (dmax) {
+  // no LDS-DMA load may still be in flight when the wave leaves the task
+  // (the batch kernel reuses the ring for its next task)
+  __builtin_amdgcn_s_waitcnt(0);
+  if constexpr (DM) {
     // differences are non-negative, so float order is unsigned-int order
     const unsigned wave_max = wave_max_u32(__float_as_uint(dm));
     if (lane == 0 && wave_max) atomicMax(dmax, wave_max);
+  }
+}
+
+// One wave task: column stripe `stripe`, row range `seg`, S steps.  With DM
+// the largest |change| made by the launch's last live step (step
+// min(t0 + S, t_end) - 1) over the pixels the task writes is merged into
+// *dmax (float bits, non-negative).
+template <int S, int C, int NCH, int SCHED, int ORDER, bool DM>
+__device__ __forceinline__ void march_task(const MarchArgs& a, int stripe,
+                                           int seg, int group,
+                                           unsigned* dmax) {
+  static_assert(S % 2 == 0 && S % C == 0, "S even and a multiple of C");
+  static_assert(NCH == 1 || NCH == 3, "one or three channels a task");
+  // channels of this task: all three, or channel `group` only
+  const long long choff = (long long)group * NCH * a.plane;
+  constexpr int U = kWave * C - 2 * S;  // written columns per stripe
+  TaskGeom tg;
+  tg.xin = a.xin + choff;
+  tg.xout = a.xout + choff;
+  tg.gin = a.g + choff;
+  tg.P = a.P;
+  tg.lane = threadIdx.x & (kWave - 1);
+  tg.lo = tg.lane * C;
+  tg.cu = stripe * U - S;  // even: U and S are even
+  tg.col0 = tg.cu + tg.lo;
+  tg.r0 = seg * a.seg_rows;
+  tg.r1 = min(tg.r0 + a.seg_rows, a.H);
+  tg.lane_out = tg.lo >= S && tg.lo < kWave * C - S;
+  const int c_lo = stripe * U, c_hi = min(c_lo + U, a.W) - 1;
+
+  if constexpr (SCHED == SCHED_GS_LEX) {
+    // A task none of whose written pixels meets the wavefront band during
+    // this launch only carries its rows over to the output buffer.
+    const int t_last = min(a.t0 + S, a.t_end) - 1;
+    if (tg.r0 + c_lo > t_last || (tg.r1 - 1) + c_hi <= a.t0 - a.band) {
+      for (int r = tg.r0; r < tg.r1; ++r) {
+        float v[C][NCH];
+        load_row<C, NCH>(tg.xin + r * tg.P + tg.cu, a.plane, tg.lo, v);
+        if (tg.lane_out)
+          store_row<C, NCH>(tg.xout + r * tg.P + tg.cu, a.plane, tg.lo, v);
+      }
+      return;
+    }
+  }
+
+  // one ring for whichever variant of the rows runs
+  using R = SweepRing<S, C, NCH>;
+  __shared__ __attribute__((aligned(16)))
+      float ring_all[kBlockWaves * R::kWaveFloats];
+  float* ring = ring_all + (int)__builtin_amdgcn_readfirstlane(
+                               (int)(threadIdx.x / kWave)) * R::kWaveFloats;
+
+  if constexpr (SCHED != SCHED_JACOBI && C == 2) {
+    const int rb = max(0, tg.r0 - S);
+    const int pi = (a.t0 - rb - 1) & 1;
+    bool band = false;
+    if constexpr (SCHED == SCHED_GS_LEX) {
+      // every pixel the task computes (rows rb .. r1 + S - 1, the stripe's
+      // 64C columns) inside the band at every step of the launch?
+      const int d_min = a.t0 - (tg.r1 + S - 1) - (tg.cu + kWave * C - 1);
+      const int d_max = a.t0 + S - 1 - rb - tg.cu;
+      band = !(d_min >= 0 && d_max < a.band);
+    }
+    if (pi) {
+      if (band)
+        march_rows<S, C, NCH, SCHED, ORDER, DM, 1, true>(a, tg, ring, dmax);
+      else
+        march_rows<S, C, NCH, SCHED, ORDER, DM, 1, false>(a, tg, ring, dmax);
+    } else {
+      if (band)
+        march_rows<S, C, NCH, SCHED, ORDER, DM, 0, true>(a, tg, ring, dmax);
+      else
+        march_rows<S, C, NCH, SCHED, ORDER, DM, 0, false>(a, tg, ring, dmax);
+    }
+  } else {
+    march_rows<S, C, NCH, SCHED, ORDER, DM, 0, SCHED == SCHED_GS_LEX>(
+        a, tg, ring, dmax);
   }
 }
 
@@ -394,13 +575,15 @@ template <int S, int C, int NCH, int SCHED, int ORDER>
 __global__ void __launch_bounds__(kBlock)
     __attribute__((amdgpu_waves_per_eu(PIE_WAVES_PER_EU)))
     march_kernel(MarchArgs a) {
-  const int task = xcd_block() * kBlockWaves + (threadIdx.x / kWave);
+  // wave-uniform task index (scalar registers for everything derived)
+  const int task = __builtin_amdgcn_readfirstlane(
+      xcd_block() * kBlockWaves + (int)(threadIdx.x / kWave));
   if (task >= a.ntask) return;  // whole wave leaves together
   // task = (channel group, segment, stripe), stripe fastest
   const int per_group = a.stripes * a.segs;
   const int group = task / per_group, rem = task % per_group;
-  march_task<S, C, NCH, SCHED, ORDER>(a, rem % a.stripes, rem / a.stripes,
-                                      group, a.dmax);
+  march_task<S, C, NCH, SCHED, ORDER, false>(a, rem % a.stripes,
+                                             rem / a.stripes, group, nullptr);
 }
 
 // Persistent sweep kernel of a batch: every wave takes tasks (tiles of any
@@ -437,8 +620,9 @@ __global__ void __launch_bounds__(kBlock) march_batch_kernel(BatchArgs b) {
     const int local = t - it.task0;
     const int per_group = it.stripes * it.segs;
     const int group = local / per_group, rem = local % per_group;
-    march_task<S, C, NCH, SCHED, ORDER>(a, rem % it.stripes, rem / it.stripes,
-                                        group, &b.items[id].dmax_bits);
+    march_task<S, C, NCH, SCHED, ORDER, true>(a, rem % it.stripes,
+                                              rem / it.stripes, group,
+                                              &b.items[id].dmax_bits);
   }
 }
 

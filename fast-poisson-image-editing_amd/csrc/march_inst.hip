@@ -33,6 +33,27 @@ hipError_t PIE_NAME(PIE_S, PIE_C, PIE_N)(int sched, int order,
   return hipErrorInvalidValue;
 }
 
+#define PIE_MNAME2(s, c, n) march_blocks_per_cu_##s##_##c##_##n
+#define PIE_MNAME(s, c, n) PIE_MNAME2(s, c, n)
+
+int PIE_MNAME(PIE_S, PIE_C, PIE_N)(int sched, int order) {
+  using namespace detail;
+  int n = 0;
+  hipError_t e = hipErrorInvalidValue;
+#define PIE_L(SC, OR)                                                       \
+  if (sched == SC && order == OR)                                          \
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(                       \
+        &n, (march_kernel<PIE_S, PIE_C, PIE_N, SC, OR>), kBlock, 0);
+  PIE_L(SCHED_JACOBI, ORDER_UDLR)
+  PIE_L(SCHED_JACOBI, ORDER_ULRD)
+  PIE_L(SCHED_GS_LEX, ORDER_UDLR)
+  PIE_L(SCHED_GS_LEX, ORDER_ULRD)
+  PIE_L(SCHED_REDBLACK, ORDER_UDLR)
+  PIE_L(SCHED_REDBLACK, ORDER_ULRD)
+#undef PIE_L
+  return e == hipSuccess && n > 0 ? n : 1;
+}
+
 #define PIE_BNAME2(s, c, n) launch_batch_##s##_##c##_##n
 #define PIE_BNAME(s, c, n) PIE_BNAME2(s, c, n)
 #define PIE_ONAME2(s, c, n) batch_blocks_per_cu_##s##_##c##_##n

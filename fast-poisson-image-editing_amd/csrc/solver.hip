@@ -43,7 +43,7 @@ int pie_fail(int code, const std::string& msg) {
 struct pie_solver {
   int kind = PIE_GRID, sched = PIE_JACOBI, order = PIE_ORDER_ULRD;
   int param_a = 0, param_b = 0;
-  int device = 0;
+  int device = 0, cus = 1;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -189,6 +189,44 @@ int alloc_grid(pie_solver* s, int H, int W) {
   return PIE_OK;
 }
 
+// Rows per wave task that make one launch of tile cfg `rounds` rounds of
+// resident waves: every resident wave slot of the chip (the kernel's
+// occupancy x CUs) takes one task per round, so no round is left partly
+// empty, and the rows per task (hence the share of recomputed halo rows)
+// is as large as that allows.
+int round_rows(const pie_solver* s, TileCfg cfg, int rounds) {
+  const int slots = pie::march_blocks_per_cu(cfg, s->sched, s->eff_order()) *
+                    s->cus * pie::kSweepBlockWaves * rounds;
+  cfg.seg_rows = s->H;
+  int stripes = 0, segs = 0;
+  pie::tile_tasks(cfg, s->H, s->W, &stripes, &segs);
+  const int columns = stripes * pie::tile_groups(cfg);
+  const int per_column = std::max(1, slots / std::max(1, columns));
+  const int rows = (s->H + per_column - 1) / per_column;
+  return std::max(rows, 2 * cfg.S);
+}
+
+// NaN guidance at the unsolved pixels and in the padding (the sweep kernel
+// tells solved pixels by it); a solved pixel whose guidance is NaN is an
+// error.  Call once the guidance and the mask are in place.
+int box_guidance(pie_solver* s) {
+  DevBuf<unsigned long long> bad;
+  PIE_HIP(bad.alloc(1));
+  PIE_HIP(hipMemsetAsync(bad.p, 0, sizeof(unsigned long long), s->stream));
+  PIE_HIP(pie::launch_box_guidance(s->g.p, s->m.p, s->plane, s->H, s->P,
+                                   pitch_of(s->W), bad.p, s->stream));
+  unsigned long long n = 0;
+  PIE_HIP(hipMemcpyAsync(&n, bad.p, sizeof(n), hipMemcpyDeviceToHost,
+                         s->stream));
+  PIE_HIP(hipStreamSynchronize(s->stream));
+  if (n) {
+    free_problem(s);
+    return fail(PIE_EINVAL, std::to_string(n) +
+                                " guidance values of solved pixels are NaN");
+  }
+  return PIE_OK;
+}
+
 int default_tile(pie_solver* s) {
   if (s->tile_fixed) return PIE_OK;
   // Below 2^22 pixels (where pie_iterate does not autotune) short row
@@ -197,13 +235,18 @@ int default_tile(pie_solver* s) {
   // task, 3.5 us with all three, 4.9 us with (8, 2, 32) and 6.5 us with
   // (8, 1, 64); a 512^2 one 1.8 us against 2.4 us; a 2048^2 one 6.3 us with
   // (8, 2, 32) against 8.4 us with (8, 1, 16) and 8.3 us with (8, 1, 64).
+  // From 2^22 pixels on the autotuner picks among all tiles; its starting
+  // point is (8, 1, 3) with one round of resident waves.
   const long long px = (long long)s->H * s->W;
-  if (px <= (1LL << 21))
+  if (px <= (1LL << 21)) {
     s->cfg = TileCfg{8, 1, 16, 1};
-  else if (px < (1LL << 22))
+  } else if (px < (1LL << 22)) {
     s->cfg = TileCfg{8, 2, 32};
-  else
-    s->cfg = TileCfg{8, 1, 64};
+  } else {
+    TileCfg c{8, 1, 64, 3};
+    c.seg_rows = round_rows(s, c, 1);
+    s->cfg = c;
+  }
   return PIE_OK;
 }
 
@@ -213,14 +256,22 @@ int autotune(pie_solver* s, int launches) {
   if (!s->grid_layout) return PIE_OK;
   int var[32][3];
   const int nv = std::min(pie::tile_variants(var, 32), 32);
-  const int rows[] = {16, 32, 64, 128, 256};
   float best = 1e30f;
   TileCfg best_cfg = s->cfg;
   int steps, band;
   schedule_steps(s, 1 << 20, &steps, &band);
-  for (int v = 0; v < nv; ++v)
-    for (int rr : rows) {
+  for (int v = 0; v < nv; ++v) {
+    TileCfg c0{var[v][0], var[v][1], 16, var[v][2]};
+    // fixed row counts, and the counts that fill one or two rounds of
+    // resident waves exactly
+    const int rows[] = {16, 32, 64, 128, 256, round_rows(s, c0, 1),
+                        round_rows(s, c0, 2)};
+    for (int i = 0; i < (int)(sizeof(rows) / sizeof(rows[0])); ++i) {
+      const int rr = rows[i];
       if (rr > 2 * s->H && rr != rows[0]) continue;
+      bool seen = false;  // a count listed twice is timed once
+      for (int k = 0; k < i; ++k) seen = seen || rows[k] == rr;
+      if (seen) continue;
       TileCfg c{var[v][0], var[v][1], rr, var[v][2]};
       // the state buffer is only read; the spare buffer takes the output.
       // t0 = H + W puts every pixel inside the wavefront band of gs_lex.
@@ -240,6 +291,7 @@ int autotune(pie_solver* s, int launches) {
         best_cfg = c;
       }
     }
+  }
   s->cfg = best_cfg;
   s->tuned = true;
   return PIE_OK;
@@ -386,6 +438,9 @@ int pie_create(int kind, int schedule, int order, int param_a, int param_b,
   s->param_a = param_a;
   s->param_b = param_b;
   hipError_t e = hipGetDevice(&s->device);
+  if (e == hipSuccess)
+    e = hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount,
+                              s->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&s->ev0);
   if (e == hipSuccess) e = hipEventCreate(&s->ev1);
@@ -489,7 +544,7 @@ int pie_reset_equ(pie_solver* s, int n, const int32_t* A, const float* X,
                                     s->g0(), s->plane, s->stream));
     PIE_HIP(hipMemcpyAsync(s->xb.p, s->xa.p, s->bufsz * sizeof(float),
                            hipMemcpyDeviceToDevice, s->stream));
-    PIE_HIP(hipStreamSynchronize(s->stream));
+    PIE_CHECK(box_guidance(s));
     s->masked = n - 1;
     PIE_CHECK(default_tile(s));
   } else {
@@ -570,7 +625,7 @@ int pie_reset_grid(pie_solver* s, int h, int w, const int32_t* mask,
   PIE_CHECK(upload(grad, px * 12, stage.p, s->stream));
   PIE_HIP(pie::launch_deinterleave(reinterpret_cast<const float*>(stage.p),
                                    s->g0(), s->plane, h, w, s->P, s->stream));
-  PIE_HIP(hipStreamSynchronize(s->stream));
+  PIE_CHECK(box_guidance(s));
   s->masked = masked;
   s->N = h * w;
   PIE_CHECK(default_tile(s));
@@ -601,7 +656,7 @@ int pie_reset_grid_images(pie_solver* s, int h, int w, const int32_t* mask,
                                  s->stream));
   PIE_HIP(hipMemcpyAsync(s->xb.p, s->xa.p, s->bufsz * sizeof(float),
                          hipMemcpyDeviceToDevice, s->stream));
-  PIE_HIP(hipStreamSynchronize(s->stream));
+  PIE_CHECK(box_guidance(s));
   s->masked = masked;
   s->N = h * w;
   PIE_CHECK(default_tile(s));
@@ -655,7 +710,7 @@ int pie_reset_equ_images(pie_solver* s, int h, int w, const int32_t* mask,
                                  s->stream));
   PIE_HIP(hipMemcpyAsync(s->xb.p, s->xa.p, s->bufsz * sizeof(float),
                          hipMemcpyDeviceToDevice, s->stream));
-  PIE_HIP(hipStreamSynchronize(s->stream));
+  PIE_CHECK(box_guidance(s));
   for (int c = 0; c < 3; ++c) s->x0[c] = 0.0f;
   s->N = n;
   s->masked = masked;

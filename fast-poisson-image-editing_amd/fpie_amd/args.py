@@ -1,5 +1,10 @@
-"""Command-line flags (fpie/args.py:10-142) plus the hip-only flags
-``--schedule`` and ``--order``."""
+"""Command-line flags.
+
+The flag names, defaults and choices are the reference's command line
+(fpie/args.py:10-142), so a user switching backends keeps the same command;
+the help text is this package's own.  ``--schedule`` and ``--order`` are
+hip-only additions.  MPI is not offered, so ``mpi_sync_interval`` is 0.
+"""
 
 from __future__ import annotations
 
@@ -10,67 +15,90 @@ import sys
 from . import __version__
 from .process import ALL_BACKEND, CPU_COUNT, DEFAULT_BACKEND
 
+# (flags, keyword arguments) in the order --help lists them
+_FLAGS = [
+    (("-v", "--version"),
+     dict(action="store_true", help="print the package version and quit")),
+    (("--check-backend",),
+     dict(action="store_true",
+          help="list the solver backends this installation can run")),
+    (("-b", "--backend"),
+     dict(type=str, choices=ALL_BACKEND or None, default=DEFAULT_BACKEND,
+          help="solver backend to run the iterations on")),
+    (("-c", "--cpu"),
+     dict(type=int, default=CPU_COUNT,
+          help="thread count of the CPU backends")),
+    (("-z", "--block-size"),
+     dict(type=int, default=1024,
+          help="threads per block of the cuda backend (kept for "
+          "compatibility; hip picks its tile itself)")),
+    (("--method",),
+     dict(type=str, choices=["equ", "grid"], default="equ",
+          help="solver layout: 'equ' numbers the masked pixels as a list "
+          "of equations, 'grid' iterates on the cropped image")),
+    (("-s", "--source"), dict(type=str, help="path of the source image")),
+    (("-m", "--mask"),
+     dict(type=str, default="",
+          help="path of the mask image; without one every source pixel "
+          "is blended")),
+    (("-t", "--target"), dict(type=str, help="path of the target image")),
+    (("-o", "--output"), dict(type=str, help="path the result is written to")),
+    (("-h0",), dict(type=int, default=0,
+                    help="row of the source image the mask's top edge "
+                    "sits on")),
+    (("-w0",), dict(type=int, default=0,
+                    help="column of the source image the mask's left edge "
+                    "sits on")),
+    (("-h1",), dict(type=int, default=0,
+                    help="row of the target image the blended region "
+                    "starts at")),
+    (("-w1",), dict(type=int, default=0,
+                    help="column of the target image the blended region "
+                    "starts at")),
+    (("-g", "--gradient"),
+     dict(type=str, choices=["max", "src", "avg"], default="max",
+          help="guidance field: the source gradient, the mean of source "
+          "and target, or whichever of the two is larger")),
+    (("-n",), dict(type=int, default=5000,
+                   help="number of solver iterations (more iterations, "
+                   "closer to the exact blend)")),
+    (("-p",), dict(type=int, default=0,
+                   help="also write an intermediate image every P "
+                   "iterations (0: only the final one)")),
+    (("--grid-x",), dict(type=int, default=8,
+                         help="tile height of the CPU grid solvers")),
+    (("--grid-y",), dict(type=int, default=8,
+                         help="tile width of the CPU grid solvers")),
+    (("--schedule",),
+     dict(type=str, default="jacobi", choices=["jacobi", "gs_lex", "redblack"],
+          help="hip update schedule: jacobi, gs_lex (the gcc backend's "
+          "Gauss-Seidel result) or redblack (openmp EquSolver)")),
+    (("--order",),
+     dict(type=str, default="ulrd", choices=["ulrd", "udlr"],
+          help="GridSolver neighbour order: ulrd (gcc/openmp) or udlr "
+          "(numpy)")),
+]
+
+
+def _parser() -> argparse.ArgumentParser:
+  p = argparse.ArgumentParser()
+  for flags, kw in _FLAGS:
+    p.add_argument(*flags, **kw)
+  return p
+
 
 def get_args(gen_type: str = "cli", argv=None) -> argparse.Namespace:
-  p = argparse.ArgumentParser()
-  p.add_argument("-v", "--version", action="store_true",
-                 help="show the version and exit")
-  p.add_argument("--check-backend", action="store_true",
-                 help="print all available backends")
-  p.add_argument("-b", "--backend", type=str, choices=ALL_BACKEND or None,
-                 default=DEFAULT_BACKEND, help="backend choice")
-  p.add_argument("-c", "--cpu", type=int, default=CPU_COUNT,
-                 help="number of CPU used")
-  p.add_argument("-z", "--block-size", type=int, default=1024,
-                 help="cuda block size (accepted; the hip tile is autotuned)")
-  p.add_argument("--method", type=str, choices=["equ", "grid"],
-                 default="equ", help="how to parallelize computation")
-  p.add_argument("-s", "--source", type=str, help="source image filename")
-  p.add_argument("-m", "--mask", type=str, default="",
-                 help="mask image filename (default is to use the whole "
-                 "source image)")
-  p.add_argument("-t", "--target", type=str, help="target image filename")
-  p.add_argument("-o", "--output", type=str, help="output image filename")
-  p.add_argument("-h0", type=int, default=0,
-                 help="mask position (height) on source image")
-  p.add_argument("-w0", type=int, default=0,
-                 help="mask position (width) on source image")
-  p.add_argument("-h1", type=int, default=0,
-                 help="mask position (height) on target image")
-  p.add_argument("-w1", type=int, default=0,
-                 help="mask position (width) on target image")
-  p.add_argument("-g", "--gradient", type=str, choices=["max", "src", "avg"],
-                 default="max", help="how to calculate gradient for PIE")
-  p.add_argument("-n", type=int, default=5000,
-                 help="how many iteration would you perfer, the more the "
-                 "better")
-  p.add_argument("-p", type=int, default=0,
-                 help="output result every P iteration")
-  p.add_argument("--grid-x", type=int, default=8,
-                 help="x axis stride for grid solver")
-  p.add_argument("--grid-y", type=int, default=8,
-                 help="y axis stride for grid solver")
-  p.add_argument("--schedule", type=str, default="jacobi",
-                 choices=["jacobi", "gs_lex", "redblack"],
-                 help="hip update schedule: jacobi, gs_lex (the gcc "
-                 "backend's Gauss-Seidel result) or redblack (openmp "
-                 "EquSolver)")
-  p.add_argument("--order", type=str, default="ulrd",
-                 choices=["ulrd", "udlr"],
-                 help="GridSolver neighbour order: ulrd (gcc/openmp) or "
-                 "udlr (numpy)")
-  args = p.parse_args(argv)
+  args = _parser().parse_args(argv)
   if args.version:
     print(__version__)
     sys.exit(0)
   if args.check_backend:
     print(ALL_BACKEND)
     sys.exit(0)
-  if not args.source or not os.path.exists(args.source):
-    print(f"Source image {args.source} not found.")
-    sys.exit(1)
-  if not args.target or not os.path.exists(args.target):
-    print(f"Target image {args.target} not found.")
-    sys.exit(1)
+  # the reference's messages for missing inputs (fpie/args.py:134-139)
+  for what, path in (("Source", args.source), ("Target", args.target)):
+    if not path or not os.path.exists(path):
+      print(f"{what} image {path} not found.")
+      sys.exit(1)
   args.mpi_sync_interval = 0
   return args

@@ -33,10 +33,13 @@ def write_image(name: str, image: np.ndarray) -> None:
 
 
 def read_images(src_name: str, mask_name: str, tgt_name: str):
-  src, tgt = read_image(src_name), read_image(tgt_name)
-  if os.path.exists(mask_name):
-    mask = read_image(mask_name)
-  else:
+  """(source, mask, target) of the command line.  A missing mask file means
+  "blend the whole source": a warning, then an all-255 mask of the source's
+  shape (the behaviour of fpie/io.py:27-39)."""
+  images = {"src": read_image(src_name), "tgt": read_image(tgt_name)}
+  if not os.path.exists(mask_name):
     warnings.warn("No mask file found, use default setting", stacklevel=2)
-    mask = np.zeros_like(src) + 255
-  return src, mask, tgt
+    images["mask"] = np.full(images["src"].shape, 255, np.uint8)
+  else:
+    images["mask"] = read_image(mask_name)
+  return images["src"], images["mask"], images["tgt"]

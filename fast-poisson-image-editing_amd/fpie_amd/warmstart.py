@@ -69,6 +69,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import multiprocessing
 import os
 import sys
 import time
@@ -281,7 +282,10 @@ def training_set(n: int, size: int = 256, render: int = 1024,
   n_rand = int(round(n * random_frac))
   M = np.zeros((n, size, size), np.uint8)
   R = np.zeros((n, size, size, 3), np.float32)
-  with ProcessPoolExecutor(max_workers=workers) as ex:
+  # spawned workers: a forked child of a process that holds a HIP runtime
+  # (this one, after any solver call) may inherit its locks mid-operation
+  ctx = multiprocessing.get_context("spawn")
+  with ProcessPoolExecutor(max_workers=workers, mp_context=ctx) as ex:
     for i, (mc, rc) in enumerate(ex.map(
         _random_job, [(size, seed0 + i) for i in range(n_rand)],
         chunksize=16)):
