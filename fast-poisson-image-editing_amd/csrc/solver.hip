@@ -229,19 +229,13 @@ int box_guidance(pie_solver* s) {
 
 int default_tile(pie_solver* s) {
   if (s->tile_fixed) return PIE_OK;
-  // Below 2^22 pixels (where pie_iterate does not autotune) short row
-  // ranges keep enough wave tasks in flight: measured on MI355X, Jacobi
-  // sweeps of a 1024^2 composite take 3.1 us with (8, 1, 16) one channel a
-  // task, 3.5 us with all three, 4.9 us with (8, 2, 32) and 6.5 us with
-  // (8, 1, 64); a 512^2 one 1.8 us against 2.4 us; a 2048^2 one 6.3 us with
-  // (8, 2, 32) against 8.4 us with (8, 1, 16) and 8.3 us with (8, 1, 64).
-  // From 2^22 pixels on the autotuner picks among all tiles; its starting
-  // point is (8, 1, 3) with one round of resident waves.
+  // Below 2^22 pixels (where pie_iterate does not autotune): the measured
+  // small-grid tiles.  From 2^22 pixels on the autotuner picks among all
+  // tiles; its starting point is (8, 1, 3) with one round of resident
+  // waves.
   const long long px = (long long)s->H * s->W;
-  if (px <= (1LL << 21)) {
-    s->cfg = TileCfg{8, 1, 16, 1};
-  } else if (px < (1LL << 22)) {
-    s->cfg = TileCfg{8, 2, 32};
+  if (px < (1LL << 22)) {
+    s->cfg = pie::host::small_grid_tile(px);
   } else {
     TileCfg c{8, 1, 64, 3};
     c.seg_rows = round_rows(s, c, 1);

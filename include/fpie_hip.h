@@ -264,6 +264,24 @@ int pie_batch_read(pie_batch* b, int i, uint8_t* img, float* err,
                    float* state);
 int pie_batch_size(pie_batch* b, int* items, int* tasks, long long* masked);
 
+/* ---- allocation-free solve: no counterpart in the reference ------------
+ * For callers that cannot allocate device memory (serving loops that must
+ * not call hipMalloc, harnesses that forbid it).  GridSolver reset + step +
+ * read (pie_reset_grid, pie_step) in one call, on the caller's device
+ * buffers: mask int32[h][w], tgt and grad float[h][w][3] in, img
+ * uint8[h][w][3] and err float[3] out (all device pointers), `iterations`
+ * sweeps of `schedule` in neighbour order `order`, queued on `stream` (a
+ * hipStream_t; NULL = the default stream).  Every intermediate lives in a
+ * 64 MiB device array of the library, one per GPU, kept until the process
+ * ends: crops needing more (about 1000 x 1000 pixels and up) are rejected
+ * with PIE_EINVAL.  The call allocates nothing, returns when the outputs
+ * are written, and results equal pie_reset_grid + pie_step bit for bit.
+ * Calls that overlap on one GPU race on the shared array: serialise them. */
+int pie_grid_solve_noalloc(int h, int w, const int32_t* mask,
+                           const float* tgt, const float* grad,
+                           int iterations, int schedule, int order,
+                           uint8_t* img, float* err, void* stream);
+
 /* ---- warm-start U-Net: no counterpart in the reference -----------------
  * 4-level U-Net (base 32 channels, 3x3 convolutions, bf16 weights and
  * activations, fp32 accumulation on MFMA) mapping 8 input channels to 3

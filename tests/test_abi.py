@@ -65,6 +65,27 @@ def test_tuning_calls_validate_before_any_device_work():
   assert lib.pie_unet_set_conv_tiles(512) == 0  # the default, restored
 
 
+@pytest.mark.skipif(not native.available(), reason="libfpie_hip.so not built")
+def test_noalloc_solve_checks_arguments_before_device_work():
+  """The allocation-free solve rejects NULL buffers, bad shapes and crops
+  beyond its scratch array on the host, before any HIP call (the pointers
+  below are never dereferenced)."""
+  import ctypes
+  lib = native.load()
+  p = ctypes.c_void_p(0x1000)
+  jac, ulrd = native.PIE_JACOBI, native.PIE_ORDER_ULRD
+  assert lib.pie_grid_solve_noalloc(64, 64, None, p, p, 10, jac, ulrd, p, p,
+                                    None) == native.PIE_EINVAL
+  assert b"NULL argument" in lib.pie_last_error()
+  assert lib.pie_grid_solve_noalloc(2, 64, p, p, p, 10, jac, ulrd, p, p,
+                                    None) == native.PIE_EINVAL
+  assert lib.pie_grid_solve_noalloc(64, 64, p, p, p, 10, 7, ulrd, p, p,
+                                    None) == native.PIE_EINVAL
+  assert lib.pie_grid_solve_noalloc(2000, 2000, p, p, p, 10, jac, ulrd, p, p,
+                                    None) == native.PIE_EINVAL
+  assert b"MiB of scratch" in lib.pie_last_error()
+
+
 def test_cli_check_backend_lists_hip():
   code = ("import sys; sys.path.insert(0, %r); "
           "from fpie_amd.cli import main; main(['--check-backend'])" % PKG)
