@@ -39,6 +39,9 @@ struct MarchArgs {
   int H, W, P;
   int t0, t_end, band;
   int stripes, segs, seg_rows, ntask;
+  // optional list of the tasks to run (ntask entries, each a task number
+  // group * stripes * segs + seg * stripes + stripe); null = every task
+  const int* tasks;
 };
 
 // One composite of a batch.  The host fills the geometry; the device keeps
@@ -81,13 +84,13 @@ struct TileCfg {
 // sweep kernel; each is built from march_inst.hip with -DPIE_S=s -DPIE_C=c
 // -DPIE_N=n.  A wave task carries all three colour channels (n = 3) or one
 // (n = 1: three task groups per tile, the channels are independent
-// problems).  Tiles of two columns per lane keep twice as many values per
-// level in registers and take one channel; one-column tiles come in both
+// problems).  Tiles of two or four columns per lane keep that many values
+// per level in registers and take one channel; one-column tiles come in both
 // forms (one channel a task: a third of the registers and three times the
 // tasks, which fills the machine on small grids).
 #define PIE_TILES(X)                                                       \
   X(4, 1, 3) X(6, 1, 3) X(8, 1, 3) X(8, 1, 1) X(4, 2, 1) X(6, 2, 1)        \
-  X(8, 2, 1) X(12, 2, 1)
+  X(8, 2, 1) X(12, 2, 1) X(4, 4, 1) X(8, 4, 1)
 // the channels per task of cfg (its N, or the default of its column count)
 constexpr int tile_nch(const TileCfg& c) {
   return c.N ? c.N : (c.C >= 2 ? 1 : 3);
@@ -113,6 +116,11 @@ void tile_tasks(const TileCfg& cfg, int H, int W, int* stripes, int* segs);
 
 hipError_t launch_march(const TileCfg& cfg, int sched, int order,
                         MarchArgs a, hipStream_t st);
+// flags[seg * stripes + stripe] = 1 if the pixels task region (stripe, seg)
+// of tile cfg writes hold a solved pixel (mask m: column-0 base, pitch P),
+// else 0; regions = stripes * segs of tile_tasks().
+hipError_t launch_region_flags(const TileCfg& cfg, const uint8_t* m, int H,
+                               int W, int P, int* flags, hipStream_t st);
 // Resident work-groups (kSweepBlockWaves waves each) of the sweep kernel
 // per CU.
 int march_blocks_per_cu(const TileCfg& cfg, int sched, int order);

@@ -579,6 +579,27 @@ __global__ void __launch_bounds__(kBlock)
   if (bad) atomicAdd(nan_solved, bad);
 }
 
+// One block per task region: does the region a task writes (rows
+// [r0, r1), columns [c0, c1) of tile geometry U, seg_rows) hold a solved
+// pixel?  Unsolved pixels never change, so a task without one has nothing
+// to do and both state buffers already agree on its pixels.
+__global__ void __launch_bounds__(kBlock)
+    region_flags_kernel(const uint8_t* __restrict__ m, int H, int W, int P,
+                        int U, int stripes, int seg_rows,
+                        int* __restrict__ flags) {
+  const int region = blockIdx.x;
+  const int stripe = region % stripes, seg = region / stripes;
+  const int r0 = seg * seg_rows, r1 = min(r0 + seg_rows, H);
+  const int c0 = stripe * U, c1 = min(c0 + U, W);
+  const int w = c1 - c0;
+  int any = 0;
+  for (long long k = threadIdx.x; k < (long long)(r1 - r0) * w && !any;
+       k += kBlock)
+    any = m[(long long)(r0 + k / w) * P + c0 + k % w] != 0;
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) flags[region] = any;
+}
+
 int grid_blocks(long long n, int cap) {
   long long b = (n + kBlock - 1) / kBlock;
   if (b < 1) b = 1;
@@ -622,7 +643,8 @@ hipError_t launch_march(const TileCfg& cfg, int sched, int order, MarchArgs a,
                         hipStream_t st) {
   tile_tasks(cfg, a.H, a.W, &a.stripes, &a.segs);
   a.seg_rows = cfg.seg_rows;
-  a.ntask = a.stripes * a.segs * tile_groups(cfg);
+  if (!a.tasks) a.ntask = a.stripes * a.segs * tile_groups(cfg);
+  if (a.ntask < 1) return hipSuccess;  // nothing to run
   const int N = tile_nch(cfg);
 #define PIE_T(s, c, n)                              \
   if (cfg.S == s && cfg.C == c && N == n)           \
@@ -641,6 +663,16 @@ hipError_t launch_batch(const TileCfg& cfg, int sched, int order,
   PIE_TILES(PIE_T)
 #undef PIE_T
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_region_flags(const TileCfg& cfg, const uint8_t* m, int H,
+                               int W, int P, int* flags, hipStream_t st) {
+  int stripes = 0, segs = 0;
+  tile_tasks(cfg, H, W, &stripes, &segs);
+  const int U = kWave * cfg.C - 2 * cfg.S;
+  hipLaunchKernelGGL(region_flags_kernel, dim3(stripes * segs), dim3(kBlock),
+                     0, st, m, H, W, P, U, stripes, cfg.seg_rows, flags);
+  return hipGetLastError();
 }
 
 int march_blocks_per_cu(const TileCfg& cfg, int sched, int order) {

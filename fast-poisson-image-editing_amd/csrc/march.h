@@ -147,9 +147,10 @@ template <int S, int C, int NCH>
 struct SweepRing {
   // slots: a divisor of S (the slot of every unrolled step is a constant);
   // three-channel tasks keep 4 rows in flight, one-channel tasks S rows
-  static constexpr int PF = NCH == 3 ? (S % 4 == 0 ? 4 : (S % 3 == 0 ? 3 : 2))
-                                     : S;
-  static constexpr int kParts = C == 2 ? 2 : 1;      // dword loads per row
+  static constexpr int PF = (NCH == 3 || C == 4)
+                                ? (S % 4 == 0 ? 4 : (S % 3 == 0 ? 3 : 2))
+                                : S;
+  static constexpr int kParts = C;                    // dword loads per row
   static constexpr int D = 2 * NCH * kParts;          // loads per row step
   static constexpr int kRowFloats = kWave * C;        // one channel row
   static constexpr int kSlotFloats = 2 * NCH * kRowFloats;
@@ -157,7 +158,7 @@ struct SweepRing {
   static constexpr int kWait = (PF - 1) * D;
   static_assert(S % PF == 0, "ring slots divide S");
   static_assert(kWait <= 63, "vmcnt field is 6 bits");
-  static_assert(C == 1 || C == 2, "one or two columns per lane");
+  static_assert(C == 1 || C == 2 || C == 4, "1, 2 or 4 columns per lane");
 };
 
 // LDS byte address of a __shared__ location: the low 32 bits of its flat
@@ -168,8 +169,9 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p) {
 
 // Issue the LDS-DMA loads of one row into ring slot `slot` (wave-uniform
 // LDS pointer).  Array 0 is the state, 1 the guidance; channel rows are
-// kRowFloats apart.  For two columns per lane, lane l loads columns l and
-// 64 + l of the stripe, so that columns 2l and 2l + 1 sit side by side.
+// kRowFloats apart.  With C columns per lane, load p of lane l fetches
+// column 64p + l of the stripe, so the stripe's columns lie in LDS in order
+// and lane l reads its columns Cl .. Cl + C - 1 with one ds_read.
 template <int S, int C, int NCH>
 __device__ __forceinline__ void ring_fill(float* slot, const float* xrow,
                                          const float* grow, long long plane,
@@ -233,6 +235,25 @@ __device__ __forceinline__ void ring_read(uint32_t addr, float (&x)[C][NCH],
     x[1][0] = xv.y;
     g[0][0] = gv.x;
     g[1][0] = gv.y;
+  } else if constexpr (C == 4 && NCH == 1) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    f4 xv, gv;
+    asm volatile(
+        "s_waitcnt vmcnt(%2)\n\t"
+        "ds_read_b128 %0, %3 offset:%4\n\t"
+        "ds_read_b128 %1, %3 offset:%5\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(xv), "=&v"(gv)
+        : "i"(WAIT), "v"(addr), "i"(OFF), "i"(OFF + RB)
+        : "memory");
+    x[0][0] = xv.x;
+    x[1][0] = xv.y;
+    x[2][0] = xv.z;
+    x[3][0] = xv.w;
+    g[0][0] = gv.x;
+    g[1][0] = gv.y;
+    g[2][0] = gv.z;
+    g[3][0] = gv.w;
   } else {
     static_assert(C == 0, "no ring read for this tile");
   }
@@ -520,9 +541,19 @@ __device__ __forceinline__ void march_task(const MarchArgs& a, int stripe,
 
   if constexpr (SCHED == SCHED_GS_LEX) {
     // A task none of whose written pixels meets the wavefront band during
-    // this launch only carries its rows over to the output buffer.
+    // this launch changes nothing.  Its rows only need to reach the output
+    // buffer where the two buffers may differ: a region the band has not
+    // reached is copied by the call's first launch (after which both
+    // buffers hold it), a region the band has left is copied by the first
+    // launch after its last update (pixel (i, j) is last updated at step
+    // i + j + band - 2).  Every later launch leaves such a task out.
     const int t_last = min(a.t0 + S, a.t_end) - 1;
-    if (tg.r0 + c_lo > t_last || (tg.r1 - 1) + c_hi <= a.t0 - a.band) {
+    const bool before = tg.r0 + c_lo > t_last;
+    const bool after = (tg.r1 - 1) + c_hi <= a.t0 - a.band;
+    if (before || after) {
+      const bool agree = before ? a.t0 >= S
+                                : (tg.r1 - 1) + c_hi <= a.t0 - S - a.band;
+      if (agree) return;
       for (int r = tg.r0; r < tg.r1; ++r) {
         float v[C][NCH];
         load_row<C, NCH>(tg.xin + r * tg.P + tg.cu, a.plane, tg.lo, v);
@@ -576,9 +607,10 @@ __global__ void __launch_bounds__(kBlock)
     __attribute__((amdgpu_waves_per_eu(PIE_WAVES_PER_EU)))
     march_kernel(MarchArgs a) {
   // wave-uniform task index (scalar registers for everything derived)
-  const int task = __builtin_amdgcn_readfirstlane(
+  const int idx = __builtin_amdgcn_readfirstlane(
       xcd_block() * kBlockWaves + (int)(threadIdx.x / kWave));
-  if (task >= a.ntask) return;  // whole wave leaves together
+  if (idx >= a.ntask) return;  // whole wave leaves together
+  const int task = a.tasks ? a.tasks[idx] : idx;
   // task = (channel group, segment, stripe), stripe fastest
   const int per_group = a.stripes * a.segs;
   const int group = task / per_group, rem = task % per_group;

@@ -14,7 +14,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <initializer_list>
+#include <map>
+#include <memory>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -82,6 +85,15 @@ struct pie_solver {
 
   TileCfg cfg{8, 1, 64};
   bool tile_fixed = false, tuned = false;
+  // Per tile (S, C, channels, rows): the tasks whose region holds a solved
+  // pixel (the others have nothing to do), and their count.
+  struct TaskList {
+    DevBuf<int> ids;
+    int count = 0;
+  };
+  std::map<std::array<int, 4>, std::unique_ptr<TaskList>> task_lists;
+  std::map<std::array<int, 4>, int> task_counts;
+  DevBuf<int> region_flags;
 
   bool timing = false;
   double t_ms = 0.0, t_updates = 0.0;
@@ -132,6 +144,9 @@ void free_problem(pie_solver* s) {
   s->N = 0;
   s->cur = 0;
   s->tuned = false;
+  s->task_lists.clear();
+  s->task_counts.clear();
+  s->region_flags.release();
 }
 
 // Total global steps of one call of `iters` sweeps and the band width.
@@ -189,21 +204,91 @@ int alloc_grid(pie_solver* s, int H, int W) {
   return PIE_OK;
 }
 
+std::array<int, 4> tile_key(const TileCfg& c) {
+  return {c.S, c.C, pie::tile_nch(c), c.seg_rows};
+}
+
+// Which task regions of tile c hold a solved pixel: flags[region] on the
+// host (region = seg * stripes + stripe).
+int region_flags(pie_solver* s, const TileCfg& c, std::vector<int>* flags) {
+  int stripes = 0, segs = 0;
+  pie::tile_tasks(c, s->H, s->W, &stripes, &segs);
+  const size_t n = (size_t)stripes * segs;
+  if (s->region_flags.n < n) PIE_HIP(s->region_flags.alloc(n));
+  PIE_HIP(pie::launch_region_flags(c, s->m0(), s->H, s->W, s->P,
+                                   s->region_flags.p, s->stream));
+  flags->assign(n, 0);
+  PIE_HIP(hipMemcpyAsync(flags->data(), s->region_flags.p, n * sizeof(int),
+                         hipMemcpyDeviceToHost, s->stream));
+  PIE_HIP(hipStreamSynchronize(s->stream));
+  return PIE_OK;
+}
+
+// Number of tasks of tile c with work (cached per tile).
+int task_count(pie_solver* s, const TileCfg& c, int* count) {
+  const auto key = tile_key(c);
+  auto it = s->task_counts.find(key);
+  if (it == s->task_counts.end()) {
+    std::vector<int> flags;
+    PIE_CHECK(region_flags(s, c, &flags));
+    int n = 0;
+    for (int f : flags) n += f != 0;
+    it = s->task_counts.emplace(key, n * pie::tile_groups(c)).first;
+  }
+  *count = it->second;
+  return PIE_OK;
+}
+
+// The list of tasks of tile c with work, on the device (cached per tile):
+// the sweep kernel runs only these.  Tasks keep their natural order, so
+// neighbouring tasks still land on one XCD.
+int task_list(pie_solver* s, const TileCfg& c, const int** ids, int* count) {
+  const auto key = tile_key(c);
+  auto it = s->task_lists.find(key);
+  if (it == s->task_lists.end()) {
+    std::vector<int> flags;
+    PIE_CHECK(region_flags(s, c, &flags));
+    const int regions = (int)flags.size();
+    std::vector<int> list;
+    for (int g = 0; g < pie::tile_groups(c); ++g)
+      for (int r = 0; r < regions; ++r)
+        if (flags[r]) list.push_back(g * regions + r);
+    std::unique_ptr<pie_solver::TaskList> tl(new pie_solver::TaskList());
+    tl->count = (int)list.size();
+    PIE_HIP(tl->ids.alloc(std::max<size_t>(list.size(), 1)));
+    if (!list.empty()) {
+      PIE_CHECK(upload(list.data(), list.size() * sizeof(int), tl->ids.p,
+                       s->stream));
+      PIE_HIP(hipStreamSynchronize(s->stream));  // list is host memory
+    }
+    s->task_counts[key] = tl->count;
+    it = s->task_lists.emplace(key, std::move(tl)).first;
+  }
+  *ids = it->second->ids.p;
+  *count = it->second->count;
+  return PIE_OK;
+}
+
 // Rows per wave task that make one launch of tile cfg `rounds` rounds of
-// resident waves: every resident wave slot of the chip (the kernel's
-// occupancy x CUs) takes one task per round, so no round is left partly
-// empty, and the rows per task (hence the share of recomputed halo rows)
-// is as large as that allows.
-int round_rows(const pie_solver* s, TileCfg cfg, int rounds) {
+// resident waves: the smallest row count whose tasks with work fit into
+// every resident wave slot of the chip (the kernel's occupancy x CUs) that
+// many times, so no round is left partly empty and the rows per task
+// (hence the share of recomputed halo rows) are as large as that allows.
+int round_rows(pie_solver* s, TileCfg cfg, int rounds, int* rows) {
   const int slots = pie::march_blocks_per_cu(cfg, s->sched, s->eff_order()) *
                     s->cus * pie::kSweepBlockWaves * rounds;
-  cfg.seg_rows = s->H;
-  int stripes = 0, segs = 0;
-  pie::tile_tasks(cfg, s->H, s->W, &stripes, &segs);
-  const int columns = stripes * pie::tile_groups(cfg);
-  const int per_column = std::max(1, slots / std::max(1, columns));
-  const int rows = (s->H + per_column - 1) / per_column;
-  return std::max(rows, 2 * cfg.S);
+  int lo = 2 * cfg.S, hi = std::max(2 * cfg.S, s->H);
+  while (lo < hi) {  // task counts fall as the rows per task grow
+    cfg.seg_rows = (lo + hi) / 2;
+    int n = 0;
+    PIE_CHECK(task_count(s, cfg, &n));
+    if (n <= slots)
+      hi = cfg.seg_rows;
+    else
+      lo = cfg.seg_rows + 1;
+  }
+  *rows = hi;
+  return PIE_OK;
 }
 
 // NaN guidance at the unsolved pixels and in the padding (the sweep kernel
@@ -238,7 +323,7 @@ int default_tile(pie_solver* s) {
     s->cfg = pie::host::small_grid_tile(px);
   } else {
     TileCfg c{8, 1, 64, 3};
-    c.seg_rows = round_rows(s, c, 1);
+    PIE_CHECK(round_rows(s, c, 1, &c.seg_rows));
     s->cfg = c;
   }
   return PIE_OK;
@@ -246,47 +331,69 @@ int default_tile(pie_solver* s) {
 
 int run_sweeps(pie_solver* s, int iters);
 
+// Time `launches` launches of tile c on the loaded problem (the state buffer
+// is only read; the spare buffer takes the output): milliseconds per sweep.
+int time_tile(pie_solver* s, const TileCfg& c, int launches, int band,
+              float* per_sweep) {
+  // t0 = H + W puts every pixel inside the wavefront band of gs_lex
+  MarchArgs a = march_args(s, s->state(), s->other(), s->H + s->W, 1 << 30,
+                           band);
+  PIE_CHECK(task_list(s, c, &a.tasks, &a.ntask));
+  PIE_HIP(pie::launch_march(c, s->sched, s->eff_order(), a, s->stream));
+  PIE_HIP(hipEventRecord(s->ev0, s->stream));
+  for (int l = 0; l < launches; ++l)
+    PIE_HIP(pie::launch_march(c, s->sched, s->eff_order(), a, s->stream));
+  PIE_HIP(hipEventRecord(s->ev1, s->stream));
+  PIE_HIP(hipEventSynchronize(s->ev1));
+  float ms = 0.0f;
+  PIE_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+  *per_sweep = ms / (launches * c.S);
+  return PIE_OK;
+}
+
+// Autotuner: every compiled tile with rows per task from a fixed list and
+// the counts that fill one or two rounds of resident waves, timed once;
+// then the fastest kBest of them timed kReps more times each (launch times
+// vary by several per cent from run to run), keeping each one's best time.
 int autotune(pie_solver* s, int launches) {
   if (!s->grid_layout) return PIE_OK;
+  constexpr int kBest = 6, kReps = 3;
   int var[32][3];
   const int nv = std::min(pie::tile_variants(var, 32), 32);
-  float best = 1e30f;
-  TileCfg best_cfg = s->cfg;
   int steps, band;
   schedule_steps(s, 1 << 20, &steps, &band);
+  std::vector<std::pair<float, TileCfg>> cand;
   for (int v = 0; v < nv; ++v) {
     TileCfg c0{var[v][0], var[v][1], 16, var[v][2]};
-    // fixed row counts, and the counts that fill one or two rounds of
-    // resident waves exactly
-    const int rows[] = {16, 32, 64, 128, 256, round_rows(s, c0, 1),
-                        round_rows(s, c0, 2)};
-    for (int i = 0; i < (int)(sizeof(rows) / sizeof(rows[0])); ++i) {
+    int r1 = 0, r2 = 0;
+    PIE_CHECK(round_rows(s, c0, 1, &r1));
+    PIE_CHECK(round_rows(s, c0, 2, &r2));
+    const int rows[] = {32, 64, 96, 128, 192, 256, 384, r1, r2};
+    const int nr = (int)(sizeof(rows) / sizeof(rows[0]));
+    for (int i = 0; i < nr; ++i) {
       const int rr = rows[i];
       if (rr > 2 * s->H && rr != rows[0]) continue;
       bool seen = false;  // a count listed twice is timed once
       for (int k = 0; k < i; ++k) seen = seen || rows[k] == rr;
       if (seen) continue;
       TileCfg c{var[v][0], var[v][1], rr, var[v][2]};
-      // the state buffer is only read; the spare buffer takes the output.
-      // t0 = H + W puts every pixel inside the wavefront band of gs_lex.
-      MarchArgs a = march_args(s, s->state(), s->other(), s->H + s->W,
-                               1 << 30, band);
-      PIE_HIP(pie::launch_march(c, s->sched, s->eff_order(), a, s->stream));
-      PIE_HIP(hipEventRecord(s->ev0, s->stream));
-      for (int l = 0; l < launches; ++l)
-        PIE_HIP(pie::launch_march(c, s->sched, s->eff_order(), a, s->stream));
-      PIE_HIP(hipEventRecord(s->ev1, s->stream));
-      PIE_HIP(hipEventSynchronize(s->ev1));
-      float ms = 0.0f;
-      PIE_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-      const float per_sweep = ms / (launches * c.S);
-      if (per_sweep < best) {
-        best = per_sweep;
-        best_cfg = c;
-      }
+      float t = 0.0f;
+      PIE_CHECK(time_tile(s, c, std::max(2, launches / 2), band, &t));
+      cand.push_back({t, c});
     }
   }
-  s->cfg = best_cfg;
+  std::sort(cand.begin(), cand.end(),
+            [](const auto& x, const auto& y) { return x.first < y.first; });
+  if ((int)cand.size() > kBest) cand.resize(kBest);
+  for (auto& c : cand)
+    for (int r = 0; r < kReps; ++r) {
+      float t = 0.0f;
+      PIE_CHECK(time_tile(s, c.second, launches, band, &t));
+      c.first = std::min(c.first, t);
+    }
+  std::sort(cand.begin(), cand.end(),
+            [](const auto& x, const auto& y) { return x.first < y.first; });
+  if (!cand.empty()) s->cfg = cand[0].second;
   s->tuned = true;
   return PIE_OK;
 }
@@ -318,8 +425,13 @@ int run_sweeps(pie_solver* s, int iters) {
   const TileCfg c = s->cfg;
   if (s->timing) PIE_HIP(hipEventRecord(s->ev0, s->stream));
   long long launches = 0;
+  const int* tasks = nullptr;
+  int ntask = 0;
+  PIE_CHECK(task_list(s, c, &tasks, &ntask));
   for (int t0 = 0; t0 < steps; t0 += c.S) {
     MarchArgs a = march_args(s, s->state(), s->other(), t0, steps, band);
+    a.tasks = tasks;
+    a.ntask = ntask;
     PIE_HIP(pie::launch_march(c, s->sched, s->eff_order(), a, s->stream));
     s->cur ^= 1;
     ++launches;
