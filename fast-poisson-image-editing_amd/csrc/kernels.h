@@ -116,11 +116,11 @@ void tile_tasks(const TileCfg& cfg, int H, int W, int* stripes, int* segs);
 
 hipError_t launch_march(const TileCfg& cfg, int sched, int order,
                         MarchArgs a, hipStream_t st);
-// flags[seg * stripes + stripe] = 1 if the pixels task region (stripe, seg)
-// of tile cfg writes hold a solved pixel (mask m: column-0 base, pitch P),
-// else 0; regions = stripes * segs of tile_tasks().
-hipError_t launch_region_flags(const TileCfg& cfg, const uint8_t* m, int H,
-                               int W, int P, int* flags, hipStream_t st);
+// occ[stripe * H + r] = 1 if row r holds a solved pixel in the columns of
+// stripe `stripe` ([stripe * U, min(stripe * U + U, W)), U the written
+// columns of a stripe), else 0 (mask m: column-0 base, row pitch P).
+hipError_t launch_stripe_rows(const uint8_t* m, int H, int W, int P, int U,
+                              int stripes, uint8_t* occ, hipStream_t st);
 // Resident work-groups (kSweepBlockWaves waves each) of the sweep kernel
 // per CU.
 int march_blocks_per_cu(const TileCfg& cfg, int sched, int order);

@@ -579,25 +579,21 @@ __global__ void __launch_bounds__(kBlock)
   if (bad) atomicAdd(nan_solved, bad);
 }
 
-// One block per task region: does the region a task writes (rows
-// [r0, r1), columns [c0, c1) of tile geometry U, seg_rows) hold a solved
-// pixel?  Unsolved pixels never change, so a task without one has nothing
-// to do and both state buffers already agree on its pixels.
+// occ[stripe * H + r] = 1 if row r of the columns stripe `stripe` writes
+// ([stripe * U, min(stripe * U + U, W))) holds a solved pixel, else 0; one
+// thread per (stripe, row).  Unsolved pixels never change, so a task whose
+// rows are all 0 has nothing to do (the host turns this into task lists).
 __global__ void __launch_bounds__(kBlock)
-    region_flags_kernel(const uint8_t* __restrict__ m, int H, int W, int P,
-                        int U, int stripes, int seg_rows,
-                        int* __restrict__ flags) {
-  const int region = blockIdx.x;
-  const int stripe = region % stripes, seg = region / stripes;
-  const int r0 = seg * seg_rows, r1 = min(r0 + seg_rows, H);
+    stripe_rows_kernel(const uint8_t* __restrict__ m, int H, int W, int P,
+                       int U, int stripes, uint8_t* __restrict__ occ) {
+  const long long id = blockIdx.x * (long long)kBlock + threadIdx.x;
+  if (id >= (long long)stripes * H) return;
+  const int stripe = (int)(id % stripes), r = (int)(id / stripes);
   const int c0 = stripe * U, c1 = min(c0 + U, W);
-  const int w = c1 - c0;
-  int any = 0;
-  for (long long k = threadIdx.x; k < (long long)(r1 - r0) * w && !any;
-       k += kBlock)
-    any = m[(long long)(r0 + k / w) * P + c0 + k % w] != 0;
-  any = __syncthreads_or(any);
-  if (threadIdx.x == 0) flags[region] = any;
+  const uint8_t* row = m + (long long)r * P;
+  uint8_t any = 0;
+  for (int c = c0; c < c1; ++c) any |= row[c];
+  occ[(long long)stripe * H + r] = any != 0;
 }
 
 int grid_blocks(long long n, int cap) {
@@ -665,13 +661,11 @@ hipError_t launch_batch(const TileCfg& cfg, int sched, int order,
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_region_flags(const TileCfg& cfg, const uint8_t* m, int H,
-                               int W, int P, int* flags, hipStream_t st) {
-  int stripes = 0, segs = 0;
-  tile_tasks(cfg, H, W, &stripes, &segs);
-  const int U = kWave * cfg.C - 2 * cfg.S;
-  hipLaunchKernelGGL(region_flags_kernel, dim3(stripes * segs), dim3(kBlock),
-                     0, st, m, H, W, P, U, stripes, cfg.seg_rows, flags);
+hipError_t launch_stripe_rows(const uint8_t* m, int H, int W, int P, int U,
+                              int stripes, uint8_t* occ, hipStream_t st) {
+  const long long n = (long long)stripes * H;
+  hipLaunchKernelGGL(stripe_rows_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, st, m, H, W, P, U, stripes, occ);
   return hipGetLastError();
 }
 

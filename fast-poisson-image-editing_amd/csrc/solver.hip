@@ -92,8 +92,9 @@ struct pie_solver {
     int count = 0;
   };
   std::map<std::array<int, 4>, std::unique_ptr<TaskList>> task_lists;
-  std::map<std::array<int, 4>, int> task_counts;
-  DevBuf<int> region_flags;
+  // per stripe width U: cum[stripe * (H + 1) + r] = rows < r of the stripe
+  // that hold a solved pixel
+  std::map<int, std::vector<int>> stripe_cum;
 
   bool timing = false;
   double t_ms = 0.0, t_updates = 0.0;
@@ -145,8 +146,7 @@ void free_problem(pie_solver* s) {
   s->cur = 0;
   s->tuned = false;
   s->task_lists.clear();
-  s->task_counts.clear();
-  s->region_flags.release();
+  s->stripe_cum.clear();
 }
 
 // Total global steps of one call of `iters` sweeps and the band width.
@@ -208,34 +208,58 @@ std::array<int, 4> tile_key(const TileCfg& c) {
   return {c.S, c.C, pie::tile_nch(c), c.seg_rows};
 }
 
-// Which task regions of tile c hold a solved pixel: flags[region] on the
-// host (region = seg * stripes + stripe).
-int region_flags(pie_solver* s, const TileCfg& c, std::vector<int>* flags) {
-  int stripes = 0, segs = 0;
-  pie::tile_tasks(c, s->H, s->W, &stripes, &segs);
-  const size_t n = (size_t)stripes * segs;
-  if (s->region_flags.n < n) PIE_HIP(s->region_flags.alloc(n));
-  PIE_HIP(pie::launch_region_flags(c, s->m0(), s->H, s->W, s->P,
-                                   s->region_flags.p, s->stream));
-  flags->assign(n, 0);
-  PIE_HIP(hipMemcpyAsync(flags->data(), s->region_flags.p, n * sizeof(int),
-                         hipMemcpyDeviceToHost, s->stream));
-  PIE_HIP(hipStreamSynchronize(s->stream));
+// Per-stripe prefix counts of the rows holding a solved pixel, for the
+// stripe width of tile c (one mask scan per width, cached).
+int stripe_cum(pie_solver* s, const TileCfg& c, const std::vector<int>** out) {
+  const int U = 64 * c.C - 2 * c.S;  // written columns of a stripe
+  auto it = s->stripe_cum.find(U);
+  if (it == s->stripe_cum.end()) {
+    const int stripes = (s->W + U - 1) / U, H = s->H;
+    DevBuf<uint8_t> occ;
+    PIE_HIP(occ.alloc((size_t)stripes * H));
+    PIE_HIP(pie::launch_stripe_rows(s->m0(), H, s->W, s->P, U, stripes, occ.p,
+                                    s->stream));
+    std::vector<uint8_t> h((size_t)stripes * H);
+    PIE_HIP(hipMemcpyAsync(h.data(), occ.p, h.size(), hipMemcpyDeviceToHost,
+                           s->stream));
+    PIE_HIP(hipStreamSynchronize(s->stream));
+    std::vector<int> cum((size_t)stripes * (H + 1), 0);
+    for (int st = 0; st < stripes; ++st)
+      for (int r = 0; r < H; ++r)
+        cum[(size_t)st * (H + 1) + r + 1] =
+            cum[(size_t)st * (H + 1) + r] + h[(size_t)st * H + r];
+    it = s->stripe_cum.emplace(U, std::move(cum)).first;
+  }
+  *out = &it->second;
   return PIE_OK;
 }
 
-// Number of tasks of tile c with work (cached per tile).
-int task_count(pie_solver* s, const TileCfg& c, int* count) {
-  const auto key = tile_key(c);
-  auto it = s->task_counts.find(key);
-  if (it == s->task_counts.end()) {
-    std::vector<int> flags;
-    PIE_CHECK(region_flags(s, c, &flags));
-    int n = 0;
-    for (int f : flags) n += f != 0;
-    it = s->task_counts.emplace(key, n * pie::tile_groups(c)).first;
+// Which task regions of tile c hold a solved pixel: flags[region] on the
+// host (region = seg * stripes + stripe).
+int region_flags(pie_solver* s, const TileCfg& c, std::vector<int>* flags) {
+  const std::vector<int>* cum = nullptr;
+  PIE_CHECK(stripe_cum(s, c, &cum));
+  int stripes = 0, segs = 0;
+  pie::tile_tasks(c, s->H, s->W, &stripes, &segs);
+  flags->assign((size_t)stripes * segs, 0);
+  const int H = s->H;
+  for (int seg = 0; seg < segs; ++seg) {
+    const int r0 = seg * c.seg_rows, r1 = std::min(r0 + c.seg_rows, H);
+    for (int st = 0; st < stripes; ++st) {
+      const int* row = cum->data() + (size_t)st * (H + 1);
+      (*flags)[(size_t)seg * stripes + st] = row[r1] > row[r0];
+    }
   }
-  *count = it->second;
+  return PIE_OK;
+}
+
+// Number of tasks of tile c with work.
+int task_count(pie_solver* s, const TileCfg& c, int* count) {
+  std::vector<int> flags;
+  PIE_CHECK(region_flags(s, c, &flags));
+  int n = 0;
+  for (int f : flags) n += f != 0;
+  *count = n * pie::tile_groups(c);
   return PIE_OK;
 }
 
@@ -261,7 +285,6 @@ int task_list(pie_solver* s, const TileCfg& c, const int** ids, int* count) {
                        s->stream));
       PIE_HIP(hipStreamSynchronize(s->stream));  // list is host memory
     }
-    s->task_counts[key] = tl->count;
     it = s->task_lists.emplace(key, std::move(tl)).first;
   }
   *ids = it->second->ids.p;
