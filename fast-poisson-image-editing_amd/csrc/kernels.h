@@ -160,6 +160,16 @@ hipError_t launch_divergence(const uint8_t* src, const uint8_t* tgt,
                              const uint8_t* m, float* x, float* g,
                              long long plane, int H, int W, int P, int mode,
                              int equ, hipStream_t st);
+// The processors' mask preparation: out[i][j] = 1 where the channel mean
+// of mask uint8[H][W][channels] is >= 128, except on the outermost rows and
+// columns; rows[i] / cols[j] (zeroed by the caller) become 1 where a row /
+// column holds a solved pixel.
+hipError_t launch_prep_mask(const uint8_t* mask, int H, int W, int channels,
+                            uint8_t* out, unsigned* rows, unsigned* cols,
+                            hipStream_t st);
+// crop int32[h][w] = m[x0 + r][y0 + c] (m uint8 with row length W)
+hipError_t launch_crop_mask(const uint8_t* m, int W, int x0, int y0, int h,
+                            int w, int32_t* crop, hipStream_t st);
 // Guidance for the sweep kernel: NaN at every unsolved pixel and in the
 // padding of the buffer g (allocation start; mask m likewise, both with row
 // pitch P, channel stride plane, segment length seg); solved pixels whose

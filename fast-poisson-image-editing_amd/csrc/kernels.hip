@@ -596,6 +596,44 @@ __global__ void __launch_bounds__(kBlock)
   occ[(long long)stripe * H + r] = any != 0;
 }
 
+// Mask preparation of the processors (fpie/process.py:209-217, 338-345):
+// a pixel is solved when the mean of its channels is >= 128 (for uint8
+// channels: their sum >= 128 * channels, the same test without rounding),
+// and the outermost rows and columns are cleared.  out[i][j] = 0/1;
+// rows[i] / cols[j] |= 1 for every solved pixel (the bounding box).
+__global__ void __launch_bounds__(kBlock)
+    prep_mask_kernel(const uint8_t* __restrict__ mask, int H, int W,
+                     int channels, uint8_t* __restrict__ out,
+                     unsigned* __restrict__ rows, unsigned* __restrict__ cols) {
+  const long long n = (long long)H * W;
+  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < n;
+       id += (long long)gridDim.x * kBlock) {
+    const int i = (int)(id / W), j = (int)(id % W);
+    int sum = 0;
+    for (int c = 0; c < channels; ++c) sum += mask[id * channels + c];
+    const bool edge = i == 0 || j == 0 || i == H - 1 || j == W - 1;
+    const uint8_t v = !edge && sum >= 128 * channels;
+    out[id] = v;
+    if (v) {
+      // one store per row / column suffices; racing stores write the same 1
+      if (!rows[i]) rows[i] = 1u;
+      if (!cols[j]) cols[j] = 1u;
+    }
+  }
+}
+
+// crop[r][c] = (int32) m[x0 + r][y0 + c] for the h x w window
+__global__ void __launch_bounds__(kBlock)
+    crop_mask_kernel(const uint8_t* __restrict__ m, int W, int x0, int y0,
+                     int h, int w, int32_t* __restrict__ crop) {
+  const long long n = (long long)h * w;
+  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < n;
+       id += (long long)gridDim.x * kBlock) {
+    const int r = (int)(id / w), c = (int)(id % w);
+    crop[id] = m[(long long)(x0 + r) * W + y0 + c];
+  }
+}
+
 int grid_blocks(long long n, int cap) {
   long long b = (n + kBlock - 1) / kBlock;
   if (b < 1) b = 1;
@@ -748,6 +786,23 @@ hipError_t launch_box_guidance(float* g, const uint8_t* m, long long plane,
                                hipStream_t st) {
   hipLaunchKernelGGL(box_kernel, dim3(grid_blocks(3LL * H * seg, 8192)),
                      dim3(kBlock), 0, st, g, m, plane, H, P, seg, nan_solved);
+  return hipGetLastError();
+}
+
+hipError_t launch_prep_mask(const uint8_t* mask, int H, int W, int channels,
+                            uint8_t* out, unsigned* rows, unsigned* cols,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(prep_mask_kernel,
+                     dim3(grid_blocks((long long)H * W, 8192)), dim3(kBlock),
+                     0, st, mask, H, W, channels, out, rows, cols);
+  return hipGetLastError();
+}
+
+hipError_t launch_crop_mask(const uint8_t* m, int W, int x0, int y0, int h,
+                            int w, int32_t* crop, hipStream_t st) {
+  hipLaunchKernelGGL(crop_mask_kernel,
+                     dim3(grid_blocks((long long)h * w, 8192)), dim3(kBlock),
+                     0, st, m, W, x0, y0, h, w, crop);
   return hipGetLastError();
 }
 

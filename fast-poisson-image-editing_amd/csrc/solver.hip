@@ -1291,6 +1291,70 @@ int pie_shape(pie_solver* s, int* h, int* w, int* n, long long* masked) {
   return PIE_OK;
 }
 
+int pie_prepare_mask(const uint8_t* mask, int h, int w, int channels,
+                     int* box, int32_t* crop, long long crop_capacity) {
+  if (!mask || !box) return fail(PIE_EINVAL, "NULL argument");
+  if (h < 1 || w < 1) return fail(PIE_EINVAL, "empty mask");
+  if (channels < 1 || channels > 4)
+    return fail(PIE_EINVAL, "channels must be 1 to 4");
+  const size_t px = (size_t)h * w;
+  hipStream_t st = nullptr;
+  PIE_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  } guard{st};
+  DevBuf<uint8_t> dmask, dm;
+  DevBuf<unsigned> rc;
+  PIE_HIP(dmask.alloc(px * channels));
+  PIE_HIP(dm.alloc(px));
+  PIE_HIP(rc.alloc((size_t)h + w));
+  PIE_CHECK(upload(mask, px * channels, dmask.p, st));
+  PIE_HIP(hipMemsetAsync(rc.p, 0, sizeof(unsigned) * ((size_t)h + w), st));
+  PIE_HIP(pie::launch_prep_mask(dmask.p, h, w, channels, dm.p, rc.p, rc.p + h,
+                                st));
+  std::vector<unsigned> any((size_t)h + w);
+  PIE_HIP(hipMemcpyAsync(any.data(), rc.p, sizeof(unsigned) * any.size(),
+                         hipMemcpyDeviceToHost, st));
+  PIE_HIP(hipStreamSynchronize(st));
+  int r_lo = -1, r_hi = -1, c_lo = -1, c_hi = -1;
+  for (int i = 0; i < h; ++i)
+    if (any[i]) {
+      if (r_lo < 0) r_lo = i;
+      r_hi = i;
+    }
+  for (int j = 0; j < w; ++j)
+    if (any[(size_t)h + j]) {
+      if (c_lo < 0) c_lo = j;
+      c_hi = j;
+    }
+  if (r_lo < 0)
+    return fail(PIE_EINVAL, "mask selects no pixel inside the image edge");
+  // fpie/process.py:219-221: the bounding box plus a 1-pixel margin
+  box[0] = r_lo - 1;
+  box[1] = r_hi + 2;
+  box[2] = c_lo - 1;
+  box[3] = c_hi + 2;
+  if (crop) {
+    const int ch = box[1] - box[0], cw = box[3] - box[2];
+    if ((long long)ch * cw > crop_capacity)
+      return fail(PIE_EINVAL, "crop buffer smaller than the " +
+                                  std::to_string(ch) + " x " +
+                                  std::to_string(cw) + " crop");
+    DevBuf<int32_t> dcrop;
+    PIE_HIP(dcrop.alloc((size_t)ch * cw));
+    PIE_HIP(pie::launch_crop_mask(dm.p, w, box[0], box[2], ch, cw, dcrop.p,
+                                  st));
+    PIE_HIP(hipMemcpyAsync(crop, dcrop.p, sizeof(int32_t) * (size_t)ch * cw,
+                           hipMemcpyDeviceToHost, st));
+    PIE_HIP(hipStreamSynchronize(st));
+  }
+  return PIE_OK;
+}
+
 int pie_device_info(char* buf, int len) {
   if (!buf || len < 1) return fail(PIE_EINVAL, "NULL buffer");
   int dev = 0;

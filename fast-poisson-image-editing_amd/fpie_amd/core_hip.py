@@ -116,10 +116,35 @@ class _Core:
 GRADIENT_MODES = {"src": 0, "avg": 1, "max": 2}
 
 
+def prepare_mask(mask: np.ndarray):
+  """The processors' mask preparation on the device (pie_prepare_mask):
+  channel mean >= 128, outermost rows/columns cleared, crop to the bounding
+  box plus a 1-px margin (fpie/process.py:209-221, 338-349).  mask: uint8
+  [h, w] or [h, w, channels].  Returns (cropped int32 mask, x0, x1, y0, y1)
+  like fpie_amd.process.prepare_mask; ValueError if nothing is solved."""
+  m = np.ascontiguousarray(mask, dtype=np.uint8)
+  if m.ndim not in (2, 3):
+    raise ValueError("mask must be 2-D or 3-D")
+  h, w = m.shape[:2]
+  channels = m.shape[2] if m.ndim == 3 else 1
+  lib = native.load()
+  box = (ctypes.c_int * 4)()
+  crop = np.empty(h * w, np.int32)
+  rc = lib.pie_prepare_mask(_ptr(m), h, w, channels, box, _ptr(crop),
+                            crop.size)
+  if rc == native.PIE_EINVAL and b"no pixel" in lib.pie_last_error():
+    raise ValueError("mask selects no pixel inside the image edge")
+  native.check(rc)
+  x0, x1, y0, y1 = (int(v) for v in box)
+  return (crop[:(x1 - x0) * (y1 - y0)].reshape(x1 - x0, y1 - y0), x0, x1, y0,
+          y1)
+
+
 class EquSolver(_Core):
   """Equation-form solver (fpie/core/base_solver.h:12-75)."""
 
   _kind = native.PIE_EQU
+  prepare_mask = staticmethod(prepare_mask)
 
   def __init__(self, block_size: int = 1024, schedule: str = "jacobi"):
     super().__init__(schedule, "udlr", block_size, 0)
@@ -203,6 +228,7 @@ class GridSolver(_Core):
   """Grid-form solver (fpie/core/base_solver.h:77-152)."""
 
   _kind = native.PIE_GRID
+  prepare_mask = staticmethod(prepare_mask)
 
   def __init__(self, grid_x: int = 8, grid_y: int = 8,
                schedule: str = "jacobi", order: str = "ulrd"):

@@ -78,6 +78,8 @@ _SIGNATURES = {
     "pie_unet_stats": ([_vp, ctypes.POINTER(ctypes.c_double),
                         ctypes.POINTER(ctypes.c_double)], _ip),
     "pie_unet_set_conv_tiles": ([_ip], _ip),
+    "pie_prepare_mask": ([_vp, _ip, _ip, _ip, _vp, _vp, ctypes.c_longlong],
+                         _ip),
     "pie_grid_solve_noalloc": ([_ip, _ip, _vp, _vp, _vp, _ip, _ip, _ip, _vp,
                                 _vp, _vp], _ip),
     "pie_batch_create": ([_ip, _ip, _ip, _ip, ctypes.POINTER(_vp)], _ip),

@@ -166,6 +166,14 @@ class BaseProcessor(ABC):
   def mixgrad(self, a: np.ndarray, b: np.ndarray) -> np.ndarray:
     return mixgrad(a, b, self.gradient)
 
+  def prepare_mask(self, mask: np.ndarray):
+    """prepare_mask on the core's device when it offers it (uint8 masks,
+    the images the command line reads), else on the host; same result."""
+    if (hasattr(self.core, "prepare_mask") and
+        np.asarray(mask).dtype == np.uint8 and np.ndim(mask) in (2, 3)):
+      return self.core.prepare_mask(mask)
+    return prepare_mask(mask)
+
   @abstractmethod
   def reset(self, src: np.ndarray, mask: np.ndarray, tgt: np.ndarray,
             mask_on_src: tuple[int, int],
@@ -201,7 +209,7 @@ class EquProcessor(BaseProcessor):
     return ids, xs.size + 1, xs[by_id], ys[by_id]
 
   def reset(self, src, mask, tgt, mask_on_src, mask_on_tgt) -> int:
-    m, x0, x1, y0, y1 = prepare_mask(mask)
+    m, x0, x1, y0, y1 = self.prepare_mask(mask)
     so = (x0 + mask_on_src[0], y0 + mask_on_src[1])
     to = (x0 + mask_on_tgt[0], y0 + mask_on_tgt[1])
     if (hasattr(self.core, "reset_images") and src.dtype == np.uint8
@@ -278,7 +286,7 @@ class GridProcessor(BaseProcessor):
     super().__init__(gradient, 0, backend, core)
 
   def reset(self, src, mask, tgt, mask_on_src, mask_on_tgt) -> int:
-    m, x0, x1, y0, y1 = prepare_mask(mask)
+    m, x0, x1, y0, y1 = self.prepare_mask(mask)
     h, w = m.shape
     s_crop = src[mask_on_src[0] + x0:mask_on_src[0] + x1,
                  mask_on_src[1] + y0:mask_on_src[1] + y1]

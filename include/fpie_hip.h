@@ -124,6 +124,17 @@ int pie_reset_equ_images(pie_solver* s, int h, int w, const int32_t* mask,
                          const uint8_t* src, const uint8_t* tgt,
                          int gradient);
 
+/* The processors' mask preparation (fpie/process.py:209-221 and 338-349),
+ * computed on the device: a pixel of mask uint8[h][w][channels] is solved
+ * when the mean of its channels is >= 128, the outermost rows and columns
+ * are cleared, and the crop is the bounding box of the solved pixels plus
+ * a 1-pixel margin.  box[4] = x0, x1, y0, y1 (the crop window [x0, x1) x
+ * [y0, y1) in mask coordinates); crop (may be NULL) receives the int32
+ * (x1 - x0) x (y1 - y0) cropped mask (0/1) if crop_capacity (int32 values)
+ * holds it.  PIE_EINVAL if no pixel is solved. */
+int pie_prepare_mask(const uint8_t* mask, int h, int w, int channels,
+                     int* box, int32_t* crop, long long crop_capacity);
+
 /* The output of EquProcessor.step (fpie/process.py:273-280: the solved
  * values clamped and truncated to uint8, fpie/core/gcc/equ.cc:82-85,
  * scattered into the target) as the h x w x 3 crop, assembled on the

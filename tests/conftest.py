@@ -24,7 +24,22 @@ def golden_cases():
   """Names of the solver fixtures written by tests/golden/make_golden.py."""
   return sorted(os.path.basename(p)[:-4]
                 for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
-                if not p.endswith("benchmark_masks.npz"))
+                if not p.endswith(("benchmark_masks.npz",
+                                   "processor_cases.npz")))
+
+
+def processor_cases():
+  """Cases recorded from the reference processor by
+  tests/golden/make_processor_golden.py: name -> dict of arrays."""
+  path = os.path.join(GOLDEN, "processor_cases.npz")
+  with np.load(path, allow_pickle=False) as f:
+    d = {k: f[k] for k in f.files}
+  out = {}
+  for name in d["names"]:
+    name = str(name)
+    out[name] = {k[len(name) + 1:]: v for k, v in d.items()
+                 if k.startswith(name + "_")}
+  return out
 
 
 def load_golden(name):
