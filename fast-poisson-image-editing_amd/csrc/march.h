@@ -235,6 +235,34 @@ __device__ __forceinline__ void ring_read(uint32_t addr, float (&x)[C][NCH],
     x[1][0] = xv.y;
     g[0][0] = gv.x;
     g[1][0] = gv.y;
+  } else if constexpr (C == 2 && NCH == 3) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    f2 x0, x1, x2, g0, g1, g2;
+    asm volatile(
+        "s_waitcnt vmcnt(%6)\n\t"
+        "ds_read_b64 %0, %7 offset:%8\n\t"
+        "ds_read_b64 %1, %7 offset:%9\n\t"
+        "ds_read_b64 %2, %7 offset:%10\n\t"
+        "ds_read_b64 %3, %7 offset:%11\n\t"
+        "ds_read_b64 %4, %7 offset:%12\n\t"
+        "ds_read_b64 %5, %7 offset:%13\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(x0), "=&v"(x1), "=&v"(x2), "=&v"(g0), "=&v"(g1), "=&v"(g2)
+        : "i"(WAIT), "v"(addr), "i"(OFF), "i"(OFF + RB), "i"(OFF + 2 * RB),
+          "i"(OFF + 3 * RB), "i"(OFF + 4 * RB), "i"(OFF + 5 * RB)
+        : "memory");
+    x[0][0] = x0.x;
+    x[1][0] = x0.y;
+    x[0][1] = x1.x;
+    x[1][1] = x1.y;
+    x[0][2] = x2.x;
+    x[1][2] = x2.y;
+    g[0][0] = g0.x;
+    g[1][0] = g0.y;
+    g[0][1] = g1.x;
+    g[1][1] = g1.y;
+    g[0][2] = g2.x;
+    g[1][2] = g2.y;
   } else if constexpr (C == 4 && NCH == 1) {
     typedef float f4 __attribute__((ext_vector_type(4)));
     f4 xv, gv;

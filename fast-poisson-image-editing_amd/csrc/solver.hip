@@ -95,6 +95,15 @@ struct pie_solver {
   // per stripe width U: cum[stripe * (H + 1) + r] = rows < r of the stripe
   // that hold a solved pixel
   std::map<int, std::vector<int>> stripe_cum;
+  // Launch sequences of step(n) calls, captured as hipGraphs and replayed
+  // when a call repeats (same sweeps, tile and starting buffer): kernel
+  // launches of small grids cost more to issue one by one than to run.
+  // Key: iters, cur, S, C, channels, rows, listed tasks.
+  struct Graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::map<std::array<int, 7>, Graph> graphs;
 
   bool timing = false;
   double t_ms = 0.0, t_updates = 0.0;
@@ -113,6 +122,14 @@ struct pie_solver {
 };
 
 namespace {
+
+void drop_graph(pie_solver* s) {
+  for (auto& kv : s->graphs) {
+    if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+    if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+  }
+  s->graphs.clear();
+}
 
 void free_problem(pie_solver* s) {
   s->loaded = false;
@@ -147,6 +164,7 @@ void free_problem(pie_solver* s) {
   s->tuned = false;
   s->task_lists.clear();
   s->stripe_cum.clear();
+  drop_graph(s);
 }
 
 // Total global steps of one call of `iters` sweeps and the band width.
@@ -354,6 +372,12 @@ int default_tile(pie_solver* s) {
 
 int run_sweeps(pie_solver* s, int iters);
 
+// step(n) calls of at least this many launches on grids below 2^22 pixels
+// replay a captured graph of their launches.
+constexpr long long kGraphMinLaunches = 16;
+// graphs kept per solver (all are dropped when one more is needed)
+constexpr size_t kMaxGraphs = 8;
+
 // Time `launches` launches of tile c on the loaded problem (the state buffer
 // is only read; the spare buffer takes the output): milliseconds per sweep.
 int time_tile(pie_solver* s, const TileCfg& c, int launches, int band,
@@ -391,7 +415,7 @@ int autotune(pie_solver* s, int launches) {
     int r1 = 0, r2 = 0;
     PIE_CHECK(round_rows(s, c0, 1, &r1));
     PIE_CHECK(round_rows(s, c0, 2, &r2));
-    const int rows[] = {32, 64, 96, 128, 192, 256, 384, r1, r2};
+    const int rows[] = {8, 16, 32, 64, 96, 128, 192, 256, 384, r1, r2};
     const int nr = (int)(sizeof(rows) / sizeof(rows[0]));
     for (int i = 0; i < nr; ++i) {
       const int rr = rows[i];
@@ -446,19 +470,63 @@ int run_sweeps(pie_solver* s, int iters) {
   int steps, band;
   schedule_steps(s, iters, &steps, &band);
   const TileCfg c = s->cfg;
-  if (s->timing) PIE_HIP(hipEventRecord(s->ev0, s->stream));
-  long long launches = 0;
   const int* tasks = nullptr;
   int ntask = 0;
   PIE_CHECK(task_list(s, c, &tasks, &ntask));
-  for (int t0 = 0; t0 < steps; t0 += c.S) {
-    MarchArgs a = march_args(s, s->state(), s->other(), t0, steps, band);
-    a.tasks = tasks;
-    a.ntask = ntask;
-    PIE_HIP(pie::launch_march(c, s->sched, s->eff_order(), a, s->stream));
-    s->cur ^= 1;
-    ++launches;
+  const long long launches = (steps + c.S - 1) / c.S;
+  // issue the launches, or capture them into a graph: the pointers and
+  // step numbers of every launch follow from (iters, cur, tile)
+  auto issue = [&](int cur0) -> int {
+    int cur = cur0;
+    for (int t0 = 0; t0 < steps; t0 += c.S) {
+      MarchArgs a = march_args(s, cur ? s->xb0() : s->xa0(),
+                               cur ? s->xa0() : s->xb0(), t0, steps, band);
+      a.tasks = tasks;
+      a.ntask = ntask;
+      PIE_HIP(pie::launch_march(c, s->sched, s->eff_order(), a, s->stream));
+      cur ^= 1;
+    }
+    return PIE_OK;
+  };
+  const bool use_graph = launches >= kGraphMinLaunches &&
+                         (long long)s->H * s->W < (1LL << 22);
+  hipGraphExec_t exec = nullptr;
+  if (use_graph) {
+    const std::array<int, 7> key{iters, s->cur, c.S, c.C, pie::tile_nch(c),
+                                 c.seg_rows, ntask};
+    auto it = s->graphs.find(key);
+    if (it == s->graphs.end()) {
+      if (s->graphs.size() >= kMaxGraphs) drop_graph(s);
+      PIE_HIP(hipStreamBeginCapture(s->stream,
+                                    hipStreamCaptureModeThreadLocal));
+      const int rc = issue(s->cur);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(s->stream, &g);
+      if (rc != PIE_OK || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc != PIE_OK ? rc
+                            : fail(PIE_EHIP, std::string("sweep graph: ") +
+                                                 hipGetErrorString(e));
+      }
+      pie_solver::Graph entry;
+      entry.graph = g;
+      const hipError_t ei =
+          hipGraphInstantiate(&entry.exec, g, nullptr, nullptr, 0);
+      if (ei != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(PIE_EHIP, std::string("sweep graph instantiate: ") +
+                                  hipGetErrorString(ei));
+      }
+      it = s->graphs.emplace(key, entry).first;
+    }
+    exec = it->second.exec;
   }
+  if (s->timing) PIE_HIP(hipEventRecord(s->ev0, s->stream));
+  if (use_graph)
+    PIE_HIP(hipGraphLaunch(exec, s->stream));
+  else
+    PIE_CHECK(issue(s->cur));
+  s->cur ^= (int)(launches & 1);
   if (s->timing) {
     PIE_HIP(hipEventRecord(s->ev1, s->stream));
     PIE_HIP(hipEventSynchronize(s->ev1));
@@ -858,8 +926,11 @@ int pie_iterate(pie_solver* s, int iterations) {
   if (!s) return fail(PIE_EINVAL, "NULL solver");
   if (!s->loaded) return fail(PIE_ESTATE, "step before reset");
   if (iterations < 0) return fail(PIE_EINVAL, "negative iteration count");
+  // autotune once per problem: at once from 2^22 pixels on, and on smaller
+  // grids for the first call long enough to repay it (2^29 pixel sweeps)
+  const long long px = (long long)s->H * s->W;
   if (s->grid_layout && !s->tuned && !s->tile_fixed &&
-      (long long)s->H * s->W >= (1LL << 22))
+      (px >= (1LL << 22) || (long long)iterations * px >= (1LL << 29)))
     PIE_CHECK(autotune(s, 6));
   return run_sweeps(s, iterations);
 }
