@@ -84,6 +84,9 @@ def bench_batch(a):
     _, m, t, g = p.core.a
     b.add(m, t, g, max_sweeps=w.iters, tol=w.tol)
     masked.append(int(np.count_nonzero(m)))
+  if not a.tile:
+    b.autotune(6)
+  tile = list(b.tile())
   size = b.size()
   print(f"# batch of {size['items']} built in "
         f"{time.perf_counter() - t_in:.1f} s: {size['tasks']} tiles, "
@@ -111,7 +114,10 @@ def bench_batch(a):
       "config": {"workload": "batch256", "items": len(items),
                  "solved_pixels": size["masked"], "tiles": size["tasks"],
                  "max_sweeps": 5000, "tol_max_abs_update": 1e-3,
-                 "tile": tile, "rounds_per_graph": 16},
+                 "tile": {"sweeps_per_launch": tile[0],
+                          "cols_per_lane": tile[1], "rows_per_task": tile[2],
+                          "channels_per_task": tile[3]},
+                 "rounds_per_graph": 16},
       "latency_p50_ms": float(np.percentile(lats, 50) * 1e3),
       "latency_p99_ms": float(np.percentile(lats, 99) * 1e3),
       "pixel_updates_per_s": upd / total,

@@ -2,15 +2,23 @@
 // and mask, each with its own stop rule, solved together.
 //
 // Every item keeps its own padded planes (same layout as pie_solver).  One
-// round is two kernels: the persistent sweep kernel, whose waves take tiles
-// of any live item from one atomic work counter and advance them by S
-// Jacobi sweeps, and an epilogue that applies the stop rules (last sweep's
-// max |update| below the item's tolerance, or its sweep budget used) and
-// resets the counter.  R rounds are captured once into a hipGraph and the
-// graph is replayed until no item is live, so the host only looks at one
-// counter every R rounds.
+// round is two kernels: the persistent sweep kernel and an epilogue.  In
+// the sweep kernel, each small ("solo") item -- few tasks, or a whole solve
+// of at most half of one work-group's even share of the batch's work -- is
+// taken by one work-group that runs launch after launch of it (S Jacobi
+// sweeps each, the stop rules after each) while the shared items' tasks
+// last, or to its end once no shared item is live; the waves then take tiles
+// of any live shared item from one atomic work counter and advance them by
+// S sweeps.  The epilogue applies the stop rules to the shared items (last
+// sweep's max |update| below the item's tolerance, or its sweep budget
+// used) and resets the counters.  Small items therefore progress several
+// launches a round and finish early, while large ones share the machine.
+// R rounds are captured once into a hipGraph and the graph is replayed
+// until no item is live, so the host only looks at one counter every R
+// rounds.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <memory>
@@ -28,6 +36,15 @@ using pie::host::pitch_of;
 using pie::host::upload;
 
 namespace {
+
+// Items of at most this many wave tasks are always solved by one
+// work-group (its waves take them in four passes a launch).
+constexpr int kSoloTasks = 4 * pie::kBatchGroupWaves;
+// Larger items are too when their whole solve (tasks x launches of their
+// sweep budget) is at most this fraction of one work-group's even share of
+// the batch: they then finish well before the batch does, in order of
+// size, while the large items still spread over the machine at the end.
+constexpr double kSoloShare = 0.5;
 
 struct ItemBuffers {
   DevBuf<float> xa, xb, g, x0;  // x0: initial state, for restarts
@@ -50,8 +67,12 @@ struct pie_batch {
   std::vector<std::unique_ptr<ItemBuffers>> items;
   std::vector<BatchItem> desc;  // descriptors as last read back
   DevBuf<BatchItem> d_items;
-  DevBuf<int> d_task_item;
-  DevBuf<unsigned> d_counters;  // [0] work queue, [1] live items
+  DevBuf<int> d_task_item, d_solo_items;
+  // [0] shared-task queue, [1] live items, [2] solo-item queue,
+  // [3] live shared items
+  DevBuf<unsigned> d_counters;
+  int nsolo = 0;
+  unsigned nshared = 0;  // shared (not solo) items
   DevBuf<unsigned long long> d_clock;
   unsigned* h_live = nullptr;   // pinned
   int ntask = 0;
@@ -82,6 +103,23 @@ int prepare(pie_batch* b) {
   const int n = (int)b->items.size();
   b->desc.assign(n, BatchItem{});
   std::vector<int> task_item;
+  std::vector<std::pair<long long, int>> solo;  // (work, item)
+  int per_cu = pie::batch_blocks_per_cu(b->cfg, PIE_JACOBI, b->order);
+  hipDeviceProp_t prop;
+  PIE_HIP(hipGetDeviceProperties(&prop, b->device));
+  b->blocks = std::max(1, per_cu * prop.multiProcessorCount);
+  // work of an item: wave tasks x launches of its sweep budget
+  std::vector<long long> work(n);
+  double total = 0.0;
+  for (int i = 0; i < n; ++i) {
+    int stripes = 0, segs = 0;
+    pie::tile_tasks(b->cfg, b->items[i]->H, b->items[i]->W, &stripes, &segs);
+    const long long launches =
+        (b->items[i]->max_sweeps + b->cfg.S - 1) / b->cfg.S;
+    work[i] = (long long)stripes * segs * pie::tile_groups(b->cfg) * launches;
+    total += (double)work[i];
+  }
+  const double solo_work = kSoloShare * total / b->blocks;
   for (int i = 0; i < n; ++i) {
     ItemBuffers& it = *b->items[i];
     BatchItem& d = b->desc[i];
@@ -98,21 +136,32 @@ int prepare(pie_batch* b) {
     d.t_end = it.max_sweeps;
     d.max_sweeps = it.max_sweeps;
     d.tol = it.tol;
-    task_item.insert(task_item.end(),
-                     (size_t)d.stripes * d.segs *
-                         pie::tile_groups(b->cfg),
-                     i);
+    const int tasks = d.stripes * d.segs * pie::tile_groups(b->cfg);
+    d.solo = tasks <= kSoloTasks || (double)work[i] <= solo_work;
+    if (d.solo) {
+      solo.push_back({work[i], i});
+    } else {
+      task_item.insert(task_item.end(), (size_t)tasks, i);
+    }
   }
+  // smallest first: the first claims of a round run longest
+  std::sort(solo.begin(), solo.end());
+  std::vector<int> solo_items;
+  for (const auto& sp : solo) solo_items.push_back(sp.second);
+  b->nsolo = (int)solo_items.size();
+  b->nshared = (unsigned)(n - b->nsolo);
   b->ntask = (int)task_item.size();
   PIE_HIP(b->d_items.alloc(std::max(n, 1)));
   PIE_HIP(b->d_task_item.alloc(std::max(b->ntask, 1)));
+  PIE_HIP(b->d_solo_items.alloc(std::max(b->nsolo, 1)));
   if (b->ntask)
     PIE_CHECK(upload(task_item.data(), sizeof(int) * task_item.size(),
                      b->d_task_item.p, b->stream));
-  int per_cu = pie::batch_blocks_per_cu(b->cfg, PIE_JACOBI, b->order);
-  hipDeviceProp_t prop;
-  PIE_HIP(hipGetDeviceProperties(&prop, b->device));
-  b->blocks = std::max(1, per_cu * prop.multiProcessorCount);
+  if (b->nsolo)
+    PIE_CHECK(upload(solo_items.data(), sizeof(int) * solo_items.size(),
+                     b->d_solo_items.p, b->stream));
+  // the uploads read host vectors: finish them before they go away
+  PIE_HIP(hipStreamSynchronize(b->stream));
   drop_graph(b);
   b->dirty = false;
   return PIE_OK;
@@ -133,7 +182,9 @@ int restart(pie_batch* b) {
   if (!b->desc.empty())
     PIE_CHECK(upload(b->desc.data(), sizeof(BatchItem) * b->desc.size(),
                      b->d_items.p, b->stream));
-  PIE_HIP(hipMemsetAsync(b->d_counters.p, 0, 2 * sizeof(unsigned), b->stream));
+  PIE_HIP(hipMemsetAsync(b->d_counters.p, 0, 3 * sizeof(unsigned), b->stream));
+  PIE_CHECK(upload(&b->nshared, sizeof(unsigned), b->d_counters.p + 3,
+                   b->stream));
   return PIE_OK;
 }
 
@@ -144,6 +195,10 @@ int capture(pie_batch* b, int rounds) {
   a.items = b->d_items.p;
   a.task_item = b->d_task_item.p;
   a.queue = b->d_counters.p;
+  a.solo_queue = b->d_counters.p + 2;
+  a.solo_items = b->d_solo_items.p;
+  a.shared_live = b->d_counters.p + 3;
+  a.nsolo = b->nsolo;
   a.ntask = b->ntask;
   a.seg_rows = b->cfg.seg_rows;
   PIE_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -154,7 +209,9 @@ int capture(pie_batch* b, int rounds) {
     if (e == hipSuccess)
       e = pie::launch_batch_epilogue(b->d_items.p, (int)b->items.size(),
                                      b->cfg.S, b->d_counters.p,
-                                     b->d_counters.p + 1, b->stream);
+                                     b->d_counters.p + 2,
+                                     b->d_counters.p + 1,
+                                     b->d_counters.p + 3, b->stream);
   }
   hipGraph_t g = nullptr;
   hipError_t e2 = hipStreamEndCapture(b->stream, &g);
@@ -196,7 +253,7 @@ int pie_batch_create(int order, int sweeps_per_launch, int cols_per_lane,
                             b->device) == hipSuccess && khz > 0)
     b->wall_hz = khz * 1e3;
   PIE_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-  PIE_HIP(b->d_counters.alloc(2));
+  PIE_HIP(b->d_counters.alloc(4));
   PIE_HIP(b->d_clock.alloc(1));
   PIE_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->h_live), sizeof(unsigned),
                         hipHostMallocDefault));
@@ -307,6 +364,89 @@ int pie_batch_add(pie_batch* b, int h, int w, const int32_t* mask,
   if (index) *index = (int)b->items.size();
   b->items.push_back(std::move(it));
   b->dirty = true;
+  return PIE_OK;
+}
+
+int pie_batch_autotune(pie_batch* b, int rounds) {
+  if (!b) return fail(PIE_EINVAL, "NULL batch");
+  if (b->items.empty()) return fail(PIE_ESTATE, "empty batch");
+  if (rounds < 1) rounds = 1;
+  int var[32][3];
+  const int nv = std::min(pie::tile_variants(var, 32), 32);
+  const int rows[] = {32, 64, 128, 256};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  PIE_HIP(hipEventCreate(&e0));
+  PIE_HIP(hipEventCreate(&e1));
+  struct Events {
+    hipEvent_t a, b;
+    ~Events() {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  } guard{e0, e1};
+  const TileCfg start = b->cfg;
+  TileCfg best = start;
+  float best_ms = 1e30f;
+  for (int v = 0; v < nv; ++v) {
+    // four columns a lane would spill in the 16-wave batch work-groups
+    if (var[v][1] > 2) continue;
+    for (int rr : rows) {
+      b->cfg = TileCfg{var[v][0], var[v][1], rr, var[v][2]};
+      b->dirty = true;
+      PIE_CHECK(prepare(b));
+      PIE_CHECK(restart(b));
+      BatchArgs a{};
+      a.items = b->d_items.p;
+      a.task_item = b->d_task_item.p;
+      a.queue = b->d_counters.p;
+      a.solo_queue = b->d_counters.p + 2;
+      a.solo_items = b->d_solo_items.p;
+      a.shared_live = b->d_counters.p + 3;
+      a.nsolo = b->nsolo;
+      a.ntask = b->ntask;
+      a.seg_rows = b->cfg.seg_rows;
+      PIE_HIP(hipEventRecord(e0, b->stream));
+      for (int r = 0; r < rounds; ++r) {
+        PIE_HIP(pie::launch_batch(b->cfg, PIE_JACOBI, b->order, a, b->blocks,
+                                  b->stream));
+        PIE_HIP(pie::launch_batch_epilogue(
+            b->d_items.p, (int)b->items.size(), b->cfg.S, b->d_counters.p,
+            b->d_counters.p + 2, b->d_counters.p + 1, b->d_counters.p + 3,
+            b->stream));
+      }
+      PIE_HIP(hipEventRecord(e1, b->stream));
+      PIE_HIP(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      PIE_HIP(hipEventElapsedTime(&ms, e0, e1));
+      // solved-pixel sweeps done (solo items run a tile-dependent number
+      // of launches a round): time per pixel sweep
+      PIE_HIP(hipMemcpyAsync(b->desc.data(), b->d_items.p,
+                             sizeof(BatchItem) * b->desc.size(),
+                             hipMemcpyDeviceToHost, b->stream));
+      PIE_HIP(hipStreamSynchronize(b->stream));
+      double done = 0.0;
+      for (size_t i = 0; i < b->desc.size(); ++i)
+        done += (double)b->desc[i].sweeps * (double)b->items[i]->masked;
+      const float per_sweep = done > 0.0 ? (float)(ms / done) : 1e30f;
+      if (per_sweep < best_ms) {
+        best_ms = per_sweep;
+        best = b->cfg;
+      }
+    }
+  }
+  b->cfg = best;
+  b->dirty = true;
+  drop_graph(b);
+  return PIE_OK;
+}
+
+int pie_batch_tile(pie_batch* b, int* sweeps, int* cols_per_lane,
+                   int* seg_rows, int* channels) {
+  if (!b) return fail(PIE_EINVAL, "NULL batch");
+  if (sweeps) *sweeps = b->cfg.S;
+  if (cols_per_lane) *cols_per_lane = b->cfg.C;
+  if (seg_rows) *seg_rows = b->cfg.seg_rows;
+  if (channels) *channels = pie::tile_nch(b->cfg);
   return PIE_OK;
 }
 

@@ -22,14 +22,16 @@ namespace pie {
 constexpr int kPadLeft = 64;
 constexpr int kPadRight = 256;  // >= 64 * (largest columns per lane)
 constexpr int kSweepBlockWaves = 4;  // waves per work-group of the sweep kernel
+constexpr int kBatchGroupWaves = 16;  // ... and of the persistent batch kernel
 
 enum Sched { SCHED_JACOBI = 0, SCHED_GS_LEX = 1, SCHED_REDBLACK = 2 };
 enum Order { ORDER_UDLR = 0, ORDER_ULRD = 1 };
 
 // One launch of the temporally blocked sweep kernel: levels 1..S of the
-// launch are global steps t0 .. t0+S-1; steps >= t_end leave the state as
-// it is.  For SCHED_GS_LEX a pixel (i, j) is updated at step t when
-// d = t - i - j is even and 0 <= d < band (band = 2 * sweeps of the call).
+// launch are global steps t0 .. t0+S-1; steps before t_lo or from t_end on
+// leave the state as it is.  For SCHED_GS_LEX a pixel (i, j) is updated at
+// step t when d = t - i - j is even and 0 <= d < band (band = 2 * sweeps of
+// the call).
 struct MarchArgs {
   const float* xin;
   float* xout;
@@ -38,6 +40,7 @@ struct MarchArgs {
   long long plane;  // elements per plane (H * P)
   int H, W, P;
   int t0, t_end, band;
+  int t_lo;         // first live step (t0, or later in an end-aligned launch)
   int stripes, segs, seg_rows, ntask;
   // optional list of the tasks to run (ntask entries, each a task number
   // group * stripes * segs + seg * stripes + stripe); null = every task
@@ -63,13 +66,18 @@ struct BatchItem {
   unsigned dmax_bits;        // max |update| of the launch in flight (bits)
   float last_dmax;           // max |update| of the last finished launch
   unsigned long long t_done; // device wall clock when the item stopped
+  int solo;                  // 1: advanced by one work-group inside the
+                             // sweep kernel, not by the epilogue
 };
 
 struct BatchArgs {
   BatchItem* items;
-  const int* task_item;  // item of every global task
-  unsigned* queue;       // work counter, reset by the epilogue
-  int ntask, seg_rows;
+  const int* task_item;   // item of every shared task
+  unsigned* queue;        // shared-task counter, reset by the epilogue
+  unsigned* solo_queue;   // solo-item counter, reset by the epilogue
+  const int* solo_items;  // the solo items, smallest first
+  const unsigned* shared_live;  // live shared items, kept by the epilogue
+  int ntask, seg_rows, nsolo;
 };
 
 // Temporal-blocking tile: S sweeps per launch, C columns per lane
@@ -125,14 +133,16 @@ hipError_t launch_stripe_rows(const uint8_t* m, int H, int W, int P, int U,
 // per CU.
 int march_blocks_per_cu(const TileCfg& cfg, int sched, int order);
 
-// Persistent batched sweep kernel (S Jacobi sweeps of every live item) and
-// its epilogue (advance live items, apply the stop rules, reset the work
-// counter, count live items into *live).
+// Persistent batched sweep kernel (a round: S Jacobi sweeps of every live
+// shared item, several launches of solo items) and its epilogue (advance
+// the live shared items, apply the stop rules, reset the work counters,
+// count live items into *live).
 hipError_t launch_batch(const TileCfg& cfg, int sched, int order,
                         const BatchArgs& b, int blocks, hipStream_t st);
 int batch_blocks_per_cu(const TileCfg& cfg, int sched, int order);
 hipError_t launch_batch_epilogue(BatchItem* items, int n, int S,
-                                 unsigned* queue, unsigned* live,
+                                 unsigned* queue, unsigned* solo_queue,
+                                 unsigned* live, unsigned* shared_live,
                                  hipStream_t st);
 // *out = device wall clock (for per-item latencies)
 hipError_t launch_clock(unsigned long long* out, hipStream_t st);

@@ -335,34 +335,34 @@ __global__ void __launch_bounds__(kBlock)
 // budget is used up; count the live items and reset the work counter.
 __global__ void __launch_bounds__(kBlock)
     batch_epilogue_kernel(BatchItem* items, int n, int S, unsigned* queue,
-                          unsigned* live) {
-  __shared__ unsigned cnt[kBlock];
+                          unsigned* solo_queue, unsigned* live,
+                          unsigned* shared_live) {
+  __shared__ unsigned cnt[2][kBlock];
   const unsigned long long now = wall_clock64();
-  unsigned c = 0;
+  unsigned c = 0, cs = 0;
   for (int i = threadIdx.x; i < n; i += kBlock) {
     BatchItem& it = items[i];
     if (it.done) continue;
-    it.launches += 1;
-    it.sweeps = min(it.sweeps + S, it.t_end);
-    const float d = __uint_as_float(it.dmax_bits);
-    it.last_dmax = d;
-    if (d < it.tol || it.sweeps >= it.max_sweeps) {
-      it.done = 1;
-      it.t_done = now;
-    } else {
-      ++c;
-    }
-    it.dmax_bits = 0u;
+    // solo items were advanced by their work-group inside the round
+    if (!it.solo) batch_advance(it, S, now);
+    c += !it.done;
+    cs += !it.done && !it.solo;
   }
-  cnt[threadIdx.x] = c;
+  cnt[0][threadIdx.x] = c;
+  cnt[1][threadIdx.x] = cs;
   __syncthreads();
   for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) cnt[threadIdx.x] += cnt[threadIdx.x + s];
+    if (threadIdx.x < s) {
+      cnt[0][threadIdx.x] += cnt[0][threadIdx.x + s];
+      cnt[1][threadIdx.x] += cnt[1][threadIdx.x + s];
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    *live = cnt[0];
+    *live = cnt[0][0];
+    *shared_live = cnt[1][0];
     *queue = 0u;
+    *solo_queue = 0u;
   }
 }
 
@@ -728,10 +728,11 @@ int batch_blocks_per_cu(const TileCfg& cfg, int sched, int order) {
 }
 
 hipError_t launch_batch_epilogue(BatchItem* items, int n, int S,
-                                 unsigned* queue, unsigned* live,
+                                 unsigned* queue, unsigned* solo_queue,
+                                 unsigned* live, unsigned* shared_live,
                                  hipStream_t st) {
   hipLaunchKernelGGL(batch_epilogue_kernel, dim3(1), dim3(kBlock), 0, st, items,
-                     n, S, queue, live);
+                     n, S, queue, solo_queue, live, shared_live);
   return hipGetLastError();
 }
 

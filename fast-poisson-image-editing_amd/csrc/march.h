@@ -143,13 +143,22 @@ __device__ __forceinline__ void store_row(float* __restrict__ rowp,
 // block ends, so its outputs are complete and the slot may be reloaded.
 // The compiler's own wait insertion would wait for all LDS-DMA loads in
 // flight before any LDS read it can see.
-template <int S, int C, int NCH>
+// Slots of the ring: a divisor of S (the slot of every unrolled step is a
+// constant); three-channel and four-column tasks keep 4 rows in flight,
+// the others S rows; at most pfmax (work-groups of many waves cap it to fit
+// their rings into the LDS).
+constexpr int ring_slots(int S, int C, int NCH, int pfmax) {
+  int want = (NCH == 3 || C == 4) ? (S % 4 == 0 ? 4 : (S % 3 == 0 ? 3 : 2))
+                                  : S;
+  if (want > pfmax)
+    for (int d = pfmax; d >= 1; --d)
+      if (S % d == 0) return d;
+  return want;
+}
+
+template <int S, int C, int NCH, int PFMAX = 64>
 struct SweepRing {
-  // slots: a divisor of S (the slot of every unrolled step is a constant);
-  // three-channel tasks keep 4 rows in flight, one-channel tasks S rows
-  static constexpr int PF = (NCH == 3 || C == 4)
-                                ? (S % 4 == 0 ? 4 : (S % 3 == 0 ? 3 : 2))
-                                : S;
+  static constexpr int PF = ring_slots(S, C, NCH, PFMAX);
   static constexpr int kParts = C;                    // dword loads per row
   static constexpr int D = 2 * NCH * kParts;          // loads per row step
   static constexpr int kRowFloats = kWave * C;        // one channel row
@@ -176,7 +185,7 @@ template <int S, int C, int NCH>
 __device__ __forceinline__ void ring_fill(float* slot, const float* xrow,
                                          const float* grow, long long plane,
                                          int lane) {
-  using R = SweepRing<S, C, NCH>;
+  using R = SweepRing<S, C, NCH>;  // (row layout only: any slot count)
 #pragma unroll
   for (int arr = 0; arr < 2; ++arr)
 #pragma unroll
@@ -316,7 +325,7 @@ struct TaskGeom {
 // (t0 - rb - 1) & 1; with BAND the gs_lex band test 0 <= t - i - j < band
 // is made per pixel, without it the whole task lies inside the band.
 template <int S, int C, int NCH, int SCHED, int ORDER, bool DM, int PI,
-          bool BAND>
+          bool BAND, int PFMAX>
 __device__ __forceinline__ void march_rows(const MarchArgs& a,
                                            const TaskGeom& tg, float* ring,
                                            unsigned* dmax) {
@@ -330,7 +339,6 @@ __device__ __forceinline__ void march_rows(const MarchArgs& a,
   auto grow = [&](int row) { return tg.gin + row * P + cu; };
 
   float dm = 0.0f;
-  const int t_last_live = min(a.t0 + S, a.t_end) - 1;
   const int rb = max(0, r0 - S);
   const int nsteps = ((r1 - 1 + S) - rb + 1 + S - 1) / S * S;
 
@@ -358,7 +366,7 @@ __device__ __forceinline__ void march_rows(const MarchArgs& a,
   // This wave's LDS ring; rows are loaded PF row steps before their use.
   // Rows past the grid are clamped to row H - 1, which is never solved, so
   // every load is unconditional.
-  using R = SweepRing<S, C, NCH>;
+  using R = SweepRing<S, C, NCH, PFMAX>;
   constexpr int PF = R::PF;
   const uint32_t ring_addr = lds_offset(ring + lo);
   static_for<0, PF>([&](auto pc) {
@@ -389,10 +397,10 @@ __device__ __forceinline__ void march_rows(const MarchArgs& a,
         float(&newer)[C][NCH] = *pick<(u & 1) != 0>(&wb[j], &wa[j]);
         const int r = rho - k;
         const int t = a.t0 + j;
-        // Steps past t_end keep their value; folding that (wave-uniform)
-        // test into the predicate keeps the level free of branches, so
-        // the window registers are renamed rather than copied.
-        const bool live = t < a.t_end;
+        // Steps outside [t_lo, t_end) keep their value; folding that
+        // (wave-uniform) test into the predicate keeps the level free of
+        // branches, so the window registers are renamed rather than copied.
+        const bool live = t >= a.t_lo && t < a.t_end;
         float out[C][NCH];
         if constexpr (COLOUR) {
           // the column of the lane that may change at this level: t - r
@@ -497,16 +505,6 @@ __device__ __forceinline__ void march_rows(const MarchArgs& a,
             for (int ch = 0; ch < NCH; ++ch)
               out[cc][ch] = upd[cc] ? acc[cc][ch] * 0.25f : newer[cc][ch];
         }
-        if constexpr (DM) {
-          // the launch's last live step, on a row this task writes
-          if (t == t_last_live && r >= r0 && r < r1 && lane_out) {
-#pragma unroll
-            for (int cc = 0; cc < C; ++cc)
-#pragma unroll
-              for (int ch = 0; ch < NCH; ++ch)
-                dm = fmaxf(dm, fabsf(out[cc][ch] - newer[cc][ch]));
-          }
-        }
 #pragma unroll
         for (int cc = 0; cc < C; ++cc)
 #pragma unroll
@@ -517,8 +515,22 @@ __device__ __forceinline__ void march_rows(const MarchArgs& a,
       });
 
       const int rs = rho - S;
-      if (lane_out && rs >= r0 && rs < r1)
+      if (lane_out && rs >= r0 && rs < r1) {
         store_row<C, NCH>(xout + rs * P + cu, a.plane, lo, fr);
+        if constexpr (DM) {
+          // the change made by the launch's last step: level S's output
+          // against its input, which is still the newer row of level S - 1
+          // (the last step is live: a launch that ends an item's budget is
+          // aligned to end there)
+          const float(&prev)[C][NCH] =
+              *pick<(u & 1) != 0>(&wb[S - 1], &wa[S - 1]);
+#pragma unroll
+          for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch)
+              dm = fmaxf(dm, fabsf(fr[cc][ch] - prev[cc][ch]));
+        }
+      }
 
       // the guidance and flags of row rho enter slot u only now: level S of
       // this step still needed the row that slot held (rho - S)
@@ -541,10 +553,11 @@ __device__ __forceinline__ void march_rows(const MarchArgs& a,
 }
 
 // One wave task: column stripe `stripe`, row range `seg`, S steps.  With DM
-// the largest |change| made by the launch's last live step (step
-// min(t0 + S, t_end) - 1) over the pixels the task writes is merged into
-// *dmax (float bits, non-negative).
-template <int S, int C, int NCH, int SCHED, int ORDER, bool DM>
+// the largest |change| made by the launch's last step (step t0 + S - 1,
+// live for every launch DM is used with) over the pixels the task writes is
+// merged into *dmax (float bits, non-negative).
+template <int S, int C, int NCH, int SCHED, int ORDER, bool DM,
+          int WAVES = kBlockWaves, int PFMAX = 64>
 __device__ __forceinline__ void march_task(const MarchArgs& a, int stripe,
                                            int seg, int group,
                                            unsigned* dmax) {
@@ -592,10 +605,10 @@ __device__ __forceinline__ void march_task(const MarchArgs& a, int stripe,
     }
   }
 
-  // one ring for whichever variant of the rows runs
-  using R = SweepRing<S, C, NCH>;
+  // one ring for whichever variant of the rows runs (WAVES waves a group)
+  using R = SweepRing<S, C, NCH, PFMAX>;
   __shared__ __attribute__((aligned(16)))
-      float ring_all[kBlockWaves * R::kWaveFloats];
+      float ring_all[WAVES * R::kWaveFloats];
   float* ring = ring_all + (int)__builtin_amdgcn_readfirstlane(
                                (int)(threadIdx.x / kWave)) * R::kWaveFloats;
 
@@ -612,17 +625,21 @@ __device__ __forceinline__ void march_task(const MarchArgs& a, int stripe,
     }
     if (pi) {
       if (band)
-        march_rows<S, C, NCH, SCHED, ORDER, DM, 1, true>(a, tg, ring, dmax);
+        march_rows<S, C, NCH, SCHED, ORDER, DM, 1, true, PFMAX>(
+            a, tg, ring, dmax);
       else
-        march_rows<S, C, NCH, SCHED, ORDER, DM, 1, false>(a, tg, ring, dmax);
+        march_rows<S, C, NCH, SCHED, ORDER, DM, 1, false, PFMAX>(
+            a, tg, ring, dmax);
     } else {
       if (band)
-        march_rows<S, C, NCH, SCHED, ORDER, DM, 0, true>(a, tg, ring, dmax);
+        march_rows<S, C, NCH, SCHED, ORDER, DM, 0, true, PFMAX>(
+            a, tg, ring, dmax);
       else
-        march_rows<S, C, NCH, SCHED, ORDER, DM, 0, false>(a, tg, ring, dmax);
+        march_rows<S, C, NCH, SCHED, ORDER, DM, 0, false, PFMAX>(
+            a, tg, ring, dmax);
     }
   } else {
-    march_rows<S, C, NCH, SCHED, ORDER, DM, 0, SCHED == SCHED_GS_LEX>(
+    march_rows<S, C, NCH, SCHED, ORDER, DM, 0, SCHED == SCHED_GS_LEX, PFMAX>(
         a, tg, ring, dmax);
   }
 }
@@ -646,13 +663,103 @@ __global__ void __launch_bounds__(kBlock)
                                              rem / a.stripes, group, nullptr);
 }
 
-// Persistent sweep kernel of a batch: every wave takes tasks (tiles of any
-// item) from one atomic work counter until the counter passes the total,
-// so uneven items share the machine.  Items already stopped are skipped.
-// Each item's launch count selects its input buffer and first step.
+// Work-groups of the persistent batch kernel: 16 waves (one work-group a
+// CU), with ring slots capped so that 16 rings fit the LDS.
+constexpr int kBatchWaves = kBatchGroupWaves;
+constexpr int kBatchBlock = kWave * kBatchWaves;
+constexpr int kBatchPF = 6;
+
+// Sweep-kernel arguments of one launch of a batch item.
+__device__ __forceinline__ MarchArgs batch_launch(const BatchItem& it, int S,
+                                                  int seg_rows) {
+  MarchArgs a{};
+  const bool odd = it.launches & 1;
+  a.xin = odd ? it.xb : it.xa;
+  a.xout = odd ? it.xa : it.xb;
+  a.g = it.g;
+  a.m = it.m;
+  a.plane = it.plane;
+  a.H = it.H;
+  a.W = it.W;
+  a.P = it.P;
+  a.t0 = it.launches * S;
+  a.t_lo = a.t0;
+  a.t_end = it.t_end;
+  if (a.t0 + S > a.t_end) {
+    // the last launch of the budget: its live steps are its last ones, so
+    // the change of the item's last sweep is measured at the store
+    a.t0 = a.t_end - S;
+  }
+  a.band = 0;
+  a.stripes = it.stripes;
+  a.segs = it.segs;
+  a.seg_rows = seg_rows;
+  return a;
+}
+
+// After a launch of a live item: count it, apply the stop rules (the
+// launch's last sweep changed no value by tol or more, or the sweep budget
+// is used up) and clear the change maximum for the next launch.
+__device__ __forceinline__ void batch_advance(BatchItem& it, int S,
+                                              unsigned long long now) {
+  it.launches += 1;
+  it.sweeps = min(it.sweeps + S, it.t_end);
+  const float d = __uint_as_float(atomicExch(&it.dmax_bits, 0u));
+  it.last_dmax = d;
+  if (d < it.tol || it.sweeps >= it.max_sweeps) {
+    it.done = 1;
+    it.t_done = now;
+  }
+}
+
+// Persistent sweep kernel of a batch (one round).  Items come in two kinds:
+//  * solo items (few tasks) are each taken by one work-group, whose waves
+//    split the item's tasks and which runs launch after launch, with a
+//    work-group barrier between launches, for as long as tasks of the
+//    shared items remain unclaimed (at least one launch a claim), or to
+//    the item's end once no shared item is live.  A small item thus
+//    advances many launches a round and finishes early;
+//  * shared items: every wave takes tasks (tiles of any live shared item)
+//    from one atomic work counter until it passes the total, so uneven
+//    items share the machine; the epilogue kernel advances them by one
+//    launch a round.
+// Items already stopped are skipped; an item's launch count selects its
+// input buffer and first step.
 template <int S, int C, int NCH, int SCHED, int ORDER>
-__global__ void __launch_bounds__(kBlock) march_batch_kernel(BatchArgs b) {
+__global__ void __launch_bounds__(kBatchBlock) march_batch_kernel(BatchArgs b) {
+  constexpr int kW = kBatchWaves;
   const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+  __shared__ int s_item, s_go;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(b.solo_queue, 1u);
+    __syncthreads();
+    const int si = s_item;
+    __syncthreads();  // s_item is rewritten by the next claim
+    if (si >= b.nsolo) break;
+    BatchItem& it = b.items[b.solo_items[si]];
+    // it.* written by thread 0 is read by every thread after a barrier
+    while (!it.done) {
+      const MarchArgs a = batch_launch(it, S, b.seg_rows);
+      const int per_group = it.stripes * it.segs;
+      const int ntask = per_group * (3 / NCH);
+      for (int t = wave; t < ntask; t += kW) {
+        const int group = t / per_group, rem = t % per_group;
+        march_task<S, C, NCH, SCHED, ORDER, true, kW, kBatchPF>(
+            a, rem % it.stripes, rem / it.stripes, group, &it.dmax_bits);
+      }
+      __syncthreads();  // every task of the launch has stored its rows
+      if (threadIdx.x == 0) {
+        batch_advance(it, S, wall_clock64());
+        // go on while shared tasks wait to be claimed, or to the item's
+        // end once no shared item is live (nothing else to wait for)
+        s_go = atomicAdd(b.queue, 0u) < (unsigned)b.ntask ||
+               atomicAdd(const_cast<unsigned*>(b.shared_live), 0u) == 0u;
+      }
+      __syncthreads();
+      if (!s_go) break;
+    }
+  }
   for (;;) {
     int t = 0;
     if (lane == 0) t = (int)atomicAdd(b.queue, 1u);
@@ -661,28 +768,13 @@ __global__ void __launch_bounds__(kBlock) march_batch_kernel(BatchArgs b) {
     const int id = b.task_item[t];
     const BatchItem& it = b.items[id];
     if (it.done) continue;
-    MarchArgs a{};
-    const bool odd = it.launches & 1;
-    a.xin = odd ? it.xb : it.xa;
-    a.xout = odd ? it.xa : it.xb;
-    a.g = it.g;
-    a.m = it.m;
-    a.plane = it.plane;
-    a.H = it.H;
-    a.W = it.W;
-    a.P = it.P;
-    a.t0 = it.launches * S;
-    a.t_end = it.t_end;
-    a.band = 0;
-    a.stripes = it.stripes;
-    a.segs = it.segs;
-    a.seg_rows = b.seg_rows;
+    const MarchArgs a = batch_launch(it, S, b.seg_rows);
     const int local = t - it.task0;
     const int per_group = it.stripes * it.segs;
     const int group = local / per_group, rem = local % per_group;
-    march_task<S, C, NCH, SCHED, ORDER, true>(a, rem % it.stripes,
-                                              rem / it.stripes, group,
-                                              &b.items[id].dmax_bits);
+    march_task<S, C, NCH, SCHED, ORDER, true, kW, kBatchPF>(
+        a, rem % it.stripes, rem / it.stripes, group,
+        &b.items[id].dmax_bits);
   }
 }
 

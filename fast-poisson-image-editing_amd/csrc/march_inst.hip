@@ -66,7 +66,7 @@ hipError_t PIE_BNAME(PIE_S, PIE_C, PIE_N)(int sched, int order,
 #define PIE_L(SC, OR)                                                          \
   if (sched == SC && order == OR) {                                           \
     hipLaunchKernelGGL((march_batch_kernel<PIE_S, PIE_C, PIE_N, SC, OR>),             \
-                       dim3(blocks), dim3(kBlock), 0, st, b);                  \
+                       dim3(blocks), dim3(kBatchBlock), 0, st, b);             \
     return hipGetLastError();                                                 \
   }
   PIE_L(SCHED_JACOBI, ORDER_UDLR)
@@ -82,7 +82,7 @@ int PIE_ONAME(PIE_S, PIE_C, PIE_N)(int sched, int order) {
 #define PIE_L(SC, OR)                                                       \
   if (sched == SC && order == OR)                                          \
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(                       \
-        &n, (march_batch_kernel<PIE_S, PIE_C, PIE_N, SC, OR>), kBlock, 0);
+        &n, (march_batch_kernel<PIE_S, PIE_C, PIE_N, SC, OR>), kBatchBlock, 0);
   PIE_L(SCHED_JACOBI, ORDER_UDLR)
   PIE_L(SCHED_JACOBI, ORDER_ULRD)
 #undef PIE_L

@@ -173,6 +173,7 @@ int pie_grid_solve_noalloc(int h, int w, const int32_t* mask,
     a.W = w;
     a.P = L.P;
     a.t0 = t0;
+    a.t_lo = t0;
     a.t_end = steps;
     a.band = 2 * iterations;
     PIE_HIP(pie::launch_march(cfg, schedule, order, a, st));

@@ -190,6 +190,7 @@ MarchArgs march_args(pie_solver* s, const float* xin, float* xout, int t0,
   a.W = s->W;
   a.P = s->P;
   a.t0 = t0;
+  a.t_lo = t0;
   a.t_end = t_end;
   a.band = band;
   return a;

@@ -384,8 +384,9 @@ class GridSolver(_Core):
 
 class BatchSolver:
   """Batched GridSolver (extension): composites of uneven size and mask,
-  each with its own stop rule, solved by one persistent Jacobi kernel over a
-  shared tile queue with the round loop in a hipGraph."""
+  each with its own stop rule, solved by one persistent Jacobi kernel (small
+  items each by one work-group, several launches a round; the others over a
+  shared tile queue) with the round loop in a hipGraph."""
 
   def __init__(self, order: str = "ulrd", sweeps_per_launch: int = 8,
                cols_per_lane: int = 1, rows_per_task: int = 64,
@@ -404,6 +405,19 @@ class BatchSolver:
     """Channels per wave task (1 or 3; 0 = the tile's default); the results
     do not depend on it."""
     native.check(self._lib.pie_batch_set_tile_channels(self._h, channels))
+
+  def autotune(self, rounds: int = 6) -> tuple[int, int, int, int]:
+    """Time every tile for a few rounds from the initial state and keep the
+    fastest; returns it (see tile())."""
+    native.check(self._lib.pie_batch_autotune(self._h, int(rounds)))
+    return self.tile()
+
+  def tile(self) -> tuple[int, int, int, int]:
+    """(sweeps per launch, columns per lane, rows per task, channels)."""
+    v = [ctypes.c_int() for _ in range(4)]
+    native.check(self._lib.pie_batch_tile(self._h, *(ctypes.byref(x)
+                                                     for x in v)))
+    return tuple(x.value for x in v)
 
   def __del__(self):
     h = getattr(self, "_h", None)

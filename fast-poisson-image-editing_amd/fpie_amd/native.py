@@ -84,6 +84,9 @@ _SIGNATURES = {
                                 _vp, _vp], _ip),
     "pie_batch_create": ([_ip, _ip, _ip, _ip, ctypes.POINTER(_vp)], _ip),
     "pie_batch_destroy": ([_vp], _ip),
+    "pie_batch_autotune": ([_vp, _ip], _ip),
+    "pie_batch_tile": ([_vp, ctypes.POINTER(_ip), ctypes.POINTER(_ip),
+                        ctypes.POINTER(_ip), ctypes.POINTER(_ip)], _ip),
     "pie_batch_set_tile_channels": ([_vp, _ip], _ip),
     "pie_batch_add": ([_vp, _ip, _ip, _vp, _vp, _vp, _ip, ctypes.c_float,
                        ctypes.POINTER(_ip)], _ip),

@@ -242,10 +242,11 @@ int pie_warm_apply(pie_solver* s, int factor, const float* corr,
 
 /* ---- batched GridSolver: no counterpart in the reference ---------------
  * Many composites of uneven size and mask, each with its own stop rule,
- * solved together by one persistent Jacobi sweep kernel over a shared work
- * queue of tiles, R rounds per hipGraph replay.  An item stops after the
- * first launch whose last sweep changed no value by `tol` or more, or after
- * `max_sweeps` sweeps. */
+ * solved together by one persistent Jacobi sweep kernel, R rounds per
+ * hipGraph replay: items of few tiles are each run by one work-group,
+ * several launches a round, the others share a work queue of tiles, one
+ * launch a round.  An item stops after the first launch whose last sweep
+ * changed no value by `tol` or more, or after `max_sweeps` sweeps. */
 typedef struct pie_batch pie_batch;
 /* order: PIE_ORDER_*; tile: sweeps per launch, columns per lane, rows per
  * task (0 = default 8, 1, 64). */
@@ -257,6 +258,14 @@ int pie_batch_create(int order, int sweeps_per_launch, int cols_per_lane,
  * work over waves differs.  Any time before pie_batch_solve. */
 int pie_batch_set_tile_channels(pie_batch* b, int channels);
 int pie_batch_destroy(pie_batch* b);
+/* Time `rounds` rounds of every tile (sweeps, columns per lane, rows per
+ * task, channels) on the batch from its initial state and keep the one with
+ * the least time per sweep; the next pie_batch_solve starts over. */
+int pie_batch_autotune(pie_batch* b, int rounds);
+/* The batch's tile: sweeps per launch, columns per lane, rows per task and
+ * channels per task (any pointer may be NULL). */
+int pie_batch_tile(pie_batch* b, int* sweeps, int* cols_per_lane,
+                   int* seg_rows, int* channels);
 /* Add one GridSolver problem (arguments as pie_reset_grid); *index gets its
  * position in the batch. */
 int pie_batch_add(pie_batch* b, int h, int w, const int32_t* mask,

@@ -129,3 +129,42 @@ def test_batch_rejects_bad_items(hip):
     b.add(m, t, t)
   with pytest.raises(RuntimeError, match="empty batch"):
     b.solve()
+
+
+def test_budget_stop_reports_the_last_sweeps_change(hip, oracle):
+  """A budget that is not a multiple of the sweeps per launch: the last
+  launch is aligned to end on the budget, and last_dmax is the change of
+  the item's last sweep (not 0)."""
+  items = _items(5, 23)
+  budgets = [13, 37, 5, 44, 9]
+  b = hip.BatchSolver(sweeps_per_launch=8)
+  for (m, t, g), k in zip(items, budgets):
+    b.add(m, t, g, max_sweeps=k, tol=1e-12)
+  b.solve(rounds_per_graph=4)
+  for i, ((m, t, g), k) in enumerate(zip(items, budgets)):
+    info = b.item(i)
+    assert info["sweeps"] == k
+    want = oracle.grid_jacobi(m, t, g, k, "ulrd")
+    assert np.array_equal(bits(b.read(i, state=True)[2]), bits(want)), i
+    before = oracle.grid_jacobi(m, t, g, k - 1, "ulrd")
+    assert info["last_dmax"] == float(np.abs(want - before).max()) > 0.0
+
+
+def test_solo_and_shared_items_match_oracle(hip, oracle):
+  """Items small enough to be run by one work-group (several launches a
+  round) and items shared over the tile queue, in one batch: every item
+  equals the oracle at its budget, and the small items finish first."""
+  items = _items(8, 29, side=(40, 700))
+  b = hip.BatchSolver(sweeps_per_launch=8)
+  budgets = [200, 120, 333, 64, 250, 17, 96, 300]
+  for (m, t, g), k in zip(items, budgets):
+    b.add(m, t, g, max_sweeps=k, tol=0.0)
+  b.solve(rounds_per_graph=4)
+  sizes = [int(np.count_nonzero(m)) for m, _, _ in items]
+  for i, ((m, t, g), k) in enumerate(zip(items, budgets)):
+    assert b.item(i)["sweeps"] == k
+    want = oracle.grid_jacobi(m, t, g, k, "ulrd", threads=16)
+    assert np.array_equal(bits(b.read(i, state=True)[2]), bits(want)), (i, k)
+  lat = [b.item(i)["latency_s"] for i in range(len(items))]
+  assert all(x >= 0.0 for x in lat)
+  print(f"\nsizes {sizes}\nlatencies {lat}")
