@@ -73,6 +73,23 @@ struct pie_unet {
   DevBuf<bf16_t> dyb, dyb2;
   DevBuf<double> acc;
   double ms_total = 0.0, flops_total = 0.0;
+  // pipelined training (pie_unet_staging / pie_unet_train_async /
+  // pie_unet_finish): two slots, each a pinned host copy of a batch, its
+  // device copy, its loss sums and the events that order the upload (on
+  // the copy stream) against the steps that read it
+  struct Slot {
+    float* hx = nullptr;
+    float* ht = nullptr;
+    uint8_t* hm = nullptr;
+    DevBuf<float> x, t;
+    DevBuf<uint8_t> m;
+    DevBuf<double> acc;
+    hipEvent_t uploaded = nullptr, consumed = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;  // device time of the step
+    bool used = false;     // an upload was enqueued
+    bool pending = false;  // a step was enqueued and not yet finished
+  } slot[2];
+  hipStream_t copy = nullptr;
 };
 
 namespace {
@@ -151,10 +168,9 @@ int conv_fwd(pie_unet* u, int l, const Slice& x, const Slice& y,
   return PIE_OK;
 }
 
-int forward(pie_unet* u) {
+int forward(pie_unet* u, const float* x) {
   hipStream_t st = u->stream;
-  PIE_HIP(pie::unet::to_bf16(u->x32.p, (int)u->P[0], 8, sl(u->in, 8, 0, 8),
-                             st));
+  PIE_HIP(pie::unet::to_bf16(x, (int)u->P[0], 8, sl(u->in, 8, 0, 8), st));
   PIE_CHECK(conv_fwd(u, E0A, sl(u->in, 8, 0, 8), sl(u->e0a, 32, 0, 32)));
   // encoder outputs go to the skip half of the concatenation buffers, with
   // their average pool for the next level; decoder level outputs to their
@@ -285,18 +301,74 @@ int load_batch(pie_unet* u, const float* x, const float* target,
   return PIE_OK;
 }
 
-int loss_and_grad(pie_unet* u, float* loss) {
-  PIE_HIP(hipMemsetAsync(u->acc.p, 0, 2 * sizeof(double), u->stream));
-  PIE_HIP(pie::unet::mse(u->out.p, u->target.p, u->mask.p, (int)u->P[0],
-                         u->dout.p, u->acc.p, u->stream));
-  PIE_HIP(pie::unet::scale(u->dout.p, u->P[0] * 3, u->acc.p, u->stream));
+// The masked loss's sums into acc (device) and its gradient back through
+// the network; `consumed` (if given) is recorded once target and mask are
+// read.
+int loss_backward(pie_unet* u, const float* target, const uint8_t* mask,
+                  double* acc, hipEvent_t consumed) {
+  PIE_HIP(hipMemsetAsync(acc, 0, 2 * sizeof(double), u->stream));
+  PIE_HIP(pie::unet::mse(u->out.p, target, mask, (int)u->P[0], u->dout.p,
+                         acc, u->stream));
+  if (consumed) PIE_HIP(hipEventRecord(consumed, u->stream));
+  PIE_HIP(pie::unet::scale(u->dout.p, u->P[0] * 3, acc, u->stream));
   PIE_CHECK(backward(u));
+  return PIE_OK;
+}
+
+int read_loss(pie_unet* u, const double* acc_dev, float* loss) {
   double acc[2];
-  PIE_HIP(hipMemcpyAsync(acc, u->acc.p, sizeof(acc), hipMemcpyDeviceToHost,
+  PIE_HIP(hipMemcpyAsync(acc, acc_dev, sizeof(acc), hipMemcpyDeviceToHost,
                          u->stream));
   PIE_HIP(hipStreamSynchronize(u->stream));
   if (loss) *loss = (float)(acc[0] / std::max(1.0, acc[1]));
   return PIE_OK;
+}
+
+int loss_and_grad(pie_unet* u, float* loss) {
+  PIE_CHECK(loss_backward(u, u->target.p, u->mask.p, u->acc.p, nullptr));
+  return read_loss(u, u->acc.p, loss);
+}
+
+// Adam on the gradient, then the bf16 operand copies of the new weights.
+int optimiser_step(pie_unet* u, float lr) {
+  ++u->step;
+  PIE_HIP(pie::unet::adam(u->param.p, u->grad.p, u->adam_m.p, u->adam_v.p,
+                          u->nparam, lr, 0.9f, 0.999f, 1e-8f, u->step,
+                          u->stream));
+  return pack_all(u);
+}
+
+int slot_alloc(pie_unet* u, int k) {
+  pie_unet::Slot& sl = u->slot[k];
+  if (sl.hx) return PIE_OK;
+  const long long P0 = u->P[0];
+  PIE_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.hx),
+                        P0 * 8 * sizeof(float), hipHostMallocDefault));
+  PIE_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.ht),
+                        P0 * 3 * sizeof(float), hipHostMallocDefault));
+  PIE_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.hm), P0,
+                        hipHostMallocDefault));
+  PIE_HIP(sl.x.alloc(P0 * 8));
+  PIE_HIP(sl.t.alloc(P0 * 3));
+  PIE_HIP(sl.m.alloc(P0));
+  PIE_HIP(sl.acc.alloc(2));
+  PIE_HIP(hipEventCreateWithFlags(&sl.uploaded, hipEventDisableTiming));
+  PIE_HIP(hipEventCreateWithFlags(&sl.consumed, hipEventDisableTiming));
+  PIE_HIP(hipEventCreate(&sl.t0));
+  PIE_HIP(hipEventCreate(&sl.t1));
+  if (!u->copy)
+    PIE_HIP(hipStreamCreateWithFlags(&u->copy, hipStreamNonBlocking));
+  return PIE_OK;
+}
+
+void slot_free(pie_unet* u) {
+  for (auto& sl : u->slot) {
+    if (sl.hx) (void)hipHostFree(sl.hx);
+    if (sl.ht) (void)hipHostFree(sl.ht);
+    if (sl.hm) (void)hipHostFree(sl.hm);
+    for (hipEvent_t e : {sl.uploaded, sl.consumed, sl.t0, sl.t1})
+      if (e) (void)hipEventDestroy(e);
+  }
 }
 
 }  // namespace
@@ -317,9 +389,7 @@ int unet_forward_on_stream(pie_unet* u, int h, int w, const float* x,
     PIE_HIP(hipEventRecord(u->ev_in, st));
     PIE_HIP(hipStreamWaitEvent(u->stream, u->ev_in, 0));
   }
-  PIE_HIP(hipMemcpyAsync(u->x32.p, x, P0 * 8 * sizeof(float),
-                         hipMemcpyDeviceToDevice, u->stream));
-  PIE_CHECK(forward(u));
+  PIE_CHECK(forward(u, x));
   PIE_HIP(hipMemcpyAsync(out, u->out.p, P0 * 3 * sizeof(float),
                          hipMemcpyDeviceToDevice, u->stream));
   if (st) {
@@ -383,6 +453,9 @@ int pie_unet_create(int batch, int h, int w, pie_unet** out) {
 int pie_unet_destroy(pie_unet* u) {
   if (!u) return PIE_OK;
   if (u->stream) (void)hipStreamSynchronize(u->stream);
+  if (u->copy) (void)hipStreamSynchronize(u->copy);
+  slot_free(u);
+  if (u->copy) (void)hipStreamDestroy(u->copy);
   if (u->ev0) (void)hipEventDestroy(u->ev0);
   if (u->ev1) (void)hipEventDestroy(u->ev1);
   if (u->ev_in) (void)hipEventDestroy(u->ev_in);
@@ -424,7 +497,7 @@ int pie_unet_get_params(pie_unet* u, float* p) {
 int pie_unet_forward(pie_unet* u, const float* x, float* out) {
   if (!u || !x || !out) return fail(PIE_EINVAL, "NULL argument");
   PIE_CHECK(load_batch(u, x, nullptr, nullptr));
-  PIE_CHECK(forward(u));
+  PIE_CHECK(forward(u, u->x32.p));
   PIE_HIP(hipMemcpyAsync(out, u->out.p, u->P[0] * 3 * sizeof(float),
                          hipMemcpyDeviceToHost, u->stream));
   PIE_HIP(hipStreamSynchronize(u->stream));
@@ -442,7 +515,7 @@ int pie_unet_loss_grad(pie_unet* u, const float* x, const float* target,
                        const uint8_t* mask, float* loss, float* grads) {
   if (!u || !x || !target || !mask) return fail(PIE_EINVAL, "NULL argument");
   PIE_CHECK(load_batch(u, x, target, mask));
-  PIE_CHECK(forward(u));
+  PIE_CHECK(forward(u, u->x32.p));
   PIE_CHECK(loss_and_grad(u, loss));
   if (grads) {
     PIE_HIP(hipMemcpyAsync(grads, u->grad.p, u->nparam * sizeof(float),
@@ -457,19 +530,74 @@ int pie_unet_train_step(pie_unet* u, const float* x, const float* target,
   if (!u || !x || !target || !mask) return fail(PIE_EINVAL, "NULL argument");
   PIE_CHECK(load_batch(u, x, target, mask));
   PIE_HIP(hipEventRecord(u->ev0, u->stream));
-  PIE_CHECK(forward(u));
+  PIE_CHECK(forward(u, u->x32.p));
   PIE_CHECK(loss_and_grad(u, loss));  // synchronises
   PIE_HIP(hipEventRecord(u->ev1, u->stream));
-  ++u->step;
-  PIE_HIP(pie::unet::adam(u->param.p, u->grad.p, u->adam_m.p, u->adam_v.p,
-                          u->nparam, lr, 0.9f, 0.999f, 1e-8f, u->step,
-                          u->stream));
-  PIE_CHECK(pack_all(u));
+  PIE_CHECK(optimiser_step(u, lr));
   PIE_HIP(hipStreamSynchronize(u->stream));
   float ms = 0.0f;
   PIE_HIP(hipEventElapsedTime(&ms, u->ev0, u->ev1));
   u->ms_total += ms;
   u->flops_total += conv_flops(u, true);
+  return PIE_OK;
+}
+
+int pie_unet_staging(pie_unet* u, int k, float** x, float** target,
+                     uint8_t** mask) {
+  if (!u || !x || !target || !mask) return fail(PIE_EINVAL, "NULL argument");
+  if (k < 0 || k > 1) return fail(PIE_EINVAL, "slot must be 0 or 1");
+  PIE_CHECK(slot_alloc(u, k));
+  pie_unet::Slot& sl = u->slot[k];
+  // the host copy is free once its last upload has been read
+  if (sl.used) PIE_HIP(hipEventSynchronize(sl.uploaded));
+  *x = sl.hx;
+  *target = sl.ht;
+  *mask = sl.hm;
+  return PIE_OK;
+}
+
+int pie_unet_train_async(pie_unet* u, int k, float lr) {
+  if (!u) return fail(PIE_EINVAL, "NULL argument");
+  if (k < 0 || k > 1 || !u->slot[k].hx)
+    return fail(PIE_EINVAL, "slot not set up by pie_unet_staging");
+  pie_unet::Slot& sl = u->slot[k];
+  if (sl.pending)
+    return fail(PIE_ESTATE, "the slot's last step is not finished");
+  const long long P0 = u->P[0];
+  // upload on the copy stream, once the step that last read the slot's
+  // device copy has read it
+  if (sl.used) PIE_HIP(hipStreamWaitEvent(u->copy, sl.consumed, 0));
+  PIE_HIP(hipMemcpyAsync(sl.x.p, sl.hx, P0 * 8 * sizeof(float),
+                         hipMemcpyHostToDevice, u->copy));
+  PIE_HIP(hipMemcpyAsync(sl.t.p, sl.ht, P0 * 3 * sizeof(float),
+                         hipMemcpyHostToDevice, u->copy));
+  PIE_HIP(hipMemcpyAsync(sl.m.p, sl.hm, P0, hipMemcpyHostToDevice, u->copy));
+  PIE_HIP(hipEventRecord(sl.uploaded, u->copy));
+  sl.used = true;
+  PIE_HIP(hipStreamWaitEvent(u->stream, sl.uploaded, 0));
+  PIE_HIP(hipEventRecord(sl.t0, u->stream));
+  PIE_CHECK(forward(u, sl.x.p));
+  PIE_CHECK(loss_backward(u, sl.t.p, sl.m.p, sl.acc.p, sl.consumed));
+  PIE_HIP(hipEventRecord(sl.t1, u->stream));
+  PIE_CHECK(optimiser_step(u, lr));
+  sl.pending = true;
+  return PIE_OK;
+}
+
+int pie_unet_finish(pie_unet* u, int k, float* loss) {
+  if (!u) return fail(PIE_EINVAL, "NULL argument");
+  if (k < 0 || k > 1) return fail(PIE_EINVAL, "slot must be 0 or 1");
+  pie_unet::Slot& sl = u->slot[k];
+  if (!sl.pending) return fail(PIE_ESTATE, "no step in flight on the slot");
+  PIE_HIP(hipEventSynchronize(sl.t1));
+  float ms = 0.0f;
+  PIE_HIP(hipEventElapsedTime(&ms, sl.t0, sl.t1));
+  u->ms_total += ms;
+  u->flops_total += conv_flops(u, true);
+  double acc[2];
+  PIE_HIP(hipMemcpy(acc, sl.acc.p, sizeof(acc), hipMemcpyDeviceToHost));
+  if (loss) *loss = (float)(acc[0] / std::max(1.0, acc[1]));
+  sl.pending = false;
   return PIE_OK;
 }
 

@@ -75,6 +75,10 @@ _SIGNATURES = {
                             ctypes.POINTER(ctypes.c_float), _vp], _ip),
     "pie_unet_train_step": ([_vp, _vp, _vp, _vp, ctypes.c_float,
                              ctypes.POINTER(ctypes.c_float)], _ip),
+    "pie_unet_staging": ([_vp, _ip, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                          ctypes.POINTER(_vp)], _ip),
+    "pie_unet_train_async": ([_vp, _ip, ctypes.c_float], _ip),
+    "pie_unet_finish": ([_vp, _ip, ctypes.POINTER(ctypes.c_float)], _ip),
     "pie_unet_stats": ([_vp, ctypes.POINTER(ctypes.c_double),
                         ctypes.POINTER(ctypes.c_double)], _ip),
     "pie_unet_set_conv_tiles": ([_ip], _ip),

@@ -171,6 +171,37 @@ class UNet:
                                                float(lr), ctypes.byref(loss)))
     return loss.value
 
+  def staging(self, k: int):
+    """Slot k's pinned host batch as numpy views (x [B, H, W, 8], target
+    [B, H, W, 3] float32, mask [B, H, W] uint8), once its last upload has
+    finished.  Fill them, then call train_async(k)."""
+    px, pt, pm = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    native.check(self._lib.pie_unet_staging(self._h, int(k), ctypes.byref(px),
+                                            ctypes.byref(pt),
+                                            ctypes.byref(pm)))
+    b, h, w = self.shape
+
+    def view(ptr, ctype, shape):
+      n = int(np.prod(shape))
+      buf = (ctype * n).from_address(ptr.value)
+      return np.frombuffer(buf, dtype=np.dtype(ctype)).reshape(shape)
+
+    return (view(px, ctypes.c_float, (b, h, w, 8)),
+            view(pt, ctypes.c_float, (b, h, w, 3)),
+            view(pm, ctypes.c_uint8, (b, h, w)))
+
+  def train_async(self, k: int, lr: float) -> None:
+    """Enqueue the upload of slot k and an Adam step on it; returns at
+    once (the loss comes from finish(k))."""
+    native.check(self._lib.pie_unet_train_async(self._h, int(k), float(lr)))
+
+  def finish(self, k: int) -> float:
+    """Wait for slot k's step; its loss."""
+    loss = ctypes.c_float()
+    native.check(self._lib.pie_unet_finish(self._h, int(k),
+                                           ctypes.byref(loss)))
+    return loss.value
+
   def stats(self) -> dict:
     ms, fl = ctypes.c_double(), ctypes.c_double()
     native.check(self._lib.pie_unet_stats(self._h, ctypes.byref(ms),
@@ -182,30 +213,47 @@ class UNet:
 # -------------------------------------------------------------- training ---
 
 
-def batch_arrays(M, X, Y, idx):
+def batch_arrays(M, X, Y, idx, out=None):
   """Network input, label and mask of the samples idx: M uint8 [n, h, w]
   masks, X [n, h, w, 3] normalised residuals, Y [n, h, w, 3] normalised
-  corrections."""
-  x = np.zeros((len(idx), M.shape[1], M.shape[2], 8), np.float32)
+  corrections.  out: (x, y, m) arrays to fill instead of new ones."""
+  if out is None:
+    out = (np.empty((len(idx), M.shape[1], M.shape[2], 8), np.float32),
+           np.empty((len(idx),) + Y.shape[1:], np.float32),
+           np.empty((len(idx),) + M.shape[1:], np.uint8))
+  x, y, m = out
+  x.fill(0.0)
   x[..., 0:3] = X[idx]
   x[..., 6] = M[idx]
-  return x, Y[idx].astype(np.float32), M[idx]
+  y[...] = Y[idx]
+  m[...] = M[idx]
+  return x, y, m
 
 
 def train(M, X, Y, epochs: int = 4, batch: int = 64, lr: float = 1e-3,
           seed: int = 0, log=print) -> tuple[UNet, dict]:
   """Adam on the masked mean squared error over shuffled minibatches (the
-  last partial batch of an epoch is dropped)."""
+  last partial batch of an epoch is dropped).  Pipelined: the next batch is
+  gathered into one pinned staging slot and uploaded while the GPU runs
+  the step on the other."""
   n, h, w = M.shape
   net = UNet(batch, h, w, seed=seed)
   rng = np.random.default_rng(seed)
   losses = []
   t0 = time.perf_counter()
+  inflight = None  # slot of the step on the GPU
   for ep in range(epochs):
     order = rng.permutation(n)
     for s in range(0, n - batch + 1, batch):
-      x, y, m = batch_arrays(M, X, Y, order[s:s + batch])
-      losses.append(net.train_step(x, y, m, lr))
+      slot = 0 if inflight is None else 1 - inflight
+      batch_arrays(M, X, Y, order[s:s + batch], out=net.staging(slot))
+      net.train_async(slot, lr)
+      if inflight is not None:
+        losses.append(net.finish(inflight))
+      inflight = slot
+    if inflight is not None:
+      losses.append(net.finish(inflight))
+      inflight = None
     log(f"epoch {ep + 1}/{epochs}: loss {np.mean(losses[-(n // batch):]):.5f} "
         f"({time.perf_counter() - t0:.1f} s)")
   st = net.stats()
@@ -215,31 +263,47 @@ def train(M, X, Y, epochs: int = 4, batch: int = 64, lr: float = 1e-3,
 
 def measure(batch: int = 64, size: int = 256, steps: int = 5) -> dict:
   """Time Adam steps of UNet(batch, size, size) on random inputs (one
-  untimed step first).  device_ms_per_step: forward + backward between HIP
-  events on the network's stream, the batch already uploaded; ms_per_step:
-  whole train_step calls including the host-to-device upload of the batch,
-  Adam and the weight packing.  conv_tflops counts 2 H W 9 Cin Cout per
-  convolution and pass."""
+  untimed step first), pipelined as in train(): each step's batch is
+  uploaded from a pinned staging slot on a copy stream while the other
+  slot's step runs.  device_ms_per_step: forward + backward between HIP
+  events on the network's stream; ms_per_step: wall time per step of the
+  whole loop (uploads, Adam and the weight packing included; the batches
+  are written into the two slots once, before the loop).  ms_per_step_sync:
+  the same step through the unpipelined train_step (upload from pageable
+  memory on the compute stream, then the step).  conv_tflops counts
+  2 H W 9 Cin Cout per convolution and pass."""
   rng = np.random.default_rng(0)
   b, s = batch, size
-  x = np.zeros((b, s, s, 8), np.float32)
-  x[..., :3] = rng.normal(0, 1, (b, s, s, 3))
-  x[..., 6] = 1.0
-  y = rng.normal(0, 1, (b, s, s, 3)).astype(np.float32)
-  m = np.ones((b, s, s), np.uint8)
   net = UNet(b, s, s, seed=0)
-  net.train_step(x, y, m, 1e-3)  # untimed: kernel loading
+  for k in range(2):
+    x, y, m = net.staging(k)
+    x.fill(0.0)
+    x[..., :3] = rng.normal(0, 1, (b, s, s, 3))
+    x[..., 6] = 1.0
+    y[...] = rng.normal(0, 1, (b, s, s, 3))
+    m.fill(1)
+  net.train_async(0, 1e-3)  # untimed: kernel loading
+  net.finish(0)
   before = net.stats()
   t0 = time.perf_counter()
-  for _ in range(steps):
-    net.train_step(x, y, m, 1e-3)
+  for k in range(steps):
+    net.staging(k % 2)  # the slot's host copy is free again
+    net.train_async(k % 2, 1e-3)
+    if k > 0:
+      net.finish((k - 1) % 2)
+  net.finish((steps - 1) % 2)
   dt = time.perf_counter() - t0
   after = net.stats()
   flops = after["conv_flops"] - before["conv_flops"]
   dev = (after["ms"] - before["ms"]) / 1e3
+  x, y, m = (np.array(a) for a in net.staging(0))
+  t1 = time.perf_counter()
+  net.train_step(x, y, m, 1e-3)
+  sync_ms = (time.perf_counter() - t1) * 1e3
   return {"batch": b, "size": s, "steps": steps,
           "ms_per_step": dt * 1e3 / steps,
           "device_ms_per_step": dev * 1e3 / steps,
+          "ms_per_step_sync": sync_ms,
           "conv_tflops": flops / dev / 1e12,
           "conv_tflops_wall": flops / dt / 1e12,
           "mfma_utilisation_vs_2500_tflops": flops / dev / 2.5e15}

@@ -329,6 +329,19 @@ int pie_unet_loss_grad(pie_unet* u, const float* x, const float* target,
 /* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the batch. */
 int pie_unet_train_step(pie_unet* u, const float* x, const float* target,
                         const uint8_t* mask, float lr, float* loss);
+/* Pipelined training (no counterpart in the reference): two staging slots,
+ * each a pinned host batch (x, target, mask in the layouts above) and its
+ * device copy.  pie_unet_staging returns slot k's host buffers once its
+ * last upload has finished (allocating them on first use); the caller
+ * fills them and calls pie_unet_train_async, which enqueues the upload on
+ * a copy stream and the Adam step on the network's stream and returns at
+ * once.  pie_unet_finish waits for slot k's step and returns its loss; it
+ * must be called before the slot's next pie_unet_train_async.  Filling and
+ * uploading one slot overlap the other slot's step. */
+int pie_unet_staging(pie_unet* u, int k, float** x, float** target,
+                     uint8_t** mask);
+int pie_unet_train_async(pie_unet* u, int k, float lr);
+int pie_unet_finish(pie_unet* u, int k, float* loss);
 /* Device time of the forward + backward passes so far and the convolution
  * FLOPs they did (for MFMA utilisation). */
 int pie_unet_stats(pie_unet* u, double* ms, double* conv_flops);

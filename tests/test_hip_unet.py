@@ -114,3 +114,37 @@ def test_adam_steps_reduce_the_loss(unet):
   assert last < 0.5 * first, (first, last)
   st = net.stats()
   assert st["conv_flops"] > 0 and st["ms"] > 0
+
+
+def test_pipelined_steps_match_train_step(unet):
+  """Adam steps through the two pinned staging slots (upload on the copy
+  stream overlapping the other slot's step) follow the same trajectory as
+  train_step on the same batches: same losses and parameters up to the
+  order of the weight gradient's fp32 atomics."""
+  batches = [_problem(2, 32, 40, 20 + k) for k in range(5)]
+  p = unet.init_params(3)
+  a = unet.UNet(2, 32, 40, params=p)
+  la = [a.train_step(x, y, m, 1e-3) for x, y, m in batches]
+  b = unet.UNet(2, 32, 40, params=p)
+  lb, inflight = [], None
+  for x, y, m in batches:
+    slot = 0 if inflight is None else 1 - inflight
+    xs, ys, ms = b.staging(slot)
+    xs[...], ys[...], ms[...] = x, y, m
+    b.train_async(slot, 1e-3)
+    if inflight is not None:
+      lb.append(b.finish(inflight))
+    inflight = slot
+  lb.append(b.finish(inflight))
+  np.testing.assert_allclose(lb, la, rtol=1e-4)
+  # Adam divides by the root of the second moment, so the atomics' last-bit
+  # differences in near-zero gradient components move single parameters by
+  # up to lr: compare against the size of the whole update
+  p0 = p.astype(np.float64)
+  pa, pb = a.params().astype(np.float64), b.params().astype(np.float64)
+  assert np.linalg.norm(pa - pb) <= 1e-2 * np.linalg.norm(pa - p0)
+  # a slot's step must be finished before the slot is used again
+  with pytest.raises(RuntimeError):
+    b.finish(inflight)
+  sa, sb = a.stats(), b.stats()
+  assert sb["conv_flops"] == sa["conv_flops"] and sb["ms"] > 0
