@@ -214,14 +214,12 @@ __global__ void __launch_bounds__(kBlock) conv3x3_kernel(ConvArgs a) {
 // whether window word k lies inside the image (the halo outside is zero,
 // applied when the registers are stored to LDS, so that the compiler keeps
 // the loads unconditional and in flight together).
-template <int WH, int WW, int CT>
-__device__ __forceinline__ uint32_t conv_stage_load(
-    const ConvArgs& a, long long img_base, int row0, int col0, int co0,
-    int c32, u32x4 (&wv)[(WH * WW * 4 + kBlock - 1) / kBlock],
-    u32x4 (&wt)[(9 * 4 * CT + kBlock - 1) / kBlock]) {
+// The window part of conv_stage_load.
+template <int WH, int WW>
+__device__ __forceinline__ uint32_t conv_win_load(
+    const ConvArgs& a, long long img_base, int row0, int col0, int c32,
+    u32x4 (&wv)[(WH * WW * 4 + kBlock - 1) / kBlock]) {
   constexpr int kWinJobs = WH * WW * 4, kWinIt = (kWinJobs + kBlock - 1) / kBlock;
-  constexpr int kWtJobs = 9 * 4 * CT, kWtIt = (kWtJobs + kBlock - 1) / kBlock;
-  const int CB = a.cin / 8;
   uint32_t ok = 0;
 #pragma unroll
   for (int k = 0; k < kWinIt; ++k) {
@@ -235,6 +233,31 @@ __device__ __forceinline__ uint32_t conv_stage_load(
         a.in + (img_base + (long long)yc * a.w + xc) * a.in_stride +
         a.in_off + c32 * 32 + gg * 8);
   }
+  return ok;
+}
+
+// The window part of conv_stage_store (win: [WH][WW][4]).
+template <int WH, int WW>
+__device__ __forceinline__ void conv_win_store(
+    const u32x4 (&wv)[(WH * WW * 4 + kBlock - 1) / kBlock], uint32_t ok,
+    u32x4* win) {
+  constexpr int kWinJobs = WH * WW * 4, kWinIt = (kWinJobs + kBlock - 1) / kBlock;
+#pragma unroll
+  for (int k = 0; k < kWinIt; ++k) {
+    const int job = threadIdx.x + k * kBlock;
+    if (job < kWinJobs)
+      win[job] = ((ok >> k) & 1u) ? wv[k] : u32x4{0u, 0u, 0u, 0u};
+  }
+}
+
+template <int WH, int WW, int CT>
+__device__ __forceinline__ uint32_t conv_stage_load(
+    const ConvArgs& a, long long img_base, int row0, int col0, int co0,
+    int c32, u32x4 (&wv)[(WH * WW * 4 + kBlock - 1) / kBlock],
+    u32x4 (&wt)[(9 * 4 * CT + kBlock - 1) / kBlock]) {
+  constexpr int kWtJobs = 9 * 4 * CT, kWtIt = (kWtJobs + kBlock - 1) / kBlock;
+  const int CB = a.cin / 8;
+  const uint32_t ok = conv_win_load<WH, WW>(a, img_base, row0, col0, c32, wv);
 #pragma unroll
   for (int k = 0; k < kWtIt; ++k) {
     const int jj = min((int)threadIdx.x + k * kBlock, kWtJobs - 1);
@@ -251,15 +274,8 @@ __device__ __forceinline__ void conv_stage_store(
     const u32x4 (&wv)[(WH * WW * 4 + kBlock - 1) / kBlock],
     const u32x4 (&wt)[(9 * 4 * CT + kBlock - 1) / kBlock], uint32_t ok,
     u32x4 (&win)[WH][WW][4], u32x4 (&ws)[9][4][CT]) {
-  constexpr int kWinJobs = WH * WW * 4, kWinIt = (kWinJobs + kBlock - 1) / kBlock;
   constexpr int kWtJobs = 9 * 4 * CT, kWtIt = (kWtJobs + kBlock - 1) / kBlock;
-#pragma unroll
-  for (int k = 0; k < kWinIt; ++k) {
-    const int job = threadIdx.x + k * kBlock;
-    if (job < kWinJobs)
-      win[(job >> 2) / WW][(job >> 2) % WW][job & 3] =
-          ((ok >> k) & 1u) ? wv[k] : u32x4{0u, 0u, 0u, 0u};
-  }
+  conv_win_store<WH, WW>(wv, ok, &win[0][0][0]);
 #pragma unroll
   for (int k = 0; k < kWtIt; ++k) {
     const int job = threadIdx.x + k * kBlock;
@@ -270,76 +286,50 @@ __device__ __forceinline__ void conv_stage_store(
   }
 }
 
+// The MFMAs of one 32-channel stage: win = the staged input window
+// [WH][WW][4] (lane group g holds channels 8g .. 8g+7), ws = the stage's
+// weights [9][4][CT].  Wave `wave` owns output rows wave MT .. wave MT +
+// MT - 1; every tap reads a shifted view of the window, and every output
+// tile accumulates the taps in the order 0 .. 8 (dy outer), whatever MT.
+// (Taps column by column, each loaded window row serving the up to three
+// output rows it neighbours, read fewer LDS words but hold three taps'
+// weights at once: 192 instead of 160 VGPRs for MT = 2, NT = 4, one wave a
+// SIMD less.)
 template <int MT, int NT>
-__global__ void __launch_bounds__(kBlock) conv3x3_lds_kernel(ConvArgs a) {
-  constexpr int TH = 4 * MT, TW = 16, WH = TH + 2, WW = TW + 2;
-  constexpr int CT = 16 * NT;
-  constexpr int kWinJobs = WH * WW * 4, kWinIt = (kWinJobs + kBlock - 1) / kBlock;
-  constexpr int kWtJobs = 9 * 4 * CT, kWtIt = (kWtJobs + kBlock - 1) / kBlock;
-  // staging arrays, reused for the output tile after the last barrier
-  struct Stage {
-    u32x4 win[WH][WW][4];
-    u32x4 ws[9][4][CT];
-  };
-  constexpr int TS = CT + 4;  // padded output row: lane groups on distinct banks
-  constexpr int kOutBytes = TH * TW * TS * 4;
-  __shared__ __attribute__((aligned(16)))
-  unsigned char smem[sizeof(Stage) > kOutBytes ? sizeof(Stage) : kOutBytes];
-  auto& win = reinterpret_cast<Stage*>(smem)->win;
-  auto& ws = reinterpret_cast<Stage*>(smem)->ws;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
-  const int g = lane >> 4, i = lane & 15;
-  const int tiles_x = (a.w + TW - 1) / TW, tiles_y = (a.h + TH - 1) / TH;
-  int tile, cob;
-  xcd_group(blockIdx.x, a.cout / CT, tile, cob);
-  if (tile >= a.n * tiles_x * tiles_y) return;  // whole block, before any barrier
-  const int img = tile / (tiles_x * tiles_y);
-  const int trem = tile % (tiles_x * tiles_y);
-  const int row0 = (trem / tiles_x) * TH, col0 = (trem % tiles_x) * TW;
-  const int co0 = cob * CT;
-  const long long img_base = (long long)img * a.h * a.w;
-  u32x4 wv[kWinIt], wt[kWtIt];
-  f32x4 acc[MT][NT];
+__device__ __forceinline__ void conv_taps(f32x4 (&acc)[MT][NT],
+                                          const u32x4* win, const u32x4* ws,
+                                          int wave, int g, int i) {
+  constexpr int WW = 18, CT = 16 * NT;
 #pragma unroll
-  for (int mt = 0; mt < MT; ++mt)
+  for (int tap = 0; tap < 9; ++tap) {
+    const int dy = tap / 3, dx = tap % 3;
+    bf16x8 A[MT], B[NT];
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-
-  const int nc32 = a.cin / 32;
-  uint32_t ok = conv_stage_load<WH, WW, CT>(a, img_base, row0, col0, co0, 0,
-                                            wv, wt);
-  for (int c32 = 0; c32 < nc32; ++c32) {
-    conv_stage_store<WH, WW, CT>(wv, wt, ok, win, ws);
-    __syncthreads();
-    if (c32 + 1 < nc32)
-      ok = conv_stage_load<WH, WW, CT>(a, img_base, row0, col0, co0, c32 + 1,
-                                       wv, wt);
+    for (int nt = 0; nt < NT; ++nt)
+      B[nt] = __builtin_bit_cast(bf16x8, ws[(tap * 4 + g) * CT + nt * 16 + i]);
 #pragma unroll
-    for (int tap = 0; tap < 9; ++tap) {
-      const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-      bf16x8 A[MT], B[NT];
+    for (int mt = 0; mt < MT; ++mt)
+      A[mt] = __builtin_bit_cast(
+          bf16x8, win[((wave * MT + mt + dy) * WW + i + dx) * 4 + g]);
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt)
-        B[nt] = __builtin_bit_cast(bf16x8, ws[tap][g][nt * 16 + i]);
-#pragma unroll
-      for (int mt = 0; mt < MT; ++mt)
-        A[mt] = __builtin_bit_cast(
-            bf16x8, win[wave * MT + mt + 1 + dy][i + 1 + dx][g]);
-#pragma unroll
-      for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt)
-          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-              A[mt], B[nt], acc[mt][nt], 0, 0, 0);
-    }
-    __syncthreads();
+        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            A[mt], B[nt], acc[mt][nt], 0, 0, 0);
   }
+}
 
-  // epilogue through LDS (the staging arrays are free after the last
-  // barrier): bias and ReLU applied per lane into a [pixel][channel] fp32
-  // tile, then 16-byte stores of 8 bf16 or 4 fp32 channels per thread
-  float* T = reinterpret_cast<float*>(smem);
+// Epilogue of the LDS-tiled convolutions, through the fp32 output tile T
+// ([pixel][channel], rows padded to CT + 4 floats; it may alias the
+// staging arrays once every wave is past their last read): bias and ReLU
+// per lane into T, then 16-byte stores of 8 bf16 or 4 fp32 channels per
+// thread, with the folded pooling / upsampling / ReLU-mask outputs.
+template <int MT, int NT>
+__device__ __forceinline__ void conv_epilogue(
+    const ConvArgs& a, const f32x4 (&acc)[MT][NT], float* T, int wave, int g,
+    int i, int img, int row0, int col0, int co0, long long img_base) {
+  constexpr int TH = 4 * MT, TW = 16, CT = 16 * NT, TS = CT + 4;
   const bool up = a.up_out && co0 < a.up_ch;
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) {
@@ -509,6 +499,60 @@ __global__ void __launch_bounds__(kBlock) conv3x3_lds_kernel(ConvArgs a) {
     }
   }
 }
+
+template <int MT, int NT>
+__global__ void __launch_bounds__(kBlock) conv3x3_lds_kernel(ConvArgs a) {
+  constexpr int TH = 4 * MT, TW = 16, WH = TH + 2, WW = TW + 2;
+  constexpr int CT = 16 * NT;
+  constexpr int kWinJobs = WH * WW * 4, kWinIt = (kWinJobs + kBlock - 1) / kBlock;
+  constexpr int kWtJobs = 9 * 4 * CT, kWtIt = (kWtJobs + kBlock - 1) / kBlock;
+  // staging arrays, reused for the output tile after the last barrier
+  struct Stage {
+    u32x4 win[WH][WW][4];
+    u32x4 ws[9][4][CT];
+  };
+  constexpr int TS = CT + 4;  // padded output row: lane groups on distinct banks
+  constexpr int kOutBytes = TH * TW * TS * 4;
+  __shared__ __attribute__((aligned(16)))
+  unsigned char smem[sizeof(Stage) > kOutBytes ? sizeof(Stage) : kOutBytes];
+  auto& win = reinterpret_cast<Stage*>(smem)->win;
+  auto& ws = reinterpret_cast<Stage*>(smem)->ws;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int g = lane >> 4, i = lane & 15;
+  const int tiles_x = (a.w + TW - 1) / TW, tiles_y = (a.h + TH - 1) / TH;
+  int tile, cob;
+  xcd_group(blockIdx.x, a.cout / CT, tile, cob);
+  if (tile >= a.n * tiles_x * tiles_y) return;  // whole block, before any barrier
+  const int img = tile / (tiles_x * tiles_y);
+  const int trem = tile % (tiles_x * tiles_y);
+  const int row0 = (trem / tiles_x) * TH, col0 = (trem % tiles_x) * TW;
+  const int co0 = cob * CT;
+  const long long img_base = (long long)img * a.h * a.w;
+  u32x4 wv[kWinIt], wt[kWtIt];
+  f32x4 acc[MT][NT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  const int nc32 = a.cin / 32;
+  uint32_t ok = conv_stage_load<WH, WW, CT>(a, img_base, row0, col0, co0, 0,
+                                            wv, wt);
+  for (int c32 = 0; c32 < nc32; ++c32) {
+    conv_stage_store<WH, WW, CT>(wv, wt, ok, win, ws);
+    __syncthreads();
+    if (c32 + 1 < nc32)
+      ok = conv_stage_load<WH, WW, CT>(a, img_base, row0, col0, co0, c32 + 1,
+                                       wv, wt);
+    conv_taps<MT, NT>(acc, &win[0][0][0], &ws[0][0][0], wave, g, i);
+    __syncthreads();
+  }
+
+  conv_epilogue<MT, NT>(a, acc, reinterpret_cast<float*>(smem), wave, g, i,
+                        img, row0, col0, co0, img_base);
+}
+
 
 // Weight gradient: a block owns 32 input x 32 output channels for all nine
 // taps and a run of 32-pixel chunks.  Each chunk's shifted inputs

@@ -148,3 +148,4 @@ def test_pipelined_steps_match_train_step(unet):
     b.finish(inflight)
   sa, sb = a.stats(), b.stats()
   assert sb["conv_flops"] == sa["conv_flops"] and sb["ms"] > 0
+
