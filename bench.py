@@ -421,7 +421,7 @@ def main():
     # the warm start's network: one Adam step of UNet(64, 256, 256) in bf16
     # on the MFMA units (dense bf16 peak 2.5 PFLOP/s)
     from fpie_amd import unet
-    line["unet_mfma"] = unet.measure(64, 256, 5)
+    line["unet_mfma"] = unet.measure(64, 256, 20)
   if not a.no_cpu_baseline:
     line["cpu_baseline"] = cpu_baseline(w, a.cpu_budget)
   print(json.dumps(line), flush=True)
