@@ -150,3 +150,34 @@ def test_packaged_network_beats_plain_jacobi(ws_mod):
     assert info["sweeps"] + n_warm < n_cold / 10, (info, n_warm, n_cold)
     assert err_warm < err_cold, (err_warm, err_cold)
     assert err_warm < runs["none"][2] / 10, (err_warm, runs["none"][2])
+
+
+def test_packaged_network_at_1024(ws_mod):
+  """The study's 1024^2 case (seed 900001, gradient src, 406,377 solved
+  pixels): the packaged network's warm start ends below the tolerance in a
+  tenth of the cold sweeps and closer to the exact solution (the oracle's
+  sparse direct solve) than the cold run."""
+  import coarse_oracle as co
+  from fpie_amd import synth
+  mask, tgt, grad, mode = synth.composite((1024, 1024), 900_001)
+  m = mask != 0
+  print(f"1024^2 ({mode}): exact solution (CPU)", flush=True)
+  x_star = co.exact_solution(mask, tgt, grad)
+
+  def err(s):
+    return float(np.abs(s.state() - x_star)[m].max())
+
+  s = _solver(mask, tgt, grad)
+  n_cold, _ = s.solve_to_residual(1e-3, 256, 10**8)
+  err_cold = err(s)
+  ws = ws_mod.WarmStart(ws_mod.load_params())
+  s = _solver(mask, tgt, grad)
+  info = ws.run(s, mask, tgt)
+  n_warm, _ = s.solve_to_residual(1e-3, 256, 10**8)
+  err_warm = err(s)
+  print(f"1024^2: sweeps cold {n_cold}, warm {info['sweeps']} + {n_warm}; "
+        f"max error vs exact: cold {err_cold:.4g} warm {err_warm:.4g}",
+        flush=True)
+  assert info["used"]
+  assert info["sweeps"] + n_warm < n_cold / 10, (info, n_warm, n_cold)
+  assert err_warm < err_cold, (err_warm, err_cold)
