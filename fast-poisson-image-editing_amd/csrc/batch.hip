@@ -49,6 +49,9 @@ constexpr double kSoloShare = 0.5;
 struct ItemBuffers {
   DevBuf<float> xa, xb, g, x0;  // x0: initial state, for restarts
   DevBuf<uint8_t> m;
+  // solved-pixel runs of each row, [begin, end) column pairs: row r's runs
+  // are run[run0[r] .. run0[r + 1]) (for the task lists of any tile)
+  std::vector<int> run0, run;
   int H = 0, W = 0, P = 0;
   long long plane = 0, masked = 0;
   long long bufsz = 0, mbytes = 0;
@@ -68,6 +71,8 @@ struct pie_batch {
   std::vector<BatchItem> desc;  // descriptors as last read back
   DevBuf<BatchItem> d_items;
   DevBuf<int> d_task_item, d_solo_items;
+  DevBuf<int> d_tlist;  // every item's task list
+  int ntask_all = 0;    // wave tasks with work, all items
   // [0] shared-task queue, [1] live items, [2] solo-item queue,
   // [3] live shared items
   DevBuf<unsigned> d_counters;
@@ -108,15 +113,41 @@ int prepare(pie_batch* b) {
   hipDeviceProp_t prop;
   PIE_HIP(hipGetDeviceProperties(&prop, b->device));
   b->blocks = std::max(1, per_cu * prop.multiProcessorCount);
+  // Task lists: the wave tasks whose written region (rows seg * R ..,
+  // columns stripe * U ..) holds a solved pixel, as ids group * stripes *
+  // segs + seg * stripes + stripe; the others have nothing to change
+  // (both state buffers hold the initial state at unsolved pixels).  All
+  // items' lists are concatenated, item i's at tlist0[i].
+  const int U = pie::kBatchLaneColumns * b->cfg.C - 2 * b->cfg.S;
+  const int R = b->cfg.seg_rows;
+  const int groups = pie::tile_groups(b->cfg);
+  std::vector<int> tlist, tlist0(n + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    const ItemBuffers& it = *b->items[i];
+    int stripes = 0, segs = 0;
+    pie::tile_tasks(b->cfg, it.H, it.W, &stripes, &segs);
+    std::vector<uint8_t> flag((size_t)stripes * segs, 0);
+    for (int r = 0; r < it.H; ++r) {
+      uint8_t* f = flag.data() + (size_t)(r / R) * stripes;
+      for (int k = it.run0[r]; k < it.run0[r + 1]; k += 2)
+        for (int st = it.run[k] / U; st <= (it.run[k + 1] - 1) / U; ++st)
+          f[st] = 1;
+    }
+    const int per_group = stripes * segs;
+    for (int g = 0; g < groups; ++g)
+      for (int region = 0; region < per_group; ++region)
+        if (flag[region]) tlist.push_back(g * per_group + region);
+    tlist0[i + 1] = (int)tlist.size();
+  }
+  PIE_HIP(b->d_tlist.alloc(std::max<size_t>(tlist.size(), 1)));
+  b->ntask_all = (int)tlist.size();
   // work of an item: wave tasks x launches of its sweep budget
   std::vector<long long> work(n);
   double total = 0.0;
   for (int i = 0; i < n; ++i) {
-    int stripes = 0, segs = 0;
-    pie::tile_tasks(b->cfg, b->items[i]->H, b->items[i]->W, &stripes, &segs);
     const long long launches =
         (b->items[i]->max_sweeps + b->cfg.S - 1) / b->cfg.S;
-    work[i] = (long long)stripes * segs * pie::tile_groups(b->cfg) * launches;
+    work[i] = (long long)(tlist0[i + 1] - tlist0[i]) * launches;
     total += (double)work[i];
   }
   const double solo_work = kSoloShare * total / b->blocks;
@@ -133,10 +164,12 @@ int prepare(pie_batch* b) {
     d.P = it.P;
     pie::tile_tasks(b->cfg, it.H, it.W, &d.stripes, &d.segs);
     d.task0 = (int)task_item.size();
+    d.tasks = b->d_tlist.p + tlist0[i];
+    d.ntask = tlist0[i + 1] - tlist0[i];
     d.t_end = it.max_sweeps;
     d.max_sweeps = it.max_sweeps;
     d.tol = it.tol;
-    const int tasks = d.stripes * d.segs * pie::tile_groups(b->cfg);
+    const int tasks = d.ntask;
     d.solo = tasks <= kSoloTasks || (double)work[i] <= solo_work;
     if (d.solo) {
       solo.push_back({work[i], i});
@@ -157,6 +190,9 @@ int prepare(pie_batch* b) {
   if (b->ntask)
     PIE_CHECK(upload(task_item.data(), sizeof(int) * task_item.size(),
                      b->d_task_item.p, b->stream));
+  if (!tlist.empty())
+    PIE_CHECK(upload(tlist.data(), sizeof(int) * tlist.size(), b->d_tlist.p,
+                     b->stream));
   if (b->nsolo)
     PIE_CHECK(upload(solo_items.data(), sizeof(int) * solo_items.size(),
                      b->d_solo_items.p, b->stream));
@@ -336,6 +372,21 @@ int pie_batch_add(pie_batch* b, int h, int w, const int32_t* mask,
     return fail(PIE_EINVAL,
                 "mask has solved pixels on the outermost rows/columns");
   it->masked = (long long)cnt[0];
+  it->run0.assign(h + 1, 0);
+  for (int r = 0; r < h; ++r) {
+    const int32_t* row = mask + (size_t)r * w;
+    for (int c = 0; c < w;) {
+      if (!row[c]) {
+        ++c;
+        continue;
+      }
+      const int c0 = c;
+      while (c < w && row[c]) ++c;
+      it->run.push_back(c0);
+      it->run.push_back(c);
+    }
+    it->run0[r + 1] = (int)it->run.size();
+  }
   PIE_CHECK(upload(tgt, px * 12, stage.p, b->stream));
   PIE_HIP(pie::launch_deinterleave(reinterpret_cast<const float*>(stage.p),
                                    it->x0.p + pie::kPadLeft, it->plane, h, w,
@@ -543,7 +594,7 @@ int pie_batch_size(pie_batch* b, int* items, int* tasks, long long* masked) {
   long long total = 0;
   for (auto& it : b->items) total += it->masked;
   if (items) *items = (int)b->items.size();
-  if (tasks) *tasks = b->ntask;
+  if (tasks) *tasks = b->ntask_all;
   if (masked) *masked = total;
   return PIE_OK;
 }

@@ -56,7 +56,9 @@ struct BatchItem {
   const uint8_t* m;
   long long plane;
   int H, W, P;
-  int stripes, segs, task0;  // wave tasks of the item, first global task
+  int stripes, segs, task0;  // tile geometry, first global (shared) task
+  const int* tasks;          // the item's wave tasks with work: ids group *
+  int ntask;                 // stripes * segs + seg * stripes + stripe
   int t_end;                 // sweeps never run past this step count
   int max_sweeps;            // stop rule: sweep budget
   float tol;                 // stop rule: max |update| of the last sweep
@@ -69,6 +71,9 @@ struct BatchItem {
   int solo;                  // 1: advanced by one work-group inside the
                              // sweep kernel, not by the epilogue
 };
+
+// columns of a wave (lanes) in the batch's task geometry
+constexpr int kBatchLaneColumns = 64;
 
 struct BatchArgs {
   BatchItem* items;

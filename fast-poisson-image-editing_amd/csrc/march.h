@@ -742,9 +742,9 @@ __global__ void __launch_bounds__(kBatchBlock) march_batch_kernel(BatchArgs b) {
     while (!it.done) {
       const MarchArgs a = batch_launch(it, S, b.seg_rows);
       const int per_group = it.stripes * it.segs;
-      const int ntask = per_group * (3 / NCH);
-      for (int t = wave; t < ntask; t += kW) {
-        const int group = t / per_group, rem = t % per_group;
+      for (int t = wave; t < it.ntask; t += kW) {
+        const int id = it.tasks[t];
+        const int group = id / per_group, rem = id % per_group;
         march_task<S, C, NCH, SCHED, ORDER, true, kW, kBatchPF>(
             a, rem % it.stripes, rem / it.stripes, group, &it.dmax_bits);
       }
@@ -765,16 +765,16 @@ __global__ void __launch_bounds__(kBatchBlock) march_batch_kernel(BatchArgs b) {
     if (lane == 0) t = (int)atomicAdd(b.queue, 1u);
     t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, kWave));
     if (t >= b.ntask) break;
-    const int id = b.task_item[t];
-    const BatchItem& it = b.items[id];
+    const int item = b.task_item[t];
+    const BatchItem& it = b.items[item];
     if (it.done) continue;
     const MarchArgs a = batch_launch(it, S, b.seg_rows);
-    const int local = t - it.task0;
+    const int id = it.tasks[t - it.task0];
     const int per_group = it.stripes * it.segs;
-    const int group = local / per_group, rem = local % per_group;
+    const int group = id / per_group, rem = id % per_group;
     march_task<S, C, NCH, SCHED, ORDER, true, kW, kBatchPF>(
         a, rem % it.stripes, rem / it.stripes, group,
-        &b.items[id].dmax_bits);
+        &b.items[item].dmax_bits);
   }
 }
 

@@ -282,6 +282,8 @@ int pie_batch_item(pie_batch* b, int i, int* sweeps, float* last_dmax,
 /* Outputs of item i as pie_step / pie_get_state (any may be NULL). */
 int pie_batch_read(pie_batch* b, int i, uint8_t* img, float* err,
                    float* state);
+/* Items, wave tasks of one launch over every item (the tile's tasks whose
+ * region holds a solved pixel) and solved pixels of the batch. */
 int pie_batch_size(pie_batch* b, int* items, int* tasks, long long* masked);
 
 /* ---- allocation-free solve: no counterpart in the reference ------------
