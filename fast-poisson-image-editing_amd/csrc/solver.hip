@@ -388,39 +388,12 @@ int time_tile(pie_solver* s, const TileCfg& c, int launches, int band,
                            band);
   PIE_CHECK(task_list(s, c, &a.tasks, &a.ntask));
   PIE_HIP(pie::launch_march(c, s->sched, s->eff_order(), a, s->stream));
+  PIE_HIP(hipEventRecord(s->ev0, s->stream));
+  for (int l = 0; l < launches; ++l)
+    PIE_HIP(pie::launch_march(c, s->sched, s->eff_order(), a, s->stream));
+  PIE_HIP(hipEventRecord(s->ev1, s->stream));
+  PIE_HIP(hipEventSynchronize(s->ev1));
   float ms = 0.0f;
-  if ((long long)s->H * s->W < (1LL << 22)) {
-    // small grids run their launches from captured graphs (run_sweeps):
-    // time them the same way, without the host's per-launch cost, and
-    // over enough launches for the few-microsecond ones
-    launches = std::max(launches, 32);
-    PIE_HIP(hipStreamBeginCapture(s->stream,
-                                  hipStreamCaptureModeThreadLocal));
-    hipError_t e = hipSuccess;
-    for (int l = 0; l < launches && e == hipSuccess; ++l)
-      e = pie::launch_march(c, s->sched, s->eff_order(), a, s->stream);
-    hipGraph_t g = nullptr;
-    const hipError_t e2 = hipStreamEndCapture(s->stream, &g);
-    hipGraphExec_t ge = nullptr;
-    if (e == hipSuccess && e2 == hipSuccess)
-      e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-    if (e == hipSuccess) e = hipGraphLaunch(ge, s->stream);  // warm
-    if (e == hipSuccess) e = hipEventRecord(s->ev0, s->stream);
-    if (e == hipSuccess) e = hipGraphLaunch(ge, s->stream);
-    if (e == hipSuccess) e = hipEventRecord(s->ev1, s->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(s->ev1);
-    if (ge) (void)hipGraphExecDestroy(ge);
-    if (g) (void)hipGraphDestroy(g);
-    if (e != hipSuccess || e2 != hipSuccess)
-      return fail(PIE_EHIP, std::string("autotune graph: ") +
-                                hipGetErrorString(e != hipSuccess ? e : e2));
-  } else {
-    PIE_HIP(hipEventRecord(s->ev0, s->stream));
-    for (int l = 0; l < launches; ++l)
-      PIE_HIP(pie::launch_march(c, s->sched, s->eff_order(), a, s->stream));
-    PIE_HIP(hipEventRecord(s->ev1, s->stream));
-    PIE_HIP(hipEventSynchronize(s->ev1));
-  }
   PIE_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
   *per_sweep = ms / (launches * c.S);
   return PIE_OK;
