@@ -672,11 +672,11 @@ __global__ void __launch_bounds__(kBlock)
                                              rem / a.stripes, group, nullptr);
 }
 
-// Work-groups of the persistent batch kernel: 16 waves (one work-group a
-// CU), with ring slots capped so that 16 rings fit the LDS.
+// Work-groups of the persistent batch kernel: 16 waves, with four ring
+// slots a wave (as the sweep kernel) so that 16 rings fit the LDS.
 constexpr int kBatchWaves = kBatchGroupWaves;
 constexpr int kBatchBlock = kWave * kBatchWaves;
-constexpr int kBatchPF = 6;
+constexpr int kBatchPF = 4;
 
 // Sweep-kernel arguments of one launch of a batch item.
 __device__ __forceinline__ MarchArgs batch_launch(const BatchItem& it, int S,
