@@ -380,17 +380,26 @@ constexpr long long kGraphMinLaunches = 16;
 constexpr size_t kMaxGraphs = 8;
 
 // Time `launches` launches of tile c on the loaded problem (the state buffer
-// is only read; the spare buffer takes the output): milliseconds per sweep.
-int time_tile(pie_solver* s, const TileCfg& c, int launches, int band,
-              float* per_sweep) {
-  // t0 = H + W puts every pixel inside the wavefront band of gs_lex
-  MarchArgs a = march_args(s, s->state(), s->other(), s->H + s->W, 1 << 30,
-                           band);
+// is only read; the spare buffer takes the output): milliseconds per step.
+// The launches start at steps spread evenly over a call of `steps` steps,
+// so that a gs_lex call's fill and drain (tasks outside the wavefront band
+// skip their rows) weigh as they do in the call.
+int time_tile(pie_solver* s, const TileCfg& c, int launches, int steps,
+              int band, float* per_sweep) {
+  const int last = std::max(0, steps - c.S);
+  auto start = [&](int l) {
+    const long long t = (2LL * l + 1) * steps / (2LL * launches);
+    return (int)std::min<long long>(t / c.S * c.S, last);
+  };
+  MarchArgs a = march_args(s, s->state(), s->other(), start(0),
+                           std::max(steps, c.S), band);
   PIE_CHECK(task_list(s, c, &a.tasks, &a.ntask));
   PIE_HIP(pie::launch_march(c, s->sched, s->eff_order(), a, s->stream));
   PIE_HIP(hipEventRecord(s->ev0, s->stream));
-  for (int l = 0; l < launches; ++l)
+  for (int l = 0; l < launches; ++l) {
+    a.t0 = a.t_lo = start(l);
     PIE_HIP(pie::launch_march(c, s->sched, s->eff_order(), a, s->stream));
+  }
   PIE_HIP(hipEventRecord(s->ev1, s->stream));
   PIE_HIP(hipEventSynchronize(s->ev1));
   float ms = 0.0f;
@@ -403,13 +412,15 @@ int time_tile(pie_solver* s, const TileCfg& c, int launches, int band,
 // the counts that fill one or two rounds of resident waves, timed once;
 // then the fastest kBest of them timed kReps more times each (launch times
 // vary by several per cent from run to run), keeping each one's best time.
-int autotune(pie_solver* s, int launches) {
+// Timed as launches of a call of `iters` sweeps (the call that triggers
+// the autotuner).
+int autotune(pie_solver* s, int launches, int iters) {
   if (!s->grid_layout) return PIE_OK;
   constexpr int kBest = 6, kReps = 3;
   int var[32][3];
   const int nv = std::min(pie::tile_variants(var, 32), 32);
   int steps, band;
-  schedule_steps(s, 1 << 20, &steps, &band);
+  schedule_steps(s, std::max(iters, 1), &steps, &band);
   std::vector<std::pair<float, TileCfg>> cand;
   for (int v = 0; v < nv; ++v) {
     TileCfg c0{var[v][0], var[v][1], 16, var[v][2]};
@@ -426,7 +437,7 @@ int autotune(pie_solver* s, int launches) {
       if (seen) continue;
       TileCfg c{var[v][0], var[v][1], rr, var[v][2]};
       float t = 0.0f;
-      PIE_CHECK(time_tile(s, c, std::max(2, launches / 2), band, &t));
+      PIE_CHECK(time_tile(s, c, std::max(2, launches / 2), steps, band, &t));
       cand.push_back({t, c});
     }
   }
@@ -436,7 +447,7 @@ int autotune(pie_solver* s, int launches) {
   for (auto& c : cand)
     for (int r = 0; r < kReps; ++r) {
       float t = 0.0f;
-      PIE_CHECK(time_tile(s, c.second, launches, band, &t));
+      PIE_CHECK(time_tile(s, c.second, launches, steps, band, &t));
       c.first = std::min(c.first, t);
     }
   std::sort(cand.begin(), cand.end(),
@@ -932,7 +943,7 @@ int pie_iterate(pie_solver* s, int iterations) {
   const long long px = (long long)s->H * s->W;
   if (s->grid_layout && !s->tuned && !s->tile_fixed &&
       (px >= (1LL << 22) || (long long)iterations * px >= (1LL << 29)))
-    PIE_CHECK(autotune(s, 6));
+    PIE_CHECK(autotune(s, 6, iterations));
   return run_sweeps(s, iterations);
 }
 
@@ -1332,7 +1343,7 @@ int pie_autotune(pie_solver* s, int launches) {
   if (!s->loaded) return fail(PIE_ESTATE, "autotune before reset");
   if (launches < 1) launches = 1;
   s->tile_fixed = false;
-  PIE_CHECK(autotune(s, launches));
+  PIE_CHECK(autotune(s, launches, 5000));
   return PIE_OK;
 }
 

@@ -176,7 +176,9 @@ int pie_get_tile(pie_solver* s, int* sweeps, int* cols_per_lane,
 int pie_set_tile_channels(pie_solver* s, int channels);
 int pie_get_tile_channels(pie_solver* s, int* channels);
 /* Time every candidate tile on the loaded problem (`launches` launches
- * each, state untouched) and keep the fastest. */
+ * each, starting at steps spread over a call of 5000 sweeps; state
+ * untouched) and keep the fastest.  pie_step / pie_iterate run the same
+ * search on their first long call, over that call's own steps. */
 int pie_autotune(pie_solver* s, int launches);
 /* Event timing of the sweep kernels on the solver's stream. */
 int pie_set_timing(pie_solver* s, int enable);
