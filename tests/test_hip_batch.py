@@ -168,3 +168,25 @@ def test_solo_and_shared_items_match_oracle(hip, oracle):
   lat = [b.item(i)["latency_s"] for i in range(len(items))]
   assert all(x >= 0.0 for x in lat)
   print(f"\nsizes {sizes}\nlatencies {lat}")
+
+
+def test_item_without_solved_pixels(hip, oracle):
+  """An item whose mask solves nothing has no wave task: it keeps its
+  initial state, stops at its budget (tol 0) or after one launch (tol > 0),
+  and the other items are unaffected."""
+  items = _items(3, 41)
+  m0, t0, g0 = items[0]
+  empty = np.zeros_like(m0)
+  b = hip.BatchSolver(sweeps_per_launch=8)
+  b.add(empty, t0, g0, max_sweeps=40, tol=0.0)
+  b.add(empty, t0, g0, max_sweeps=40, tol=1e-3)
+  for m, t, g in items[1:]:
+    b.add(m, t, g, max_sweeps=24, tol=0.0)
+  b.solve(rounds_per_graph=4)
+  want0 = np.ascontiguousarray(t0, np.float32)
+  assert b.item(0)["sweeps"] == 40 and b.item(1)["sweeps"] == 8
+  for i in (0, 1):
+    assert np.array_equal(bits(b.read(i, state=True)[2]), bits(want0))
+  for i, (m, t, g) in enumerate(items[1:], start=2):
+    want = oracle.grid_jacobi(m, t, g, 24, "ulrd")
+    assert np.array_equal(bits(b.read(i, state=True)[2]), bits(want)), i
