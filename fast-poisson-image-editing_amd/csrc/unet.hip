@@ -69,7 +69,7 @@ struct pie_unet {
   // fp32 host-facing buffers and gradients
   DevBuf<float> x32, out, dout, target;
   DevBuf<uint8_t> mask;
-  DevBuf<float> gC0, gC1, gC2;
+  DevBuf<bf16_t> gC0, gC1, gC2;
   DevBuf<bf16_t> dyb, dyb2;
   DevBuf<double> acc;
   double ms_total = 0.0, flops_total = 0.0;
@@ -125,9 +125,9 @@ int alloc_all(pie_unet* u) {
   PIE_HIP(u->mask.alloc(P0));
   // gradient scratch: the largest per-level working set
   long long ga = std::max({P0 * 32, P1 * 64, P2 * 128, P3 * 256});
-  PIE_HIP(u->gC0.alloc(P0 * 96));
-  PIE_HIP(u->gC1.alloc(P1 * 192));
-  PIE_HIP(u->gC2.alloc(P2 * 384));
+  PIE_HIP(u->gC0.alloc(P0 * 32));
+  PIE_HIP(u->gC1.alloc(P1 * 64));
+  PIE_HIP(u->gC2.alloc(P2 * 128));
   PIE_HIP(u->dyb.alloc(ga));
   PIE_HIP(u->dyb2.alloc(ga));
   PIE_HIP(u->acc.alloc(2));
@@ -209,13 +209,14 @@ int forward(pie_unet* u, const float* x) {
 // already in dyb (bf16, masked by lb's ReLU): lb's input is la's ReLU output
 // xb, and nothing else reads it, so lb's data gradient is written by the
 // convolution's epilogue as la's output gradient, masked by xb > 0 and
-// rounded to bf16 once from the fp32 sum.  la's data gradient (fp32) goes to
-// dx unless dx is null; with up, its first up->ch channels (an upsampled
-// input) go through the upsampling backward into up->out instead; with pb
-// (an average-pooled input), all of it goes through the pooling backward
-// into pb->out instead of dx.
+// rounded to bf16 once from the fp32 sum.  With up, la's data gradient's
+// first up->ch channels (an upsampled input) go through the upsampling
+// backward into up->out and the others (the skip channels) to dx as bf16
+// [P][cin - up->ch]; with pb (an average-pooled input), all of it goes
+// through the pooling backward into pb->out; with neither, it is not
+// computed (la reads the network input).
 int conv_pair_bwd(pie_unet* u, int la, int lb, const Slice& xb,
-                  const Slice& xa, float* dx,
+                  const Slice& xa, bf16_t* dx,
                   const pie::unet::UpSum* up = nullptr,
                   const pie::unet::PoolBwd* pb = nullptr) {
   const ConvSpec& ca = kConvs[la];
@@ -235,13 +236,15 @@ int conv_pair_bwd(pie_unet* u, int la, int lb, const Slice& xb,
   PIE_HIP(pie::unet::conv3x3_wgrad(xa, u->B, h, w, u->dyb2.p, ca.cout,
                                    u->grad.p + u->woff[la],
                                    u->grad.p + u->boff[la], st));
-  if (dx || pb) {
+  if (up || pb) {
     pie::unet::ConvFolds fa;
     fa.up = up;
     fa.pool_bwd = pb;
+    const int skip = up ? ca.cin - up->ch : 0;
+    const Slice dxs{dx, skip, -(up ? up->ch : 0), skip};
     PIE_HIP(pie::unet::conv3x3(dya, u->B, h, w, u->wkt[la].p, ca.cin, nullptr,
-                               false, nullptr, pb ? nullptr : dx, ca.cin,
-                               false, st, fa));
+                               false, up ? &dxs : nullptr, nullptr, 0, false,
+                               st, fa));
   }
   return PIE_OK;
 }
@@ -271,15 +274,15 @@ int backward(pie_unet* u) {
   // data gradient's epilogue adds it (a quarter to each of the four pixels)
   // to the skip channels' gradient from the decoder and applies the ReLU of
   // the finer level's output
-  const pie::unet::PoolBwd pb2{dy, u->gC2.p, 384, 256,
+  const pie::unet::PoolBwd pb2{dy, u->gC2.p, 128, 0,
                                sl(u->c2, 384, 256, 128)};
   PIE_CHECK(conv_pair_bwd(u, BNA, BNB, sl(u->bna, 256, 0, 256),
                           sl(u->p2, 128, 0, 128), nullptr, nullptr, &pb2));
-  const pie::unet::PoolBwd pb1{dy, u->gC1.p, 192, 128,
+  const pie::unet::PoolBwd pb1{dy, u->gC1.p, 64, 0,
                                sl(u->c1, 192, 128, 64)};
   PIE_CHECK(conv_pair_bwd(u, E2A, E2B, sl(u->e2a, 128, 0, 128),
                           sl(u->p1, 64, 0, 64), nullptr, nullptr, &pb1));
-  const pie::unet::PoolBwd pb0{dy, u->gC0.p, 96, 64, sl(u->c0, 96, 64, 32)};
+  const pie::unet::PoolBwd pb0{dy, u->gC0.p, 32, 0, sl(u->c0, 96, 64, 32)};
   PIE_CHECK(conv_pair_bwd(u, E1A, E1B, sl(u->e1a, 64, 0, 64),
                           sl(u->p0, 32, 0, 32), nullptr, nullptr, &pb0));
   PIE_CHECK(conv_pair_bwd(u, E0A, E0B, sl(u->e0a, 32, 0, 32),

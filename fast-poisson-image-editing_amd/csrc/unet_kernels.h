@@ -33,12 +33,14 @@ struct Slice {
 //   output is the mask;
 // - up (data gradient of a convolution whose input is a nearest 2x upsample
 //   in its first up->ch channels, concatenated with a skip connection): the
-//   first up->ch output channels are not written to out_f32 but summed over
+//   first up->ch output channels are not written to out_f32 or out_bf (one
+//   of the two takes the skip channels, out_bf's slice offset then counting
+//   from channel 0, so an offset of -up->ch starts them at 0) but summed over
 //   each 2 x 2 pixel block (fine pixels (0,0), (0,1), (1,0), (1,1) in that
 //   order), masked by up->mask > 0 of the half-resolution level and written
 //   as bf16 [n][h/2][w/2][up->ch] to up->out (the upsampling backward with
 //   the backward of the ReLU that made its input); up->ch a multiple of 64,
-//   h and w even, no bf16 output;
+//   h and w even, no mask;
 // - pool / upf (Cout channels each; needs out_bf): the bf16 output's 2 x 2
 //   average pool (the four bf16 values summed in the order (0,0), (0,1),
 //   (1,0), (1,1), times 1/4, rounded; h and w even) and its 2x nearest
@@ -47,7 +49,7 @@ struct Slice {
 //   average pool of a skip connection; no out_bf, no out_f32): for each of
 //   the four pixels q of the double-resolution level under output pixel p,
 //   out[q][co] = bf16(y[q][co] > 0 ? gs[q][co] + v[p][co] / 4 : 0) (gs: the
-//   skip channels' fp32 gradient from the decoder, y: the skip output),
+//   skip channels' bf16 gradient from the decoder, y: the skip output),
 //   out [n][2h][2w][Cout].
 struct UpSum {
   bf16_t* out;
@@ -56,7 +58,7 @@ struct UpSum {
 };
 struct PoolBwd {
   bf16_t* out;
-  const float* gs;
+  const bf16_t* gs;
   int gs_stride, gs_off;
   Slice y;
 };

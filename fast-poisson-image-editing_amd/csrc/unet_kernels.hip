@@ -95,7 +95,7 @@ struct ConvArgs {
   // upsampling backward folded in (LDS-tiled kernel): output channels below
   // up_ch are summed over 2 x 2 pixel blocks, masked by up_mask > 0 of the
   // half-resolution level and written as bf16 [n][h/2][w/2][up_ch] to up_out;
-  // the other channels go to out_f32 as usual
+  // the other channels go to out_f32 or out_bf as usual
   bf16_t* up_out;
   int up_ch;
   const bf16_t* up_mask;
@@ -110,7 +110,7 @@ struct ConvArgs {
   // four double-resolution pixels q under output pixel p, pb_out[q][co] =
   // bf16(pb_y[q][co] > 0 ? pb_gs[q][co] + v[p][co] / 4 : 0)
   bf16_t* pb_out;
-  const float* pb_gs;
+  const bf16_t* pb_gs;
   int pb_gs_stride, pb_gs_off;
   const bf16_t* pb_y;
   int pb_y_stride, pb_y_off;
@@ -345,7 +345,7 @@ __device__ __forceinline__ void conv_epilogue(
       }
   }
   __syncthreads();
-  if (a.out_bf) {
+  if (a.out_bf && !up) {
     constexpr int kJobs = TH * TW * (CT / 8);
 #pragma unroll
     for (int k = 0; k < (kJobs + kBlock - 1) / kBlock; ++k) {
@@ -471,12 +471,20 @@ __device__ __forceinline__ void conv_epilogue(
         const int c = co0 + c8 * 8;
 #pragma unroll
         for (int k2 = 0; k2 < 4; ++k2) {
-          const f32x4* gp = reinterpret_cast<const f32x4*>(
+          const u32x4 gs = *reinterpret_cast<const u32x4*>(
               a.pb_gs + q[k2] * a.pb_gs_stride + a.pb_gs_off + c);
           const u32x4 yv = *reinterpret_cast<const u32x4*>(
               a.pb_y + q[k2] * a.pb_y_stride + a.pb_y_off + c);
+          f32x4 g0, g1;
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            g0[2 * j] = __uint_as_float(gs[j] << 16);
+            g0[2 * j + 1] = __uint_as_float(gs[j] & 0xffff0000u);
+            g1[2 * j] = __uint_as_float(gs[j + 2] << 16);
+            g1[2 * j + 1] = __uint_as_float(gs[j + 2] & 0xffff0000u);
+          }
           *reinterpret_cast<u32x4*>(a.pb_out + q[k2] * a.cout + c) =
-              relu_mask_bf16(yv, gp[0] + v0, gp[1] + v1);
+              relu_mask_bf16(yv, g0 + v0, g1 + v1);
         }
       }
     }
@@ -1085,7 +1093,7 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
   if (up && (!up->out || up->ch <= 0 || up->ch % 64 || up->ch > cout ||
              up->mask.ch != up->ch || up->mask.stride % 8 ||
              up->mask.off % 8 || h % 2 || w % 2 || in.ch % 32 ||
-             cout % 32 || (up->ch < cout && !out_f32) || out_bf ||
+             cout % 32 || (up->ch < cout && !out_f32 == !out_bf) || mask ||
              accumulate))
     return hipErrorInvalidValue;
   for (const Slice* f : {pool_out, upf_out})
