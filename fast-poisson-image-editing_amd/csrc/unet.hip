@@ -170,8 +170,11 @@ int conv_fwd(pie_unet* u, int l, const Slice& x, const Slice& y,
 
 int forward(pie_unet* u, const float* x) {
   hipStream_t st = u->stream;
-  PIE_HIP(pie::unet::to_bf16(x, (int)u->P[0], 8, sl(u->in, 8, 0, 8), st));
-  PIE_CHECK(conv_fwd(u, E0A, sl(u->in, 8, 0, 8), sl(u->e0a, 32, 0, 32)));
+  // the first layer reads the fp32 input and leaves its bf16 copy in u->in
+  // for the weight gradient
+  PIE_HIP(pie::unet::conv3x3_in8(x, u->in.p, u->B, u->H, u->W, u->wk[E0A].p,
+                                 kConvs[E0A].cout, u->param.p + u->boff[E0A],
+                                 true, sl(u->e0a, 32, 0, 32), st));
   // encoder outputs go to the skip half of the concatenation buffers, with
   // their average pool for the next level; decoder level outputs to their
   // own buffers, with their upsample into the finer level's concatenation

@@ -91,9 +91,13 @@ hipError_t conv3x3_wgrad(const Slice& x, int n, int h, int w,
 hipError_t pack_weights(const float* w, int cin, int cout, bf16_t* wk,
                         bf16_t* wkt, hipStream_t st);
 
-// input fp32 [P][8] -> bf16 slice
-hipError_t to_bf16(const float* x, int p, int ch, const Slice& out,
-                   hipStream_t st);
+// The first convolution (8 input channels) straight from the fp32 input
+// x [P][8] (16-byte aligned): each value rounded to bf16 (nearest even;
+// written to x_bf [P][8] unless x_bf is null), then as conv3x3 with a bf16
+// output slice and no folds; Cout a multiple of 32.
+hipError_t conv3x3_in8(const float* x, bf16_t* x_bf, int n, int h, int w,
+                       const bf16_t* wk, int cout, const float* bias,
+                       bool relu, const Slice& out, hipStream_t st);
 // head: out[p][0..3) = sum_c x[p][c] * bf16(w[c][o]) + b[o]  (x bf16 32 ch)
 hipError_t head_fwd(const Slice& x, int p, const float* w, const float* b,
                     float* out, hipStream_t st);

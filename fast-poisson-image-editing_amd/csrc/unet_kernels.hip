@@ -114,6 +114,10 @@ struct ConvArgs {
   int pb_gs_stride, pb_gs_off;
   const bf16_t* pb_y;
   int pb_y_stride, pb_y_off;
+  // first layer (conv3x3_in8_kernel): fp32 input [P][8] instead of in, and
+  // its bf16 rounding written to in_copy [P][8] when in_copy is not null
+  const float* in_f32;
+  bf16_t* in_copy;
 };
 
 // Wave tile: MT x 16 pixels by NT x 16 output channels; a block is four
@@ -561,6 +565,116 @@ __global__ void __launch_bounds__(kBlock) conv3x3_lds_kernel(ConvArgs a) {
                         img, row0, col0, co0, img_base);
 }
 
+// First layer (8 input channels, fp32 input): a block computes a (4 MT) x
+// 16-pixel tile for 16 NT output channels.  The fp32 input window (tile
+// plus a one-pixel halo, zero outside the image) is rounded to bf16 and
+// staged in LDS once, one 16-byte word of 8 channels a pixel; the tile's
+// own pixels also go to in_copy (the bf16 layer input, which the weight
+// gradient reads).  K = 9 taps x 8 channels in the weights' order
+// (k = 8 tap + e), padded to three MFMA steps: step s, lane group g takes
+// tap 4 s + g, taps 9 .. 11 being zero.
+template <int MT, int NT>
+__global__ void __launch_bounds__(kBlock) conv3x3_in8_kernel(ConvArgs a) {
+  constexpr int TH = 4 * MT, TW = 16, WH = TH + 2, WW = TW + 2;
+  constexpr int CT = 16 * NT;
+  constexpr int kWinJobs = WH * WW, kWinIt = (kWinJobs + kBlock - 1) / kBlock;
+  constexpr int TS = CT + 4;
+  constexpr int kOutBytes = TH * TW * TS * 4, kWinBytes = kWinJobs * 16;
+  __shared__ __attribute__((aligned(16)))
+  unsigned char smem[kWinBytes > kOutBytes ? kWinBytes : kOutBytes];
+  u32x4* win = reinterpret_cast<u32x4*>(smem);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int g = lane >> 4, i = lane & 15;
+  const int tiles_x = (a.w + TW - 1) / TW, tiles_y = (a.h + TH - 1) / TH;
+  int tile, cob;
+  xcd_group(blockIdx.x, a.cout / CT, tile, cob);
+  if (tile >= a.n * tiles_x * tiles_y) return;  // whole block, before any barrier
+  const int img = tile / (tiles_x * tiles_y);
+  const int trem = tile % (tiles_x * tiles_y);
+  const int row0 = (trem / tiles_x) * TH, col0 = (trem % tiles_x) * TW;
+  const int co0 = cob * CT;
+  const long long img_base = (long long)img * a.h * a.w;
+
+  // window loads first (clamped addresses, all in flight together), then
+  // the weights of the three steps straight into registers
+  f32x4 lo[kWinIt], hi[kWinIt];
+  uint32_t ok = 0;
+#pragma unroll
+  for (int k = 0; k < kWinIt; ++k) {
+    const int jj = min((int)threadIdx.x + k * kBlock, kWinJobs - 1);
+    const int y = row0 - 1 + jj / WW, x = col0 - 1 + jj % WW;
+    if (y >= 0 && y < a.h && x >= 0 && x < a.w) ok |= 1u << k;
+    const int yc = min(max(y, 0), a.h - 1), xc = min(max(x, 0), a.w - 1);
+    const f32x4* src = reinterpret_cast<const f32x4*>(
+        a.in_f32 + (img_base + (long long)yc * a.w + xc) * 8);
+    lo[k] = src[0];
+    hi[k] = src[1];
+  }
+  bf16x8 B[3][NT];
+#pragma unroll
+  for (int s = 0; s < 3; ++s)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const int tap = min(4 * s + g, 8);
+      B[s][nt] = 4 * s + g < 9
+                     ? load8(a.wk + ((long long)tap * a.cout + co0 + nt * 16 +
+                                     i) * 8)
+                     : zero8();
+    }
+#pragma unroll
+  for (int k = 0; k < kWinIt; ++k) {
+    const int job = threadIdx.x + k * kBlock;
+    if (job < kWinJobs) {
+      u32x4 v = u32x4{0u, 0u, 0u, 0u};
+      if ((ok >> k) & 1u) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          v[j] = (uint32_t)f2bf(lo[k][2 * j]) |
+                 ((uint32_t)f2bf(lo[k][2 * j + 1]) << 16);
+          v[j + 2] = (uint32_t)f2bf(hi[k][2 * j]) |
+                     ((uint32_t)f2bf(hi[k][2 * j + 1]) << 16);
+        }
+        const int wy = job / WW, wx = job % WW;
+        if (a.in_copy && wy >= 1 && wy <= TH && wx >= 1 && wx <= TW)
+          *reinterpret_cast<u32x4*>(
+              a.in_copy + (img_base + (long long)(row0 - 1 + wy) * a.w +
+                           col0 - 1 + wx) * 8) = v;
+      }
+      win[job] = v;
+    }
+  }
+  __syncthreads();
+
+  f32x4 acc[MT][NT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    const int tap = min(4 * s + g, 8);
+    const int dy = tap / 3, dx = tap % 3;
+    bf16x8 A[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+      A[mt] = 4 * s + g < 9
+                  ? __builtin_bit_cast(
+                        bf16x8, win[(wave * MT + mt + dy) * WW + i + dx])
+                  : zero8();
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            A[mt], B[s][nt], acc[mt][nt], 0, 0, 0);
+  }
+  __syncthreads();  // the output tile reuses the window's LDS
+
+  conv_epilogue<MT, NT>(a, acc, reinterpret_cast<float*>(smem), wave, g, i,
+                        img, row0, col0, co0, img_base);
+}
+
 
 // Weight gradient: a block owns 32 input x 32 output channels for all nine
 // taps and a run of 32-pixel chunks.  Each chunk's shifted inputs
@@ -900,15 +1014,6 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 __global__ void __launch_bounds__(kBlock)
-    to_bf16_kernel(const float* __restrict__ x, long long p, int ch,
-                   bf16_t* __restrict__ out, int stride, int off) {
-  const long long n = p * ch;
-  for (long long id = blockIdx.x * (long long)kBlock + threadIdx.x; id < n;
-       id += (long long)gridDim.x * kBlock)
-    out[(id / ch) * stride + off + id % ch] = f2bf(x[id]);
-}
-
-__global__ void __launch_bounds__(kBlock)
     head_fwd_kernel(const bf16_t* __restrict__ x, int x_stride, int x_off,
                     long long p, const float* __restrict__ w,
                     const float* __restrict__ b, float* __restrict__ out) {
@@ -1229,11 +1334,34 @@ hipError_t pack_weights(const float* w, int cin, int cout, bf16_t* wk,
   return hipGetLastError();
 }
 
-hipError_t to_bf16(const float* x, int p, int ch, const Slice& out,
-                   hipStream_t st) {
-  hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for((long long)p * ch)),
-                     dim3(kBlock), 0, st, x, (long long)p, ch,
-                     static_cast<bf16_t*>(out.base), out.stride, out.off);
+hipError_t conv3x3_in8(const float* x, bf16_t* x_bf, int n, int h, int w,
+                       const bf16_t* wk, int cout, const float* bias,
+                       bool relu, const Slice& out, hipStream_t st) {
+  if (!x || !wk || cout % 32 || out.stride % 8 || out.off % 8 ||
+      reinterpret_cast<uintptr_t>(x) % 16 ||
+      reinterpret_cast<uintptr_t>(x_bf) % 16)
+    return hipErrorInvalidValue;
+  ConvArgs a{};
+  a.in_f32 = x;
+  a.in_copy = x_bf;
+  a.cin = 8;
+  a.n = n;
+  a.h = h;
+  a.w = w;
+  a.wk = wk;
+  a.cout = cout;
+  a.bias = bias;
+  a.relu = relu;
+  a.out_bf = static_cast<bf16_t*>(out.base);
+  a.out_stride = out.stride;
+  a.out_off = out.off;
+  // 16 x 16-pixel tiles, 32 output channels a block
+  constexpr int MT = 4, NT = 2;
+  const long long tiles = (long long)n * ((h + 4 * MT - 1) / (4 * MT)) *
+                          ((w + 15) / 16);
+  hipLaunchKernelGGL((conv3x3_in8_kernel<MT, NT>),
+                     dim3(xcd_grid(tiles, cout / (16 * NT))), dim3(kBlock), 0,
+                     st, a);
   return hipGetLastError();
 }
 
