@@ -322,10 +322,13 @@ struct TaskGeom {
 // lane's two columns may change at each level: both schedules update one
 // colour of the checkerboard per step, and with an even first column the
 // colour is the same for every lane of a row.  PI is the task's parity
-// (t0 - rb - 1) & 1; with BAND the gs_lex band test 0 <= t - i - j < band
-// is made per pixel, without it the whole task lies inside the band.
+// (t0 - rb - 1) & 1; with BAND (1 or 2) the gs_lex band test
+// 0 <= t - i - j < band is made per pixel, without it the whole task lies
+// inside the band; BAND 2: every step of the launch lies inside the call
+// (no per-level liveness masks: with the band test they no longer fit the
+// scalar registers).
 template <int S, int C, int NCH, int SCHED, int ORDER, bool DM, int PI,
-          bool BAND, int PFMAX>
+          int BAND, int PFMAX>
 __device__ __forceinline__ void march_rows(const MarchArgs& a,
                                            const TaskGeom& tg, float* ring,
                                            unsigned* dmax) {
@@ -376,7 +379,10 @@ __device__ __forceinline__ void march_rows(const MarchArgs& a,
                          a.plane, lane);
   });
 
-  for (int base = 0; base < nsteps; base += S) {
+  // gs_lex band test (BAND): the lane's t - r - col0 at level k of row
+  // step u of this iteration is dl + 2k - u; dlb = dl - band
+  int dl = a.t0 - 1 - rb - col0, dlb = dl - a.band;
+  for (int base = 0; base < nsteps; base += S, dl -= S, dlb -= S) {
     static_for<0, S>([&](auto uc) {
       constexpr int u = decltype(uc)::value;
       constexpr int q = u % PF;
@@ -400,7 +406,7 @@ __device__ __forceinline__ void march_rows(const MarchArgs& a,
         // Steps outside [t_lo, t_end) keep their value; folding that
         // (wave-uniform) test into the predicate keeps the level free of
         // branches, so the window registers are renamed rather than copied.
-        const bool live = t >= a.t_lo && t < a.t_end;
+        const bool live = BAND == 2 || (t >= a.t_lo && t < a.t_end);
         float out[C][NCH];
         if constexpr (COLOUR) {
           // the column of the lane that may change at this level: t - r
@@ -410,8 +416,10 @@ __device__ __forceinline__ void march_rows(const MarchArgs& a,
           constexpr int oth = 1 - act;
           bool upd = live && mb[slot][act];
           if constexpr (BAND) {
-            const int d = t - r - (col0 + act);
-            upd = upd && d >= 0 && d < a.band;
+            // d = t - r - (col0 + act) = dl + 2k - u - act: compared with
+            // a constant, so no per-level bound is held in a register
+            constexpr int c = u + act - 2 * k;
+            upd = upd && dl >= c && dlb < c;
           }
           float acc[NCH];
 #pragma unroll
@@ -623,24 +631,32 @@ __device__ __forceinline__ void march_task(const MarchArgs& a, int stripe,
       const int d_max = a.t0 + S - 1 - rb - tg.cu;
       band = !(d_min >= 0 && d_max < a.band);
     }
+    const bool all_live = a.t_lo <= a.t0 && a.t0 + S <= a.t_end;
+    auto rows = [&](auto pc, auto bc) {
+      march_rows<S, C, NCH, SCHED, ORDER, DM, decltype(pc)::value,
+                 decltype(bc)::value, PFMAX>(a, tg, ring, dmax);
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
     if (pi) {
-      if (band)
-        march_rows<S, C, NCH, SCHED, ORDER, DM, 1, true, PFMAX>(
-            a, tg, ring, dmax);
+      if (!band)
+        rows(I1{}, I0{});
+      else if (all_live)
+        rows(I1{}, I2{});
       else
-        march_rows<S, C, NCH, SCHED, ORDER, DM, 1, false, PFMAX>(
-            a, tg, ring, dmax);
+        rows(I1{}, I1{});
     } else {
-      if (band)
-        march_rows<S, C, NCH, SCHED, ORDER, DM, 0, true, PFMAX>(
-            a, tg, ring, dmax);
+      if (!band)
+        rows(I0{}, I0{});
+      else if (all_live)
+        rows(I0{}, I2{});
       else
-        march_rows<S, C, NCH, SCHED, ORDER, DM, 0, false, PFMAX>(
-            a, tg, ring, dmax);
+        rows(I0{}, I1{});
     }
   } else {
-    march_rows<S, C, NCH, SCHED, ORDER, DM, 0, SCHED == SCHED_GS_LEX, PFMAX>(
-        a, tg, ring, dmax);
+    march_rows<S, C, NCH, SCHED, ORDER, DM, 0,
+               SCHED == SCHED_GS_LEX ? 1 : 0, PFMAX>(a, tg, ring, dmax);
   }
 }
 
