@@ -611,6 +611,21 @@ __device__ __forceinline__ void march_task(const MarchArgs& a, int stripe,
       }
       return;
     }
+    // The same holds row by row: rows below `front` hold no computed pixel
+    // the band reaches during the launch (and once t0 >= S both buffers
+    // hold their first values), rows up to `back` hold only written pixels
+    // last updated before the previous launch (both buffers hold their
+    // final values).  Only the rows between are marched (with S halo rows
+    // beyond each end, as for any task), which shortens the tasks at the
+    // band's edges.  The range is not empty: the task is neither before
+    // nor after the band.
+    const int front = a.t0 >= S ? t_last - max(tg.cu, 0) : tg.r1 - 1;
+    const int back = a.t0 - S - a.band - c_hi;
+    const int r0 = max(tg.r0, back + 1), r1 = min(tg.r1, front + 1);
+    if (r0 < r1) {
+      tg.r0 = r0;
+      tg.r1 = r1;
+    }
   }
 
   // one ring for whichever variant of the rows runs (WAVES waves a group)
