@@ -102,13 +102,13 @@ int alloc_all(pie_unet* u) {
   const long long P0 = u->P[0], P1 = u->P[1], P2 = u->P[2], P3 = u->P[3];
   PIE_HIP(u->in.alloc(P0 * 8));
   PIE_HIP(u->e0a.alloc(P0 * 32));
-  PIE_HIP(u->c0.alloc(P0 * 96));
+  PIE_HIP(u->c0.alloc(P0 * 32));
   PIE_HIP(u->p0.alloc(P1 * 32));
   PIE_HIP(u->e1a.alloc(P1 * 64));
-  PIE_HIP(u->c1.alloc(P1 * 192));
+  PIE_HIP(u->c1.alloc(P1 * 64));
   PIE_HIP(u->p1.alloc(P2 * 64));
   PIE_HIP(u->e2a.alloc(P2 * 128));
-  PIE_HIP(u->c2.alloc(P2 * 384));
+  PIE_HIP(u->c2.alloc(P2 * 128));
   PIE_HIP(u->p2.alloc(P3 * 128));
   PIE_HIP(u->bna.alloc(P3 * 256));
   PIE_HIP(u->bnb.alloc(P3 * 256));
@@ -153,15 +153,16 @@ double conv_flops(const pie_unet* u, bool backward) {
   return f;
 }
 
-// Forward of conv l (bias, ReLU) into the bf16 slice y; with pool / upf,
-// also y's 2 x 2 average pool / 2x nearest upsample.
+// Forward of conv l (bias, ReLU) into the bf16 slice y; with pool, also
+// y's 2 x 2 average pool; with xu, x's first xu->ch channels are the 2x
+// nearest upsample of xu (read at its resolution).
 int conv_fwd(pie_unet* u, int l, const Slice& x, const Slice& y,
-             const Slice* pool = nullptr, const Slice* upf = nullptr) {
+             const Slice* pool = nullptr, const Slice* xu = nullptr) {
   const ConvSpec& c = kConvs[l];
   const int s = 1 << c.level;
   pie::unet::ConvFolds f;
   f.pool = pool;
-  f.upf = upf;
+  f.in_up = xu;
   PIE_HIP(pie::unet::conv3x3(x, u->B, u->H / s, u->W / s, u->wk[l].p, c.cout,
                              u->param.p + u->boff[l], true, &y, nullptr, 0,
                              false, u->stream, f));
@@ -175,31 +176,34 @@ int forward(pie_unet* u, const float* x) {
   PIE_HIP(pie::unet::conv3x3_in8(x, u->in.p, u->B, u->H, u->W, u->wk[E0A].p,
                                  kConvs[E0A].cout, u->param.p + u->boff[E0A],
                                  true, sl(u->e0a, 32, 0, 32), st));
-  // encoder outputs go to the skip half of the concatenation buffers, with
-  // their average pool for the next level; decoder level outputs to their
-  // own buffers, with their upsample into the finer level's concatenation
+  // encoder outputs go to the skip buffers c0, c1, c2, with their average
+  // pool for the next level.  A decoder level's first convolution reads
+  // the concatenation (upsample of the coarser output) || skip: channels
+  // below the upsample's width from the coarser output at its own
+  // resolution, the rest from the skip buffer (a slice whose offset starts
+  // the skip channels at its channel 0)
   const Slice p0 = sl(u->p0, 32, 0, 32), p1 = sl(u->p1, 64, 0, 64),
               p2 = sl(u->p2, 128, 0, 128);
-  PIE_CHECK(conv_fwd(u, E0B, sl(u->e0a, 32, 0, 32), sl(u->c0, 96, 64, 32),
+  PIE_CHECK(conv_fwd(u, E0B, sl(u->e0a, 32, 0, 32), sl(u->c0, 32, 0, 32),
                      &p0));
   PIE_CHECK(conv_fwd(u, E1A, p0, sl(u->e1a, 64, 0, 64)));
-  PIE_CHECK(conv_fwd(u, E1B, sl(u->e1a, 64, 0, 64), sl(u->c1, 192, 128, 64),
+  PIE_CHECK(conv_fwd(u, E1B, sl(u->e1a, 64, 0, 64), sl(u->c1, 64, 0, 64),
                      &p1));
   PIE_CHECK(conv_fwd(u, E2A, p1, sl(u->e2a, 128, 0, 128)));
-  PIE_CHECK(conv_fwd(u, E2B, sl(u->e2a, 128, 0, 128),
-                     sl(u->c2, 384, 256, 128), &p2));
+  PIE_CHECK(conv_fwd(u, E2B, sl(u->e2a, 128, 0, 128), sl(u->c2, 128, 0, 128),
+                     &p2));
   PIE_CHECK(conv_fwd(u, BNA, p2, sl(u->bna, 256, 0, 256)));
-  const Slice u2 = sl(u->c2, 384, 0, 256), u1 = sl(u->c1, 192, 0, 128),
-              u0 = sl(u->c0, 96, 0, 64);
-  PIE_CHECK(conv_fwd(u, BNB, sl(u->bna, 256, 0, 256), sl(u->bnb, 256, 0, 256),
-                     nullptr, &u2));
-  PIE_CHECK(conv_fwd(u, D2A, sl(u->c2, 384, 0, 384), sl(u->d2a, 128, 0, 128)));
-  PIE_CHECK(conv_fwd(u, D2B, sl(u->d2a, 128, 0, 128), sl(u->d2b, 128, 0, 128),
-                     nullptr, &u1));
-  PIE_CHECK(conv_fwd(u, D1A, sl(u->c1, 192, 0, 192), sl(u->d1a, 64, 0, 64)));
-  PIE_CHECK(conv_fwd(u, D1B, sl(u->d1a, 64, 0, 64), sl(u->d1b, 64, 0, 64),
-                     nullptr, &u0));
-  PIE_CHECK(conv_fwd(u, D0A, sl(u->c0, 96, 0, 96), sl(u->d0a, 32, 0, 32)));
+  const Slice bnb = sl(u->bnb, 256, 0, 256), d2b = sl(u->d2b, 128, 0, 128),
+              d1b = sl(u->d1b, 64, 0, 64);
+  PIE_CHECK(conv_fwd(u, BNB, sl(u->bna, 256, 0, 256), bnb));
+  PIE_CHECK(conv_fwd(u, D2A, sl(u->c2, 128, -256, 384),
+                     sl(u->d2a, 128, 0, 128), nullptr, &bnb));
+  PIE_CHECK(conv_fwd(u, D2B, sl(u->d2a, 128, 0, 128), d2b));
+  PIE_CHECK(conv_fwd(u, D1A, sl(u->c1, 64, -128, 192), sl(u->d1a, 64, 0, 64),
+                     nullptr, &d2b));
+  PIE_CHECK(conv_fwd(u, D1B, sl(u->d1a, 64, 0, 64), d1b));
+  PIE_CHECK(conv_fwd(u, D0A, sl(u->c0, 32, -64, 96), sl(u->d0a, 32, 0, 32),
+                     nullptr, &d1b));
   PIE_CHECK(conv_fwd(u, D0B, sl(u->d0a, 32, 0, 32), sl(u->d0b, 32, 0, 32)));
   const long long hw = u->woff[kNumConv];  // head weights
   PIE_HIP(pie::unet::head_fwd(sl(u->d0b, 32, 0, 32), (int)u->P[0],
@@ -221,7 +225,8 @@ int forward(pie_unet* u, const float* x) {
 int conv_pair_bwd(pie_unet* u, int la, int lb, const Slice& xb,
                   const Slice& xa, bf16_t* dx,
                   const pie::unet::UpSum* up = nullptr,
-                  const pie::unet::PoolBwd* pb = nullptr) {
+                  const pie::unet::PoolBwd* pb = nullptr,
+                  const Slice* xa_up = nullptr) {
   const ConvSpec& ca = kConvs[la];
   const ConvSpec& cb = kConvs[lb];
   const int s = 1 << cb.level;
@@ -238,7 +243,7 @@ int conv_pair_bwd(pie_unet* u, int la, int lb, const Slice& xb,
                              false, &dya, nullptr, 0, false, st, fb));
   PIE_HIP(pie::unet::conv3x3_wgrad(xa, u->B, h, w, u->dyb2.p, ca.cout,
                                    u->grad.p + u->woff[la],
-                                   u->grad.p + u->boff[la], st));
+                                   u->grad.p + u->boff[la], st, xa_up));
   if (up || pb) {
     pie::unet::ConvFolds fa;
     fa.up = up;
@@ -264,28 +269,31 @@ int backward(pie_unet* u) {
                               u->grad.p + hw + 96, dy, st));
   // the decoder's first convolutions take an upsampled input: their data
   // gradient's epilogue makes the coarser level's output gradient
-  const pie::unet::UpSum up0{dy, 64, sl(u->d1b, 64, 0, 64)};
+  const Slice bnb = sl(u->bnb, 256, 0, 256), d2b = sl(u->d2b, 128, 0, 128),
+              d1b = sl(u->d1b, 64, 0, 64);
+  const pie::unet::UpSum up0{dy, 64, d1b};
   PIE_CHECK(conv_pair_bwd(u, D0A, D0B, sl(u->d0a, 32, 0, 32),
-                          sl(u->c0, 96, 0, 96), u->gC0.p, &up0));
-  const pie::unet::UpSum up1{dy, 128, sl(u->d2b, 128, 0, 128)};
+                          sl(u->c0, 32, -64, 96), u->gC0.p, &up0, nullptr,
+                          &d1b));
+  const pie::unet::UpSum up1{dy, 128, d2b};
   PIE_CHECK(conv_pair_bwd(u, D1A, D1B, sl(u->d1a, 64, 0, 64),
-                          sl(u->c1, 192, 0, 192), u->gC1.p, &up1));
-  const pie::unet::UpSum up2{dy, 256, sl(u->bnb, 256, 0, 256)};
+                          sl(u->c1, 64, -128, 192), u->gC1.p, &up1, nullptr,
+                          &d2b));
+  const pie::unet::UpSum up2{dy, 256, bnb};
   PIE_CHECK(conv_pair_bwd(u, D2A, D2B, sl(u->d2a, 128, 0, 128),
-                          sl(u->c2, 384, 0, 384), u->gC2.p, &up2));
+                          sl(u->c2, 128, -256, 384), u->gC2.p, &up2, nullptr,
+                          &bnb));
   // encoder: the first convolutions take an average-pooled input: their
   // data gradient's epilogue adds it (a quarter to each of the four pixels)
   // to the skip channels' gradient from the decoder and applies the ReLU of
   // the finer level's output
-  const pie::unet::PoolBwd pb2{dy, u->gC2.p, 128, 0,
-                               sl(u->c2, 384, 256, 128)};
+  const pie::unet::PoolBwd pb2{dy, u->gC2.p, 128, 0, sl(u->c2, 128, 0, 128)};
   PIE_CHECK(conv_pair_bwd(u, BNA, BNB, sl(u->bna, 256, 0, 256),
                           sl(u->p2, 128, 0, 128), nullptr, nullptr, &pb2));
-  const pie::unet::PoolBwd pb1{dy, u->gC1.p, 64, 0,
-                               sl(u->c1, 192, 128, 64)};
+  const pie::unet::PoolBwd pb1{dy, u->gC1.p, 64, 0, sl(u->c1, 64, 0, 64)};
   PIE_CHECK(conv_pair_bwd(u, E2A, E2B, sl(u->e2a, 128, 0, 128),
                           sl(u->p1, 64, 0, 64), nullptr, nullptr, &pb1));
-  const pie::unet::PoolBwd pb0{dy, u->gC0.p, 32, 0, sl(u->c0, 96, 64, 32)};
+  const pie::unet::PoolBwd pb0{dy, u->gC0.p, 32, 0, sl(u->c0, 32, 0, 32)};
   PIE_CHECK(conv_pair_bwd(u, E1A, E1B, sl(u->e1a, 64, 0, 64),
                           sl(u->p0, 32, 0, 32), nullptr, nullptr, &pb0));
   PIE_CHECK(conv_pair_bwd(u, E0A, E0B, sl(u->e0a, 32, 0, 32),

@@ -63,6 +63,11 @@ struct PoolBwd {
   Slice y;
 };
 struct ConvFolds {
+  // not a fold: the input's first in_up->ch channels (a multiple of 32) are
+  // the nearest 2x upsample of in_up, read at its half resolution (pixel
+  // (y, x) of image n <- in_up's (n, y / 2, x / 2)); in's slice offset may
+  // then be negative, as long as the channels it addresses are >= in_up->ch
+  const Slice* in_up = nullptr;
   const Slice* mask = nullptr;
   const UpSum* up = nullptr;
   const Slice* pool = nullptr;
@@ -80,10 +85,11 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
 
 // dW[tap][ci][co] += sum_p x[p + offset(tap)][ci] * dy[p][co] (fp32
 // atomics) and db[co] += sum_p dy[p][co]; x is a bf16 slice, dy bf16 with
-// Cout contiguous channels.
+// Cout contiguous channels; with xu, x's first xu->ch channels are read at
+// half resolution from xu, as ConvFolds::in_up (Cout a multiple of 32).
 hipError_t conv3x3_wgrad(const Slice& x, int n, int h, int w,
                          const bf16_t* dy, int cout, float* dw, float* db,
-                         hipStream_t st);
+                         hipStream_t st, const Slice* xu = nullptr);
 
 // Pack fp32 master weights w[tap][ci][co] into the forward operand
 // wk[kb][co][8] (kb = tap*(ci/8)+cb) and the data-gradient operand

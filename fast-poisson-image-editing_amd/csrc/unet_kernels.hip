@@ -118,6 +118,11 @@ struct ConvArgs {
   // its bf16 rounding written to in_copy [P][8] when in_copy is not null
   const float* in_f32;
   bf16_t* in_copy;
+  // input channels [0, inu_ch) read at half resolution from inu (the
+  // nearest 2x upsample of a coarser level, not materialised): pixel (y, x)
+  // of image n takes inu's pixel (n, y / 2, x / 2) (LDS-tiled kernel)
+  const bf16_t* inu;
+  int inu_stride, inu_off, inu_ch;
 };
 
 // Wave tile: MT x 16 pixels by NT x 16 output channels; a block is four
@@ -219,12 +224,23 @@ __global__ void __launch_bounds__(kBlock) conv3x3_kernel(ConvArgs a) {
 // applied when the registers are stored to LDS, so that the compiler keeps
 // the loads unconditional and in flight together).
 // The window part of conv_stage_load.
-template <int WH, int WW>
+template <int WH, int WW, bool UPIN>
 __device__ __forceinline__ uint32_t conv_win_load(
     const ConvArgs& a, long long img_base, int row0, int col0, int c32,
     u32x4 (&wv)[(WH * WW * 4 + kBlock - 1) / kBlock]) {
   constexpr int kWinJobs = WH * WW * 4, kWinIt = (kWinJobs + kBlock - 1) / kBlock;
   uint32_t ok = 0;
+  // the stage's source (wave-uniform): the half-resolution one for
+  // channels below inu_ch; the image's base folded into the pointer, so
+  // that a load needs only a 32-bit pixel offset
+  const bool half = UPIN && c32 * 32 < a.inu_ch;
+  const int stride = half ? a.inu_stride : a.in_stride;
+  const int sh = half ? 1 : 0;
+  const int wr = a.w >> sh;  // row pitch of the source in pixels
+  const bf16_t* src =
+      (half ? a.inu + a.inu_off + (img_base >> 2) * stride
+            : a.in + a.in_off + img_base * stride) +
+      c32 * 32;
 #pragma unroll
   for (int k = 0; k < kWinIt; ++k) {
     const int job = threadIdx.x + k * kBlock;
@@ -233,9 +249,9 @@ __device__ __forceinline__ uint32_t conv_win_load(
     const int y = row0 - 1 + pix / WW, x = col0 - 1 + pix % WW;
     if (y >= 0 && y < a.h && x >= 0 && x < a.w) ok |= 1u << k;
     const int yc = min(max(y, 0), a.h - 1), xc = min(max(x, 0), a.w - 1);
-    wv[k] = *reinterpret_cast<const u32x4*>(
-        a.in + (img_base + (long long)yc * a.w + xc) * a.in_stride +
-        a.in_off + c32 * 32 + gg * 8);
+    const int p = (yc >> sh) * wr + (xc >> sh);
+    wv[k] = *reinterpret_cast<const u32x4*>(src + (long long)p * stride +
+                                            gg * 8);
   }
   return ok;
 }
@@ -254,14 +270,15 @@ __device__ __forceinline__ void conv_win_store(
   }
 }
 
-template <int WH, int WW, int CT>
+template <int WH, int WW, int CT, bool UPIN>
 __device__ __forceinline__ uint32_t conv_stage_load(
     const ConvArgs& a, long long img_base, int row0, int col0, int co0,
     int c32, u32x4 (&wv)[(WH * WW * 4 + kBlock - 1) / kBlock],
     u32x4 (&wt)[(9 * 4 * CT + kBlock - 1) / kBlock]) {
   constexpr int kWtJobs = 9 * 4 * CT, kWtIt = (kWtJobs + kBlock - 1) / kBlock;
   const int CB = a.cin / 8;
-  const uint32_t ok = conv_win_load<WH, WW>(a, img_base, row0, col0, c32, wv);
+  const uint32_t ok =
+      conv_win_load<WH, WW, UPIN>(a, img_base, row0, col0, c32, wv);
 #pragma unroll
   for (int k = 0; k < kWtIt; ++k) {
     const int jj = min((int)threadIdx.x + k * kBlock, kWtJobs - 1);
@@ -512,7 +529,8 @@ __device__ __forceinline__ void conv_epilogue(
   }
 }
 
-template <int MT, int NT>
+// UPIN: the input's first inu_ch channels are read at half resolution
+template <int MT, int NT, bool UPIN>
 __global__ void __launch_bounds__(kBlock) conv3x3_lds_kernel(ConvArgs a) {
   constexpr int TH = 4 * MT, TW = 16, WH = TH + 2, WW = TW + 2;
   constexpr int CT = 16 * NT;
@@ -549,13 +567,13 @@ __global__ void __launch_bounds__(kBlock) conv3x3_lds_kernel(ConvArgs a) {
     for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 
   const int nc32 = a.cin / 32;
-  uint32_t ok = conv_stage_load<WH, WW, CT>(a, img_base, row0, col0, co0, 0,
+  uint32_t ok = conv_stage_load<WH, WW, CT, UPIN>(a, img_base, row0, col0, co0, 0,
                                             wv, wt);
   for (int c32 = 0; c32 < nc32; ++c32) {
     conv_stage_store<WH, WW, CT>(wv, wt, ok, win, ws);
     __syncthreads();
     if (c32 + 1 < nc32)
-      ok = conv_stage_load<WH, WW, CT>(a, img_base, row0, col0, co0, c32 + 1,
+      ok = conv_stage_load<WH, WW, CT, UPIN>(a, img_base, row0, col0, co0, c32 + 1,
                                        wv, wt);
     conv_taps<MT, NT>(acc, &win[0][0][0], &ws[0][0][0], wave, g, i);
     __syncthreads();
@@ -829,13 +847,14 @@ __device__ __forceinline__ bf16x8 tr_pair(const bf16_t* lo, const bf16_t* hi) {
   return __builtin_bit_cast(bf16x8, c);
 }
 
-template <int CO, int kWtH>
+template <int CO, int kWtH, bool UPIN>
 __global__ void __launch_bounds__(kBlock)
     wgrad_lds_kernel(const bf16_t* __restrict__ x, int x_stride, int x_off,
                      int cin, int n, int h, int w,
                      const bf16_t* __restrict__ dy, int cout,
                      float* __restrict__ dw, float* __restrict__ db,
-                     int tiles_per_block) {
+                     int tiles_per_block, const bf16_t* __restrict__ xu,
+                     int xu_stride, int xu_off, int xu_ch) {
   constexpr int kWtWH = kWtH + 2, kWtWW = kWtW + 2;  // input window
   constexpr int NBW = CO / 32;                      // output blocks per wave
   constexpr int DG = CO / 8;                        // gradient channel groups
@@ -856,6 +875,9 @@ __global__ void __launch_bounds__(kBlock)
   int grp, mem;
   xcd_group(blockIdx.x, nci * (cout / CO), grp, mem);
   const int ci0 = (mem % nci) * 32, co0 = (mem / nci) * CO;
+  // the block's input channels come from the half-resolution source
+  // (pixel (y, x) <- (y / 2, x / 2)) when below xu_ch
+  const bool half = UPIN && ci0 < xu_ch;
   const int tiles_x = (w + kWtW - 1) / kWtW, tiles_y = (h + kWtH - 1) / kWtH;
   const long long ntiles = (long long)n * tiles_x * tiles_y;
   const long long t_begin = (long long)grp * tiles_per_block;
@@ -877,6 +899,13 @@ __global__ void __launch_bounds__(kBlock)
     const int row0 = (trem / tiles_x) * kWtH, col0 = (trem % tiles_x) * kWtW;
     const long long ib = (long long)img * h;
     okx = okd = 0;
+    // the image's base in the (wave-uniform) source pointer, 32-bit pixel
+    // offsets per load
+    const int sh = half ? 1 : 0, wr = w >> sh;
+    const int xs_stride = half ? xu_stride : x_stride;
+    const bf16_t* xsrc =
+        half ? xu + xu_off + (long long)img * (h >> 1) * wr * xu_stride
+             : x + x_off + ib * w * x_stride;
 #pragma unroll
     for (int k = 0; k < XIT; ++k) {
       const int job = min(tid + k * kBlock, XJ - 1);
@@ -887,8 +916,9 @@ __global__ void __launch_bounds__(kBlock)
           xx < w)
         okx |= 1u << k;
       const int yc = min(max(y, 0), h - 1), xc = min(max(xx, 0), w - 1);
+      const int q = (yc >> sh) * wr + (xc >> sh);
       xv[k] = *reinterpret_cast<const u32x4*>(
-          x + ((ib + yc) * w + xc) * x_stride + x_off + min(ch, cin - 8));
+          xsrc + (long long)q * xs_stride + min(ch, cin - 8));
     }
 #pragma unroll
     for (int k = 0; k < DIT; ++k) {
@@ -1185,8 +1215,14 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
   const Slice* pool_out = folds.pool;
   const Slice* upf_out = folds.upf;
   const PoolBwd* pb = folds.pool_bwd;
+  const Slice* inu = folds.in_up;
   if (in.ch % 8 || cout % 16 || in.stride % 8 || in.off % 8)
     return hipErrorInvalidValue;
+  if (inu && (!inu->base || inu->ch <= 0 || inu->ch % 32 || inu->ch > in.ch ||
+              inu->stride % 8 || inu->off % 8 || h % 2 || w % 2 ||
+              in.ch % 32 || cout % 32))
+    return hipErrorInvalidValue;
+  if (in.off < 0 && (!inu || in.off + inu->ch < 0)) return hipErrorInvalidValue;
   if (pb && (!pb->out || !pb->gs || pb->y.ch != cout || pb->y.stride % 8 ||
              pb->y.off % 8 || pb->gs_stride % 8 || pb->gs_off % 8 ||
              pb->gs_off + cout > pb->gs_stride || out_bf || out_f32 || up ||
@@ -1244,6 +1280,10 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
   a.pb_y = pb ? static_cast<const bf16_t*>(pb->y.base) : nullptr;
   a.pb_y_stride = pb ? pb->y.stride : 0;
   a.pb_y_off = pb ? pb->y.off : 0;
+  a.inu = inu ? static_cast<const bf16_t*>(inu->base) : nullptr;
+  a.inu_stride = inu ? inu->stride : 0;
+  a.inu_off = inu ? inu->off : 0;
+  a.inu_ch = inu ? inu->ch : 0;
   const long long P = (long long)n * h * w;
   const bool vec_out = (!out_bf || (out_bf->stride % 8 == 0 &&
                                     out_bf->off % 8 == 0)) &&
@@ -1261,20 +1301,35 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
       // (when they still give two blocks a CU: small inputs, such as one
       // 256 x 256 image, keep the 8 x 16 tiles, whose extra blocks hide
       // more latency)
-      hipLaunchKernelGGL((conv3x3_lds_kernel<4, 4>),
-                         dim3(xcd_grid(tiles4, cout / 64)), dim3(kBlock), 0,
-                         st, a);
-    } else if (cout % 64 == 0)
-      hipLaunchKernelGGL((conv3x3_lds_kernel<LMT, 4>),
-                         dim3(xcd_grid(tiles, cout / 64)), dim3(kBlock), 0, st,
-                         a);
-    else
-      hipLaunchKernelGGL((conv3x3_lds_kernel<LMT, 2>),
+      if (inu)
+        hipLaunchKernelGGL((conv3x3_lds_kernel<4, 4, true>),
+                           dim3(xcd_grid(tiles4, cout / 64)), dim3(kBlock), 0,
+                           st, a);
+      else
+        hipLaunchKernelGGL((conv3x3_lds_kernel<4, 4, false>),
+                           dim3(xcd_grid(tiles4, cout / 64)), dim3(kBlock), 0,
+                           st, a);
+    } else if (cout % 64 == 0) {
+      if (inu)
+        hipLaunchKernelGGL((conv3x3_lds_kernel<LMT, 4, true>),
+                           dim3(xcd_grid(tiles, cout / 64)), dim3(kBlock), 0,
+                           st, a);
+      else
+        hipLaunchKernelGGL((conv3x3_lds_kernel<LMT, 4, false>),
+                           dim3(xcd_grid(tiles, cout / 64)), dim3(kBlock), 0,
+                           st, a);
+    } else if (inu) {
+      hipLaunchKernelGGL((conv3x3_lds_kernel<LMT, 2, true>),
                          dim3(xcd_grid(tiles, cout / 32)), dim3(kBlock), 0, st,
                          a);
+    } else {
+      hipLaunchKernelGGL((conv3x3_lds_kernel<LMT, 2, false>),
+                         dim3(xcd_grid(tiles, cout / 32)), dim3(kBlock), 0, st,
+                         a);
+    }
     return hipGetLastError();
   }
-  if (mask || up || pool_out || upf_out || pb)
+  if (mask || up || pool_out || upf_out || pb || inu)
     return hipErrorInvalidValue;  // unaligned output slice
   constexpr int MT = 2;
   if (cout % 32 == 0) {
@@ -1290,9 +1345,16 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
 
 hipError_t conv3x3_wgrad(const Slice& x, int n, int h, int w,
                          const bf16_t* dy, int cout, float* dw, float* db,
-                         hipStream_t st) {
+                         hipStream_t st, const Slice* xu) {
   if (x.ch % 8 || cout % 8 || x.stride % 8 || x.off % 8)
     return hipErrorInvalidValue;
+  if (xu && (xu->ch <= 0 || xu->ch % 32 || xu->ch > x.ch || xu->stride % 8 ||
+             xu->off % 8 || h % 2 || w % 2 || cout % 32))
+    return hipErrorInvalidValue;
+  if (x.off < 0 && (!xu || x.off + xu->ch < 0)) return hipErrorInvalidValue;
+  const bf16_t* xu_p = xu ? static_cast<const bf16_t*>(xu->base) : nullptr;
+  const int xu_stride = xu ? xu->stride : 0, xu_off = xu ? xu->off : 0;
+  const int xu_ch = xu ? xu->ch : 0;
   if (cout % 32 == 0) {
     constexpr int kWtH = 16;  // 16 x 16 tiles: 1.27x window halo, not 1.41x
     const long long ntiles = (long long)n * ((h + kWtH - 1) / kWtH) *
@@ -1304,14 +1366,18 @@ hipError_t conv3x3_wgrad(const Slice& x, int n, int h, int w,
     if (per < 1) per = 1;
     const int xb = (int)((ntiles + per - 1) / per);
     const dim3 grid(xcd_grid(xb, tiles));
+    auto launch = [&](auto kernel) {
+      hipLaunchKernelGGL(kernel, grid, dim3(kBlock), 0, st,
+                         static_cast<const bf16_t*>(x.base), x.stride, x.off,
+                         x.ch, n, h, w, dy, cout, dw, db, (int)per, xu_p,
+                         xu_stride, xu_off, xu_ch);
+    };
     if (co == 64)
-      hipLaunchKernelGGL((wgrad_lds_kernel<64, kWtH>), grid, dim3(kBlock), 0, st,
-                         static_cast<const bf16_t*>(x.base), x.stride, x.off,
-                         x.ch, n, h, w, dy, cout, dw, db, (int)per);
+      launch(xu ? wgrad_lds_kernel<64, kWtH, true>
+                : wgrad_lds_kernel<64, kWtH, false>);
     else
-      hipLaunchKernelGGL((wgrad_lds_kernel<32, kWtH>), grid, dim3(kBlock), 0, st,
-                         static_cast<const bf16_t*>(x.base), x.stride, x.off,
-                         x.ch, n, h, w, dy, cout, dw, db, (int)per);
+      launch(xu ? wgrad_lds_kernel<32, kWtH, true>
+                : wgrad_lds_kernel<32, kWtH, false>);
     return hipGetLastError();
   }
   const long long P = (long long)n * h * w;
