@@ -40,7 +40,8 @@ def tiles() -> list[tuple[int, int, int]]:
 
 def _compile(src: str, obj: str, defines: list[str]) -> str:
   # PIE_DEFINES: extra -D flags for tuning builds (e.g. the sweep kernel's
-  # PIE_PREFETCH / PIE_WAVES_PER_EU / PIE_GQ_LDS knobs in csrc/march.h)
+  # PIE_RING_MAX / PIE_WAVES_PER_EU in csrc/march.h, PIE_FINE_SHARE in
+  # csrc/solver.hip)
   defines = defines + os.environ.get("PIE_DEFINES", "").split()
   cmd = [HIPCC, *FLAGS, *defines, "-c", src, "-o", obj]
   r = subprocess.run(cmd, capture_output=True, text=True)

@@ -457,6 +457,19 @@ int autotune(pie_solver* s, int launches, int iters) {
   return PIE_OK;
 }
 
+// Pixels (i, j) of an H x W grid with i + j <= k.
+long long diag_count(long long k, long long H, long long W) {
+  auto tri = [](long long n) { return n > 0 ? n * (n + 1) / 2 : 0LL; };
+  return tri(k + 1) - tri(k - H + 1) - tri(k - W + 1) + tri(k - H - W + 1);
+}
+
+// gs_lex launches whose wavefront band covers less than this share of the
+// grid (the fill and the drain of a call) take a tile with a quarter of the
+// rows per task: such a launch lasts one task's march, not its work.
+#ifndef PIE_FINE_SHARE
+#define PIE_FINE_SHARE 0.25
+#endif
+
 int run_sweeps(pie_solver* s, int iters) {
   if (iters <= 0) return PIE_OK;
   if (!s->grid_layout) {  // EquSolver on an arbitrary A: one sweep a launch
@@ -486,16 +499,37 @@ int run_sweeps(pie_solver* s, int iters) {
   int ntask = 0;
   PIE_CHECK(task_list(s, c, &tasks, &ntask));
   const long long launches = (steps + c.S - 1) / c.S;
-  // issue the launches, or capture them into a graph: the pointers and
-  // step numbers of every launch follow from (iters, cur, tile)
+  // gs_lex: the same tile with a quarter of the rows per task, for the
+  // launches whose band covers little of the grid (the decomposition does
+  // not change the result: every launch advances every pixel S steps)
+  TileCfg cf = c;
+  const int* ftasks = nullptr;
+  int fntask = 0;
+  const bool fine = s->sched == PIE_GS_LEX && c.seg_rows >= 32;
+  if (fine) {
+    cf.seg_rows = (c.seg_rows + 3) / 4;
+    PIE_CHECK(task_list(s, cf, &ftasks, &fntask));
+  }
+  const double area = (double)s->H * s->W;
+  auto sparse = [&](int t0) {
+    // pixels the band covers at some step of the launch (i + j in
+    // (t0 - band - S, t0 + S)), as a share of the grid
+    const long long hi = diag_count(t0 + c.S - 1, s->H, s->W);
+    const long long lo = diag_count((long long)t0 - band - c.S, s->H, s->W);
+    return (double)(hi - lo) < PIE_FINE_SHARE * area;
+  };
+  // issue the launches, or capture them into a graph: the pointers, step
+  // numbers and tiles of every launch follow from (iters, cur, tile)
   auto issue = [&](int cur0) -> int {
     int cur = cur0;
     for (int t0 = 0; t0 < steps; t0 += c.S) {
       MarchArgs a = march_args(s, cur ? s->xb0() : s->xa0(),
                                cur ? s->xa0() : s->xb0(), t0, steps, band);
-      a.tasks = tasks;
-      a.ntask = ntask;
-      PIE_HIP(pie::launch_march(c, s->sched, s->eff_order(), a, s->stream));
+      const bool f = fine && sparse(t0);
+      a.tasks = f ? ftasks : tasks;
+      a.ntask = f ? fntask : ntask;
+      PIE_HIP(pie::launch_march(f ? cf : c, s->sched, s->eff_order(), a,
+                                s->stream));
       cur ^= 1;
     }
     return PIE_OK;
