@@ -179,3 +179,115 @@ def test_two_grid_cycle_converges():
   assert errs[2] < errs[1] / 2.5, errs
   assert errs[3] < errs[2] / 2.5, errs
   assert errs[3] < errs[0] / 15, errs
+
+
+# -- the learned coarse correction's flexible CG (coarse_oracle.net_start) --
+#
+# * a perfect predictor (the exact coarse error A_c^-1 r, A_c assembled
+#   independently as a Kronecker sum restricted to the solved cells) makes
+#   one step return the exact correction, and a second step (zero residual,
+#   zero direction) change nothing;
+# * predicting the residual itself gives one steepest-descent step,
+#   c = (r.r / r.A r) r;
+# * with arbitrary predictions the final residual is orthogonal to every
+#   direction taken (the Galerkin property of A-orthogonalised directions
+#   with exact line searches), which a wrong sign or a missed
+#   orthogonalisation breaks;
+# * the low-pass smoother halves an isolated cell per sweep and averages a
+#   pair as (v + v / 4) / 2.
+
+
+def _coarse_case(seed, hc=12, wc=14):
+  rng = np.random.default_rng(seed)
+  mc = np.zeros((hc, wc), bool)
+  mc[1:-1, 1:-1] = rng.random((hc - 2, wc - 2)) < 0.85
+  rc = rng.normal(0.0, 1.0, (hc, wc, 3)) * mc[..., None]
+  return rc, mc
+
+
+def _coarse_matrix(mc):
+  """A_c on the solved cells: 4 on the diagonal, -1 between 4-neighbours,
+  unsolved cells (and the outside) held at zero."""
+  hc, wc = mc.shape
+  def t(n):
+    return sp.diags([-np.ones(n - 1), 2 * np.ones(n), -np.ones(n - 1)],
+                    [-1, 0, 1])
+  L = (sp.kron(t(hc), sp.eye(wc)) + sp.kron(sp.eye(hc), t(wc))).tocsr()
+  idx = np.flatnonzero(mc.reshape(-1))
+  return L[idx][:, idx], idx
+
+
+def _apply(mc, p):
+  A, idx = _coarse_matrix(mc)
+  out = np.zeros(p.shape)
+  for c in range(3):
+    out.reshape(-1, 3)[idx, c] = A @ p.reshape(-1, 3)[idx, c]
+  return out
+
+
+def _solve(mc, r):
+  import scipy.sparse.linalg as spl
+  A, idx = _coarse_matrix(mc)
+  out = np.zeros(r.shape)
+  for c in range(3):
+    out.reshape(-1, 3)[idx, c] = spl.spsolve(A.tocsc(), r.reshape(-1, 3)[idx, c])
+  return out
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_net_start_with_a_perfect_predictor_is_exact(seed):
+  rc, mc = _coarse_case(seed)
+  exact = _solve(mc, rc)
+  one = co.net_start(rc, mc, lambda r, m: _solve(m, r), 1, 0)
+  np.testing.assert_allclose(one, exact, rtol=1e-6, atol=1e-6 * np.abs(exact).max())
+  two = co.net_start(rc, mc, lambda r, m: _solve(m, r), 2, 0)
+  np.testing.assert_allclose(two, exact, rtol=1e-6, atol=1e-6 * np.abs(exact).max())
+
+
+def test_net_start_steepest_descent_step():
+  rc, mc = _coarse_case(3)
+  got = co.net_start(rc, mc, lambda r, m: r, 1, 0)
+  ar = _apply(mc, rc)
+  a = np.einsum("ijc,ijc->c", rc, rc) / np.einsum("ijc,ijc->c", rc, ar)
+  np.testing.assert_allclose(got, (rc * a).astype(np.float32), rtol=1e-6,
+                             atol=1e-7)
+
+
+@pytest.mark.parametrize("steps,sweeps", [(2, 0), (3, 2), (4, 4)])
+def test_net_start_residual_orthogonal_to_its_directions(steps, sweeps):
+  rc, mc = _coarse_case(10 + steps)
+  rng = np.random.default_rng(steps)
+  taken = []
+  def predict(r, m):
+    z = rng.normal(0.0, 1.0, r.shape)
+    taken.append(z)
+    return z
+  c = co.net_start(rc, mc, predict, steps, sweeps).astype(np.float64)
+  assert len(taken) == steps
+  res = rc - _apply(mc, c)
+  for z in taken:
+    d = co.lowpass(mc, z, sweeps) * mc[..., None]
+    num = np.abs(np.einsum("ijc,ijc->c", res, d))
+    scale = np.linalg.norm(rc) * np.linalg.norm(d)
+    assert (num <= 1e-5 * scale).all(), (num, scale)
+  # and the correction lowers the energy of the error below the start's
+  e0 = _solve(mc, rc)
+  def energy(e):
+    return np.einsum("ijc,ijc->c", e, _apply(mc, e))
+  assert (energy(e0 - c) < energy(e0)).all()
+
+
+def test_lowpass_isolated_cell_and_pair():
+  mc = np.zeros((5, 6), bool)
+  mc[2, 2] = True
+  p = np.zeros((5, 6, 3))
+  p[2, 2] = [1.0, -2.0, 4.0]
+  for s in range(4):
+    np.testing.assert_array_equal(co.lowpass(mc, p, s)[2, 2],
+                                  p[2, 2] / 2.0 ** s)
+  mc[2, 3] = True
+  p[2, 3] = p[2, 2]
+  q = co.lowpass(mc, p, 1)
+  np.testing.assert_array_equal(q[2, 2], (p[2, 2] + p[2, 3] / 4) / 2)
+  np.testing.assert_array_equal(q[2, 3], q[2, 2])
+  assert (q[mc == 0] == 0).all()
