@@ -92,6 +92,15 @@ struct pie_solver {
     int count = 0;
   };
   std::map<std::array<int, 4>, std::unique_ptr<TaskList>> task_lists;
+  // gs_lex: the tasks with work ordered by their first anti-diagonal
+  // r0 + c_lo (the launches whose band covers little of the grid run one
+  // window of this order: the tasks that may have work there)
+  struct DiagList {
+    DevBuf<int> ids;
+    std::vector<int> d0;  // r0 + c_lo of each entry, ascending
+    int span = 0;         // largest (r1 - 1 + c_hi) - (r0 + c_lo)
+  };
+  std::map<std::array<int, 4>, std::unique_ptr<DiagList>> diag_lists;
   // per stripe width U: cum[stripe * (H + 1) + r] = rows < r of the stripe
   // that hold a solved pixel
   std::map<int, std::vector<int>> stripe_cum;
@@ -163,6 +172,7 @@ void free_problem(pie_solver* s) {
   s->cur = 0;
   s->tuned = false;
   s->task_lists.clear();
+  s->diag_lists.clear();
   s->stripe_cum.clear();
   drop_graph(s);
 }
@@ -308,6 +318,53 @@ int task_list(pie_solver* s, const TileCfg& c, const int** ids, int* count) {
   }
   *ids = it->second->ids.p;
   *count = it->second->count;
+  return PIE_OK;
+}
+
+// Tile c's tasks with work in the order of their first anti-diagonal
+// (cached per tile; see pie_solver::DiagList).
+int diag_list(pie_solver* s, const TileCfg& c,
+              const pie_solver::DiagList** out) {
+  const auto key = tile_key(c);
+  auto it = s->diag_lists.find(key);
+  if (it == s->diag_lists.end()) {
+    std::vector<int> flags;
+    PIE_CHECK(region_flags(s, c, &flags));
+    int stripes = 0, segs = 0;
+    pie::tile_tasks(c, s->H, s->W, &stripes, &segs);
+    const int U = 64 * c.C - 2 * c.S;  // written columns of a stripe
+    const int regions = (int)flags.size();
+    std::vector<std::pair<int, int>> order;  // (r0 + c_lo, task id)
+    std::unique_ptr<pie_solver::DiagList> dl(new pie_solver::DiagList());
+    for (int g = 0; g < pie::tile_groups(c); ++g)
+      for (int r = 0; r < regions; ++r) {
+        if (!flags[r]) continue;
+        const int seg = r / stripes, st = r % stripes;
+        const int r0 = seg * c.seg_rows, r1 = std::min(r0 + c.seg_rows, s->H);
+        const int c_lo = st * U, c_hi = std::min(c_lo + U, s->W) - 1;
+        order.push_back({r0 + c_lo, g * regions + r});
+        dl->span = std::max(dl->span, (r1 - 1 + c_hi) - (r0 + c_lo));
+      }
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::pair<int, int>& x,
+                        const std::pair<int, int>& y) {
+                       return x.first < y.first;
+                     });
+    std::vector<int> ids(order.size());
+    dl->d0.resize(order.size());
+    for (size_t k = 0; k < order.size(); ++k) {
+      dl->d0[k] = order[k].first;
+      ids[k] = order[k].second;
+    }
+    PIE_HIP(dl->ids.alloc(std::max<size_t>(ids.size(), 1)));
+    if (!ids.empty()) {
+      PIE_CHECK(upload(ids.data(), ids.size() * sizeof(int), dl->ids.p,
+                       s->stream));
+      PIE_HIP(hipStreamSynchronize(s->stream));  // ids is host memory
+    }
+    it = s->diag_lists.emplace(key, std::move(dl)).first;
+  }
+  *out = it->second.get();
   return PIE_OK;
 }
 
@@ -503,12 +560,11 @@ int run_sweeps(pie_solver* s, int iters) {
   // launches whose band covers little of the grid (the decomposition does
   // not change the result: every launch advances every pixel S steps)
   TileCfg cf = c;
-  const int* ftasks = nullptr;
-  int fntask = 0;
+  const pie_solver::DiagList* dl = nullptr;
   const bool fine = s->sched == PIE_GS_LEX && c.seg_rows >= 32;
   if (fine) {
     cf.seg_rows = (c.seg_rows + 3) / 4;
-    PIE_CHECK(task_list(s, cf, &ftasks, &fntask));
+    PIE_CHECK(diag_list(s, cf, &dl));
   }
   const double area = (double)s->H * s->W;
   auto sparse = [&](int t0) {
@@ -526,8 +582,25 @@ int run_sweeps(pie_solver* s, int iters) {
       MarchArgs a = march_args(s, cur ? s->xb0() : s->xa0(),
                                cur ? s->xa0() : s->xb0(), t0, steps, band);
       const bool f = fine && sparse(t0);
-      a.tasks = f ? ftasks : tasks;
-      a.ntask = f ? fntask : ntask;
+      a.tasks = tasks;
+      a.ntask = ntask;
+      if (f) {
+        // the window of the diagonal order holding every task that may have
+        // work: not before the band once t0 >= S (r0 + c_lo <= last step),
+        // not after it with both buffers agreeing ((r1 - 1) + c_hi >
+        // t0 - S - band, so r0 + c_lo > t0 - S - band - span)
+        const std::vector<int>& d0 = dl->d0;
+        const long long x = (long long)t0 - c.S - band - dl->span;
+        const int t_last = std::min(t0 + c.S, steps) - 1;
+        const size_t lo =
+            std::upper_bound(d0.begin(), d0.end(), x) - d0.begin();
+        const size_t hi =
+            t0 < c.S ? d0.size()
+                     : std::upper_bound(d0.begin(), d0.end(), t_last) -
+                           d0.begin();
+        a.tasks = dl->ids.p + lo;
+        a.ntask = hi > lo ? (int)(hi - lo) : 0;
+      }
       PIE_HIP(pie::launch_march(f ? cf : c, s->sched, s->eff_order(), a,
                                 s->stream));
       cur ^= 1;
