@@ -41,10 +41,9 @@ struct Slice {
 //   as bf16 [n][h/2][w/2][up->ch] to up->out (the upsampling backward with
 //   the backward of the ReLU that made its input); up->ch a multiple of 64,
 //   h and w even, no mask;
-// - pool / upf (Cout channels each; needs out_bf): the bf16 output's 2 x 2
-//   average pool (the four bf16 values summed in the order (0,0), (0,1),
-//   (1,0), (1,1), times 1/4, rounded; h and w even) and its 2x nearest
-//   upsample are written there as well;
+// - pool (Cout channels; needs out_bf): the bf16 output's 2 x 2 average
+//   pool (the four bf16 values summed in the order (0,0), (0,1), (1,0),
+//   (1,1), times 1/4, rounded; h and w even) is written there as well;
 // - pool_bwd (data gradient of a convolution whose input is the 2 x 2
 //   average pool of a skip connection; no out_bf, no out_f32): for each of
 //   the four pixels q of the double-resolution level under output pixel p,
@@ -71,7 +70,6 @@ struct ConvFolds {
   const Slice* mask = nullptr;
   const UpSum* up = nullptr;
   const Slice* pool = nullptr;
-  const Slice* upf = nullptr;
   const PoolBwd* pool_bwd = nullptr;
 };
 // Layers with 128+ input channels (and Cout a multiple of 64) use 16 x 16

@@ -100,12 +100,10 @@ struct ConvArgs {
   int up_ch;
   const bf16_t* up_mask;
   int up_mask_stride, up_mask_off;
-  // forward folds (LDS-tiled kernel, with out_bf): the bf16 output's 2 x 2
-  // average pool into pool_out, its 2x nearest upsample into upf_out
+  // forward fold (LDS-tiled kernel, with out_bf): the bf16 output's 2 x 2
+  // average pool into pool_out
   bf16_t* pool_out;
   int pool_stride, pool_off;
-  bf16_t* upf_out;
-  int upf_stride, upf_off;
   // pooling backward folded in (data gradient; no fp32 output): for the
   // four double-resolution pixels q under output pixel p, pb_out[q][co] =
   // bf16(pb_y[q][co] > 0 ? pb_gs[q][co] + v[p][co] / 4 : 0)
@@ -396,16 +394,6 @@ __device__ __forceinline__ void conv_epilogue(
         o[3] = (uint32_t)f2bf(v1[2]) | ((uint32_t)f2bf(v1[3]) << 16);
         *reinterpret_cast<u32x4*>(a.out_bf + p * a.out_stride + a.out_off +
                                   co0 + c8 * 8) = o;
-        if (a.upf_out) {
-          const int W2 = 2 * a.w;
-          const long long f =
-              ((long long)img * 2 * a.h + 2 * y) * W2 + 2 * x;
-          bf16_t* q = a.upf_out + a.upf_off + co0 + c8 * 8;
-          *reinterpret_cast<u32x4*>(q + f * a.upf_stride) = o;
-          *reinterpret_cast<u32x4*>(q + (f + 1) * a.upf_stride) = o;
-          *reinterpret_cast<u32x4*>(q + (f + W2) * a.upf_stride) = o;
-          *reinterpret_cast<u32x4*>(q + (f + W2 + 1) * a.upf_stride) = o;
-        }
       }
     }
   }
@@ -1213,7 +1201,6 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
   const Slice* mask = folds.mask;
   const UpSum* up = folds.up;
   const Slice* pool_out = folds.pool;
-  const Slice* upf_out = folds.upf;
   const PoolBwd* pb = folds.pool_bwd;
   const Slice* inu = folds.in_up;
   if (in.ch % 8 || cout % 16 || in.stride % 8 || in.off % 8)
@@ -1237,10 +1224,10 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
              cout % 32 || (up->ch < cout && !out_f32 == !out_bf) || mask ||
              accumulate))
     return hipErrorInvalidValue;
-  for (const Slice* f : {pool_out, upf_out})
-    if (f && (!out_bf || f->ch != cout || f->stride % 8 || f->off % 8 ||
-              in.ch % 32 || cout % 32 || (f == pool_out && (h % 2 || w % 2))))
-      return hipErrorInvalidValue;
+  if (pool_out && (!out_bf || pool_out->ch != cout || pool_out->stride % 8 ||
+                   pool_out->off % 8 || in.ch % 32 || cout % 32 || h % 2 ||
+                   w % 2))
+    return hipErrorInvalidValue;
   ConvArgs a{};
   a.in = static_cast<const bf16_t*>(in.base);
   a.in_stride = in.stride;
@@ -1270,9 +1257,6 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
   a.pool_out = pool_out ? static_cast<bf16_t*>(pool_out->base) : nullptr;
   a.pool_stride = pool_out ? pool_out->stride : 0;
   a.pool_off = pool_out ? pool_out->off : 0;
-  a.upf_out = upf_out ? static_cast<bf16_t*>(upf_out->base) : nullptr;
-  a.upf_stride = upf_out ? upf_out->stride : 0;
-  a.upf_off = upf_out ? upf_out->off : 0;
   a.pb_out = pb ? pb->out : nullptr;
   a.pb_gs = pb ? pb->gs : nullptr;
   a.pb_gs_stride = pb ? pb->gs_stride : 0;
@@ -1329,7 +1313,7 @@ hipError_t conv3x3(const Slice& in, int n, int h, int w, const bf16_t* wk,
     }
     return hipGetLastError();
   }
-  if (mask || up || pool_out || upf_out || pb || inu)
+  if (mask || up || pool_out || pb || inu)
     return hipErrorInvalidValue;  // unaligned output slice
   constexpr int MT = 2;
   if (cout % 32 == 0) {
